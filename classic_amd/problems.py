@@ -1,0 +1,72 @@
+"""Initial data for the benchmark and test problems (the ``qinit`` role of
+the reference's plug-in contract, src/2d/claw2ez.f:244-245).
+
+Arrays are returned in the reference layout q(meqn, 1-mbc:mx+mbc,
+1-mbc:my+mbc), i.e. C-contiguous numpy arrays of shape (my+2mbc, mx+2mbc,
+meqn); ghost cells are left at zero (bc2 fills them before every step).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def cell_centers(lower, upper, n, mbc=2):
+    d = (upper - lower) / n
+    idx = np.arange(1 - mbc, n + mbc + 1)
+    return lower + (idx - 0.5) * d, d
+
+
+def acoustics_pulse(mx=200, my=200, mbc=2, lower=(-1.0, -1.0), upper=(1.0, 1.0),
+                    width=0.35):
+    """2D acoustics, q = (p, u, v): p = 1 + cos(pi r / width) for r < width,
+    else p = 1, with r the distance of the cell centre from the origin;
+    u = v = 0.  (The literal formula jumps from 0 to 1 at r = width.)"""
+    xc, dx = cell_centers(lower[0], upper[0], mx, mbc)
+    yc, dy = cell_centers(lower[1], upper[1], my, mbc)
+    q = np.zeros((my + 2 * mbc, mx + 2 * mbc, 3))
+    X, Y = np.meshgrid(xc, yc)
+    r = np.sqrt(X * X + Y * Y)
+    p = np.where(r < width, 1.0 + np.cos(np.pi * r / width), 1.0)
+    q[..., 0] = p
+    q[:mbc, :, :] = 0.0
+    q[-mbc:, :, :] = 0.0
+    q[:, :mbc, :] = 0.0
+    q[:, -mbc:, :] = 0.0
+    return q, dx, dy
+
+
+def euler_quadrant_states(seed):
+    """Per-quadrant primitive states (rho, u, v, p) drawn from
+    numpy.random.default_rng(seed) in the fixed order NE, NW, SW, SE, each
+    as rho ~ U[0.1, 2], u ~ U[-1, 1], v ~ U[-1, 1], p ~ U[0.1, 2]."""
+    rng = np.random.default_rng(seed)
+    states = []
+    for _ in range(4):
+        rho = rng.uniform(0.1, 2.0)
+        u = rng.uniform(-1.0, 1.0)
+        v = rng.uniform(-1.0, 1.0)
+        p = rng.uniform(0.1, 2.0)
+        states.append((rho, u, v, p))
+    return states
+
+
+def euler_quadrants(mx=4096, my=4096, seed=0, gamma=1.4, mbc=2, split=(0.5, 0.5),
+                    lower=(0.0, 0.0), upper=(1.0, 1.0), states=None):
+    """2D Euler four-quadrant Riemann problem on [0,1]^2 split at (0.5, 0.5);
+    q = (rho, rho u, rho v, E) with E = p/(gamma-1) + rho (u^2+v^2)/2."""
+    xc, dx = cell_centers(lower[0], upper[0], mx, mbc)
+    yc, dy = cell_centers(lower[1], upper[1], my, mbc)
+    if states is None:
+        states = euler_quadrant_states(seed)
+    q = np.zeros((my + 2 * mbc, mx + 2 * mbc, 4))
+    east = xc >= split[0]
+    north = yc >= split[1]
+    for k, (rho, u, v, p) in enumerate(states):
+        ex = east if k in (0, 3) else ~east
+        ny = north if k in (0, 1) else ~north
+        mask = ny[:, None] & ex[None, :]
+        q[mask, 0] = rho
+        q[mask, 1] = rho * u
+        q[mask, 2] = rho * v
+        q[mask, 3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return q, dx, dy
