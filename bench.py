@@ -1,0 +1,154 @@
+"""Benchmark: 2D compressible Euler, 4096x4096 cells, 200 steps, FP64.
+
+Workload (the headline configuration): gamma = 1.4, Roe solver with the
+Harten-Hyman entropy fix, MC limiter on all 3 waves, unsplit with transverse
+corrections (method = 1,2,2), extrapolation BCs, four-quadrant initial data
+split at (0.5, 0.5) on [0,1]^2 drawn from numpy.random.default_rng(seed),
+CFL 0.9 with variable dt.  A "step" is one accepted time step of the claw2
+loop (bc2 + fused step kernel + Courant check) over the whole grid, with q
+resident in HBM; the timed region is the device-side claw2 loop of exactly
+--steps accepted steps.
+
+Prints one JSON line: value = Gcell-updates/s of the whole run; roofline =
+algorithmic HBM bytes (64 B per cell-update: read 4 fields of qold, write 4 of
+qnew) per step-kernel launch / its average duration from HIP events on the
+library's stream; cpu_baseline = the single-thread CPU oracle (same
+arithmetic) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s + %HBM peak; L1 err vs 4x-fine plain run at equal GPU time"
+WORKLOAD = "euler2d_4096x4096_roe_efix_mc_unsplit_transverse_200steps"
+BYTES_PER_CELL_UPDATE = 64  # (2*meqn + maux) * 8 with meqn = 4, maux = 0
+
+
+def profile_traffic():
+    """HBM bytes per step-kernel launch measured with rocprofv3 counters
+    (FETCH_SIZE + WRITE_SIZE, corrected by the copy-kernel calibration) as
+    recorded in profiles/; None until such a measurement exists."""
+    path = os.path.join(ROOT, "profiles", "traffic_euler4096.json")
+    if os.path.exists(path):
+        try:
+            return json.load(open(path)).get("bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def cpu_baseline(mx, my, seed, nsteps=1):
+    """Time the CPU oracle (single thread, -O2) on `nsteps` steps of the same
+    workload; returns (Gcell-updates/s, seconds, sample description)."""
+    try:
+        from oracle import oracle as O
+        from classic_amd import problems as P
+    except Exception as e:  # pragma: no cover
+        return None, 0.0, f"oracle unavailable: {e}"
+    try:
+        O.lib()
+    except Exception as e:
+        return None, 0.0, f"oracle unavailable: {e}"
+    q, dx, dy = P.euler_quadrants(mx, my, seed=seed)
+    p = O.Problem(4, 3, mx, my, O.SOLVER_EULER, O.solver_params(gamma=1.4, efix=True),
+                  method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4, 4], mthbc=(1, 1, 1, 1))
+    t0 = time.perf_counter()
+    r = O.claw2(p, q, dx, dy, 0.0, 1e9, [0.9 * dx / 3.0, 1e99], [1.0, 0.9], nsteps)
+    el = time.perf_counter() - t0
+    steps = int(r.nv[1])
+    return (mx * my * steps) / el / 1e9, el, f"{mx}x{my} Euler, {steps} step(s), 1 thread"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the single-grid step is not sharded across GPUs.", file=sys.stderr)
+        sys.exit(2)
+
+    from classic_amd import SOLVER_EULER, gpu
+    from classic_amd import problems as P
+
+    n = args.n
+    q0, dx, dy = P.euler_quadrants(n, n, seed=args.seed)
+    method, mthlim, mthbc = (1, 2, 2, 0, 0, 0, 0), [4, 4, 4], (1, 1, 1, 1)
+    g = gpu.Grid2D(SOLVER_EULER, n, n, [1.4, 1.0])
+    g.set_q(q0)
+    # first dt from the CFL-desired rule of the initial data: the retake
+    # logic corrects it on the first step if it is too large
+    dt0 = 0.9 * dx / 3.0
+    # warmup: untimed fixed-dt steps on a copy of the initial data
+    if args.warmup > 0:
+        g.time_steps(dx, dy, 0.25 * dx, method, mthlim, mthbc, args.warmup)
+    g.set_q(q0)
+
+    t0 = time.perf_counter()
+    out = g.claw2(dx, dy, 0.0, 1e30, [dt0, 1e99], [1.0, 0.9], args.steps, method, mthlim, mthbc)
+    wall = time.perf_counter() - t0
+    steps = int(out["nv"][1])
+    if steps != args.steps:
+        raise SystemExit(f"claw2 took {steps} steps, expected {args.steps} (info={out['info']})")
+    cells = n * n
+    value = cells * steps / wall / 1e9
+
+    # per-launch kernel time at the final dt (same data, continuing the run)
+    dt_last = float(out["dtv"][4])
+    ksteps = max(10, min(args.steps, 50))
+    total_ms, kern_ms, _ = g.time_steps(dx, dy, dt_last, method, mthlim, mthbc, ksteps)
+    kavg_s = kern_ms / ksteps / 1e3
+    achieved = cells * BYTES_PER_CELL_UPDATE / kavg_s / 1e9
+    g.close()
+
+    cpu = None
+    if not args.no_cpu:
+        v, el, sample = cpu_baseline(n, n, args.seed, args.cpu_steps)
+        if v is not None:
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+                   "sample": f"{sample} ({el:.1f} s), oracle C restatement compiled -O2"}
+
+    rec = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded four-quadrant Euler initial data)",
+        "config": {"workload": WORKLOAD, "mx": n, "my": n, "seed": args.seed,
+                   "method": list(method), "mthlim": mthlim, "mthbc": list(mthbc),
+                   "cfl_desired": 0.9, "cfl_max": 1.0},
+        "hbm_frac_of_peak": value * BYTES_PER_CELL_UPDATE / HBM_PEAK_GBS,
+        "kernel_ms_avg": kavg_s * 1e3,
+        "step_overhead_ms_avg": (total_ms - kern_ms) / ksteps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profile_traffic()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

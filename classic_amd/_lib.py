@@ -1,0 +1,80 @@
+"""ctypes binding of libclawgpu.so (the C-ABI declared in include/clawgpu.h).
+
+The library is built in-tree by ``python -m classic_amd.build``.  Loading
+fails loudly when it is missing: there is no CPU fallback in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libclawgpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "clawgpu.h")
+
+OK = 0
+ERR_ARG = -1
+ERR_UNSUPPORTED = -2
+ERR_HIP = -3
+ERR_NONFINITE = -4
+
+SOLVER_ACOUSTICS = 1
+SOLVER_EULER = 2
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_F = C.POINTER(C.c_float)
+_V = C.c_void_p
+_d = C.c_double
+_i = C.c_int
+
+_SIGS = {
+    "clawgpu_abi_version": ([], _i),
+    "clawgpu_last_error": ([], C.c_char_p),
+    "clawgpu_create": ([C.POINTER(_V), _i, _i, _i, _i, _i, _i, _i, _D, _i], _i),
+    "clawgpu_destroy": ([_V], _i),
+    "clawgpu_set_q": ([_V, _D], _i),
+    "clawgpu_get_q": ([_V, _D], _i),
+    "clawgpu_bc2": ([_V, _I], _i),
+    "clawgpu_step2": ([_V, _d, _d, _d, _I, _I, _i, _D], _i),
+    "clawgpu_claw2": ([_V, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
+    "clawgpu_set_segment_rows": ([_V, _i], _i),
+    "clawgpu_time_steps": ([_V, _d, _d, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
+}
+
+_lib = None
+
+
+class ClawGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"clawgpu error {code}: {msg}")
+        self.code = code
+
+
+def declared_symbols():
+    """Names of the functions declared in include/clawgpu.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(clawgpu_\w+)\s*\(", text, re.M)))
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m classic_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != OK:
+        msg = load().clawgpu_last_error()
+        raise ClawGpuError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,146 @@
+"""Python front end of the device step: a context per grid.
+
+``Grid2D`` mirrors the arguments of the reference's claw2/step2 (sizes,
+method(1:7), mthlim, mthbc) and keeps q resident on the GPU.  Host arrays use
+the reference layout q(meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc), i.e. C-contiguous
+numpy arrays of shape (my+2*mbc, mx+2*mbc, meqn).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SOLVER_ACOUSTICS, SOLVER_EULER, check
+
+SOLVER_SHAPE = {SOLVER_ACOUSTICS: (3, 2), SOLVER_EULER: (4, 3)}
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ivec(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} integers, got {a.size}")
+    return a
+
+
+def solver_params(solver_id, rho=1.0, bulk=4.0, gamma=1.4, efix=True):
+    if solver_id == SOLVER_ACOUSTICS:
+        return [float(rho), float(bulk)]
+    if solver_id == SOLVER_EULER:
+        return [float(gamma), 1.0 if efix else 0.0]
+    raise ValueError("unknown solver")
+
+
+class Grid2D:
+    """Device-resident single grid (one clawgpu context)."""
+
+    def __init__(self, solver_id, mx, my, params, mbc=2, maux=0):
+        self.lib = _lib.load()
+        meqn, mwaves = SOLVER_SHAPE[solver_id]
+        self.solver_id, self.mx, self.my, self.mbc = solver_id, mx, my, mbc
+        self.meqn, self.mwaves, self.maux = meqn, mwaves, maux
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        h = C.c_void_p()
+        check(self.lib.clawgpu_create(C.byref(h), meqn, mwaves, maux, mbc, mx, my, solver_id,
+                                      _dp(p), p.size))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clawgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.my + 2 * self.mbc, self.mx + 2 * self.mbc, self.meqn)
+
+    def set_q(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.shape != self.shape:
+            raise ValueError(f"q has shape {q.shape}, expected {self.shape}")
+        check(self.lib.clawgpu_set_q(self.h, _dp(q)))
+
+    def get_q(self):
+        q = np.empty(self.shape)
+        check(self.lib.clawgpu_get_q(self.h, _dp(q)))
+        return q
+
+    def bc2(self, mthbc):
+        check(self.lib.clawgpu_bc2(self.h, _ip(_ivec(mthbc, 4))))
+
+    def step2(self, dx, dy, dt, method, mthlim, use_fwave=False):
+        """One unsplit step on the device q (ghosts already set); returns cfl."""
+        cfl = C.c_double(0.0)
+        check(self.lib.clawgpu_step2(self.h, dx, dy, dt, _ip(_ivec(method, 7)),
+                                     _ip(_ivec(mthlim, self.mwaves)), int(bool(use_fwave)),
+                                     C.byref(cfl)))
+        return cfl.value
+
+    def claw2(self, dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc,
+              use_fwave=False):
+        """claw2 time loop on the device q; returns dict of the outputs."""
+        te = C.c_double(tend)
+        dtv_a = np.zeros(5)
+        dtv_a[: len(dtv)] = dtv
+        cflv_a = np.zeros(4)
+        cflv_a[: len(cflv)] = cflv
+        nv = np.array([nmax, 0], dtype=np.int32)
+        info = C.c_int(0)
+        m = _ivec(method, 7).copy()
+        m[6] = self.maux
+        rc = self.lib.clawgpu_claw2(self.h, dx, dy, tstart, C.byref(te), _dp(dtv_a), _dp(cflv_a),
+                                    _ip(nv), _ip(m), _ip(_ivec(mthlim, self.mwaves)),
+                                    _ip(_ivec(mthbc, 4)), int(bool(use_fwave)), C.byref(info))
+        check(rc)
+        return dict(info=info.value, tend=te.value, dtv=dtv_a, cflv=cflv_a, nv=nv)
+
+    def set_segment_rows(self, rows):
+        check(self.lib.clawgpu_set_segment_rows(self.h, int(rows)))
+
+    def time_steps(self, dx, dy, dt, method, mthlim, mthbc, nsteps, use_fwave=False):
+        """nsteps fixed-dt steps timed with HIP events on the library's
+        stream; returns (total_ms, step_kernel_ms, cfl_max)."""
+        ms = C.c_float(0.0)
+        kms = C.c_float(0.0)
+        cfl = C.c_double(0.0)
+        check(self.lib.clawgpu_time_steps(self.h, dx, dy, dt, _ip(_ivec(method, 7)),
+                                          _ip(_ivec(mthlim, self.mwaves)), _ip(_ivec(mthbc, 4)),
+                                          int(bool(use_fwave)), int(nsteps), C.byref(ms),
+                                          C.byref(kms), C.byref(cfl)))
+        return ms.value, kms.value, cfl.value
+
+
+def step2(solver_id, params, qold, dx, dy, dt, method, mthlim, use_fwave=False, mbc=2):
+    """One step with the reference step2 contract on host arrays: returns
+    (qnew, cfl).  qold must have its ghost cells set; ghost cells of qnew are
+    copied from qold (the reference leaves them partially updated)."""
+    qold = np.ascontiguousarray(qold, dtype=np.float64)
+    my2, mx2, meqn = qold.shape
+    with Grid2D(solver_id, mx2 - 2 * mbc, my2 - 2 * mbc, params, mbc=mbc) as g:
+        g.set_q(qold)
+        cfl = g.step2(dx, dy, dt, method, mthlim, use_fwave)
+        qn = g.get_q()
+    out = qold.copy()
+    out[mbc:-mbc, mbc:-mbc] = qn[mbc:-mbc, mbc:-mbc]
+    return out, cfl

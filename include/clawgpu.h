@@ -1,0 +1,96 @@
+/*
+ * clawgpu.h -- C-ABI of libclawgpu.so, the MI355X (gfx950) implementation of
+ * Clawpack Classic's single-grid wave-propagation step.
+ *
+ * Plain C types only: pointers, sizes, doubles and ints.  Host arrays use the
+ * reference layout q(meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc) (Fortran order, m
+ * fastest).  The library keeps q resident on the GPU between calls in a
+ * context; only the Courant number and frame copies cross to the host.
+ *
+ * Every entry point returns CLAWGPU_OK (0) or a negative error code and never
+ * aborts the process; clawgpu_last_error() describes the last failure of the
+ * calling thread.  Calls on one context must come from one host thread at a
+ * time; they are ordered on the context's own HIP stream.
+ *
+ * Riemann solvers are selected by id (the GPU cannot call the host rpn2/rpt2
+ * procedure pointers of the reference contract, src/2d/claw2.f:312-402):
+ *   CLAWGPU_SOLVER_ACOUSTICS  meqn 3, mwaves 2, params (rho, bulk)
+ *   CLAWGPU_SOLVER_EULER      meqn 4, mwaves 3, params (gamma, efix 0/1)
+ */
+#ifndef CLAWGPU_H
+#define CLAWGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLAWGPU_ABI_VERSION 1
+
+enum {
+    CLAWGPU_OK = 0,
+    CLAWGPU_ERR_ARG = -1,         /* invalid argument */
+    CLAWGPU_ERR_UNSUPPORTED = -2, /* valid for the reference, not implemented here */
+    CLAWGPU_ERR_HIP = -3,         /* HIP runtime error (no device, out of memory, ...) */
+    CLAWGPU_ERR_NONFINITE = -4    /* a wave speed was NaN: the step cannot be judged */
+};
+
+enum { CLAWGPU_SOLVER_ACOUSTICS = 1, CLAWGPU_SOLVER_EULER = 2 };
+
+typedef struct clawgpu_ctx clawgpu_ctx;
+
+/* Library identification and the last error message of this thread. */
+int clawgpu_abi_version(void);
+const char* clawgpu_last_error(void);
+
+/* Create / destroy a device context for one grid.  Replaces the allocation
+ * of q, aux and the work array in src/2d/claw2ez.f:173-263 (the library owns
+ * two device copies of q used as qold/qnew in turn, so copyq2,
+ * src/2d/copyq2.f:4-20, becomes a buffer swap).  maux must be 0 for the
+ * built-in solvers. */
+int clawgpu_create(clawgpu_ctx** ctx, int meqn, int mwaves, int maux, int mbc, int mx, int my,
+                   int solver_id, const double* params, int nparams);
+int clawgpu_destroy(clawgpu_ctx* ctx);
+
+/* Copy q (reference layout, ghost cells included) to / from the device. */
+int clawgpu_set_q(clawgpu_ctx* ctx, const double* q);
+int clawgpu_get_q(clawgpu_ctx* ctx, double* q);
+
+/* Fill the ghost cells of the device q: src/2d/bc2.f:4-215 with
+ * mthbc(1:4) = (left, right, bottom, top) in {1 extrap, 2 periodic, 3 wall}.
+ * mthbc = 0 (user BCs) is not supported on the device. */
+int clawgpu_bc2(clawgpu_ctx* ctx, const int* mthbc);
+
+/* One unsplit step, the contract of step2 (src/2d/step2.f90:2-251): the
+ * device q (ghost cells already set) is qold, the result replaces it, and
+ * *cfl receives the Courant number of the step.  method(1:7) and
+ * mthlim(1:mwaves) as in src/2d/claw2.f:164-270; method(3) in {0,1,2},
+ * method(6) = 0.  The scratch arguments of step2 (qadd, fadd, gadd, q1d,
+ * dtdx1d, dtdy1d, aux1-3, work) and the rpn2/rpt2 procedure arguments have
+ * no counterpart: scratch lives in registers, the solver is the context's. */
+int clawgpu_step2(clawgpu_ctx* ctx, double dx, double dy, double dt, const int* method,
+                  const int* mthlim, int use_fwave, double* cfl);
+
+/* The claw2 time loop (src/2d/claw2.f:5-8 signature, :494-784 semantics) on
+ * the device q: bc2, step, Courant check with retake on rejection, variable
+ * or fixed dt, and the outputs tend, dtv(3:5), cflv(3:4), nv(2) and info
+ * (0, 2, 3, 5, 6, 11, 12 as in :288-300).  info = -5 is returned (with
+ * CLAWGPU_ERR_NONFINITE) when a wave speed is NaN, where the reference would
+ * loop forever. */
+int clawgpu_claw2(clawgpu_ctx* ctx, double dx, double dy, double tstart, double* tend,
+                  double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
+                  const int* mthbc, int use_fwave, int* info);
+
+/* Tuning and measurement helpers (no reference counterpart): rows per
+ * wavefront segment of the step kernel, and nsteps fixed-dt steps timed with
+ * HIP events on the context's stream without per-step host synchronisation:
+ * *total_ms spans every launch, *step_kernel_ms sums the step-kernel launches
+ * alone, *cfl_max receives the largest Courant number of those steps. */
+int clawgpu_set_segment_rows(clawgpu_ctx* ctx, int rows);
+int clawgpu_time_steps(clawgpu_ctx* ctx, double dx, double dy, double dt, const int* method,
+                       const int* mthlim, const int* mthbc, int use_fwave, int nsteps,
+                       float* total_ms, float* step_kernel_ms, double* cfl_max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,36 @@
+"""Shared helpers for the parity tests (comparison bars and problem setups)."""
+import numpy as np
+
+from classic_amd import problems as P
+
+
+def interior(q, mbc=2):
+    return q[mbc:-mbc, mbc:-mbc]
+
+
+def max_rel_err(a, b, mbc=2):
+    """Per-field max |a-b| / max|b| over the interior, maximised over fields."""
+    a = interior(a, mbc)
+    b = interior(b, mbc)
+    worst = 0.0
+    for m in range(a.shape[-1]):
+        scale = np.abs(b[..., m]).max()
+        d = np.abs(a[..., m] - b[..., m]).max()
+        worst = max(worst, d / scale if scale > 0 else d)
+    return worst
+
+
+def smooth_acoustics(mx, my, seed=0, mbc=2):
+    rng = np.random.default_rng(seed)
+    q = np.zeros((my + 2 * mbc, mx + 2 * mbc, 3))
+    y, x = np.meshgrid(np.linspace(0, 1, my + 2 * mbc), np.linspace(0, 1, mx + 2 * mbc),
+                       indexing="ij")
+    q[..., 0] = 1.0 + np.exp(-40 * ((x - 0.4) ** 2 + (y - 0.55) ** 2))
+    q[..., 1] = 0.2 * np.sin(3 * x + 1) * np.cos(2 * y)
+    q[..., 2] = 0.1 * np.cos(5 * y) + 0.05 * rng.standard_normal(q.shape[:2])
+    q[..., 0] += 0.1 * (x > 0.7)  # a jump so the limiter has work
+    return q
+
+
+def euler_problem(mx, my, seed):
+    return P.euler_quadrants(mx, my, seed=seed)
