@@ -78,6 +78,13 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--arith", choices=["fast", "exact"], default="fast",
+                    help="fast: FMA + reciprocal reuse, held to 1e-12 of the reference by the "
+                         "tests; exact: the reference's rounding bit for bit")
+    ap.add_argument("--seg-rows", type=int, default=32)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain device copy of a known size (for the rocprof "
+                         "FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -90,7 +97,9 @@ def main():
     n = args.n
     q0, dx, dy = P.euler_quadrants(n, n, seed=args.seed)
     method, mthlim, mthbc = (1, 2, 2, 0, 0, 0, 0), [4, 4, 4], (1, 1, 1, 1)
-    g = gpu.Grid2D(SOLVER_EULER, n, n, [1.4, 1.0])
+    arith = 1 if args.arith == "fast" else 0
+    g = gpu.Grid2D(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith)
+    g.set_segment_rows(args.seg_rows)
     g.set_q(q0)
     # first dt from the CFL-desired rule of the initial data: the retake
     # logic corrects it on the first step if it is too large
@@ -115,7 +124,17 @@ def main():
     total_ms, kern_ms, _ = g.time_steps(dx, dy, dt_last, method, mthlim, mthbc, ksteps)
     kavg_s = kern_ms / ksteps / 1e3
     achieved = cells * BYTES_PER_CELL_UPDATE / kavg_s / 1e9
+    # the same kernel time with the other arithmetic, for reference
+    g.set_arith(1 - arith)
+    _, kern_other, _ = g.time_steps(dx, dy, dt_last, method, mthlim, mthbc, ksteps)
     g.close()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
     cpu = None
     if not args.no_cpu:
@@ -141,7 +160,9 @@ def main():
                    "method": list(method), "mthlim": mthlim, "mthbc": list(mthbc),
                    "cfl_desired": 0.9, "cfl_max": 1.0},
         "hbm_frac_of_peak": value * BYTES_PER_CELL_UPDATE / HBM_PEAK_GBS,
+        "arith": args.arith,
         "kernel_ms_avg": kavg_s * 1e3,
+        "kernel_ms_avg_other_arith": kern_other / ksteps,
         "step_overhead_ms_avg": (total_ms - kern_ms) / ksteps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profile_traffic()},

@@ -10,7 +10,7 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libclawgpu.so")
+LIB_PATH = os.environ.get("CLAWGPU_LIB") or os.path.join(HERE, "libclawgpu.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "clawgpu.h")
 
 OK = 0
@@ -21,6 +21,9 @@ ERR_NONFINITE = -4
 
 SOLVER_ACOUSTICS = 1
 SOLVER_EULER = 2
+
+ARITH_EXACT = 0
+ARITH_FAST = 1
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
@@ -40,6 +43,7 @@ _SIGS = {
     "clawgpu_step2": ([_V, _d, _d, _d, _I, _I, _i, _D], _i),
     "clawgpu_claw2": ([_V, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
     "clawgpu_set_segment_rows": ([_V, _i], _i),
+    "clawgpu_set_arith": ([_V, _i], _i),
     "clawgpu_time_steps": ([_V, _d, _d, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
 }
 

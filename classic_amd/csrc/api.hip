@@ -1,31 +1,7 @@
-// Fused 2D wave-propagation step for gfx950 (FP64).
-//
-// Replaces, for one unsplit time step, the reference's step2 -> flux2 ->
-// rpn2/limiter/rpt2 path (src/2d/step2.f90:2-251, src/2d/flux2.f90:5-235,
-// src/2d/inlinelimiter.f90:25-105) plus the bc2 ghost fill
-// (src/2d/bc2.f:4-215), in one kernel launch per step.
-//
-// Work decomposition ("column sweep").  Every 64-lane wavefront owns a strip
-// of 64 adjacent columns and a segment of rows.  Lane l holds column
-// i = 61*strip + l; lanes 1..61 produce output cells, lanes 0, 62 and 63 are
-// halo.  The wave walks down its segment one row J per iteration:
-//   * x-interfaces of row J are solved one per lane; the limiter's
-//     neighbouring waves and the contributions owed to the left neighbour
-//     cell come from adjacent lanes through cross-lane shuffles;
-//   * the y-interface between rows J-1 and J is solved per lane; the limiter
-//     needs the interfaces above and below, which the lane still holds in
-//     registers from the previous iteration, so the y-direction pipeline
-//     lags one row and finishes cell (i, J-2) at iteration J.
-// Every cell update is accumulated in registers in the reference's exact
-// order (x-sweep rows j-1, j, j+1, then y-sweep columns i-1, i, i+1, with the
-// parenthesisation of step2.f90:141-145 and :224-228), so with FMA
-// contraction off the result equals the reference arithmetic bit for bit.
-// q is read once from HBM (plus a 3-column / 3-row halo) and written once.
-//
-// Device layout: field planes (structure of arrays).  Element (m, i, j) of a
-// grid with mbc ghost cells lives at  base + m*fs + (j+mbc-1)*pitch +
-// (i+mbc-1), so the 64 lanes of a wave read 64 consecutive doubles of one
-// row of one field (512 B, coalesced).
+// Host side of libclawgpu.so: device contexts, the C-ABI of
+// include/clawgpu.h, the ghost-cell fill (bc2) and layout conversion kernels.
+// The fused step kernels live in step2d_kernel.hpp (instantiated by
+// step2d_exact.hip and step2d_fast.hip).
 
 #include <hip/hip_runtime.h>
 
@@ -35,304 +11,10 @@
 #include <cstring>
 #include <new>
 
-#include "riemann2d.hpp"
+#include "common.hpp"
 #include "../../include/clawgpu.h"
 
 namespace clawgpu {
-
-constexpr int kLanes = 64;
-constexpr int kOut = 61;  // output columns per wave strip
-constexpr int kWavesPerBlock = 4;
-constexpr int kMaxWaves = 8;
-
-struct Layout2 {
-    int mx, my, mbc, pitch;
-    long fs;
-    __host__ __device__ long off(int i, int j) const
-    {
-        return (long)(j + mbc - 1) * pitch + (i + mbc - 1);
-    }
-};
-
-struct StepArgs {
-    const double* qold;
-    double* qnew;
-    Layout2 L;
-    double dtdx, dtdy;
-    int order, trans, use_fwave;
-    int lim[kMaxWaves];
-    int nstrips, seglen, nsegs;
-    unsigned long long* cfl_bits;  // max of the Courant number, as raw bits
-    SolverParams P;
-};
-
-__device__ inline double shup(double v) { return __shfl_up(v, 1); }
-__device__ inline double shdn(double v) { return __shfl_down(v, 1); }
-
-// Max of a non-negative double over the wave, then one atomic per wave.
-// A NaN speed is reported as +inf so that the host can refuse the step.
-__device__ inline void publish_cfl(double cfl, unsigned long long* bits)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) cfl = fmax(cfl, __shfl_xor(cfl, d));
-    if ((threadIdx.x & 63) == 0) {
-        unsigned long long b = (unsigned long long)__double_as_longlong(cfl);
-        atomicMax(bits, b);
-    }
-}
-
-__device__ inline double cfl_term(double dtdx, double s)
-{
-    double t = fmax(dtdx * s, -(dtdx * s));
-    return (t == t) ? t : INFINITY;
-}
-
-template <class S>
-__global__ __launch_bounds__(256) void step2_unsplit_kernel(StepArgs a)
-{
-    constexpr int M = S::M, MW = S::MW;
-    const int lane = threadIdx.x & 63;
-    const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (wid >= a.nstrips * a.nsegs) return;  // uniform per wave
-    const int strip = wid % a.nstrips;
-    const int seg = wid / a.nstrips;
-    const Layout2 L = a.L;
-    const int mx = L.mx, my = L.my, mbc = L.mbc;
-    const int i = kOut * strip + lane;
-    const int ic = i < mx + mbc ? i : mx + mbc;          // clamped column for loads
-    const int icl = (i - 1 < mx + mbc) ? i - 1 : mx + mbc;  // left neighbour (lane 0 only)
-    const int r0 = 1 + seg * a.seglen;
-    const int r1 = min(r0 + a.seglen, my + 1);  // output rows r0 .. r1-1
-    const bool out_lane = lane >= 1 && lane <= kOut && i <= mx;
-    const bool xcfl_lane = i >= 1 && i <= mx + 1;
-    const bool ycfl_lane = i <= mx + 1;
-    const double dtdx = a.dtdx, dtdy = a.dtdy;
-    const bool second = a.order == 2;
-    const bool tr2 = second && a.trans == 2;
-    const double* __restrict__ qo = a.qold;
-    double* __restrict__ qn = a.qnew;
-
-    double qprev[M];
-#pragma unroll
-    for (int m = 0; m < M; m++) qprev[m] = qo[m * L.fs + L.off(ic, r0 - 2)];
-
-    double vA[M], vB[M], g2prev[M];
-    double wyp[MW][M], syp[MW], amyp[M], apyp[M], dotly[MW];
-    double qa_part[M], fa_part[M], gp1[M], gp2[M];
-    typename S::Cache cyp;
-#pragma unroll
-    for (int m = 0; m < M; m++) {
-        vA[m] = vB[m] = g2prev[m] = 0.0;
-        amyp[m] = apyp[m] = qa_part[m] = fa_part[m] = gp1[m] = gp2[m] = 0.0;
-    }
-#pragma unroll
-    for (int mw = 0; mw < MW; mw++) {
-        syp[mw] = dotly[mw] = 0.0;
-#pragma unroll
-        for (int m = 0; m < M; m++) wyp[mw][m] = 0.0;
-    }
-    double cfl = 0.0;
-
-    for (int J = r0 - 1; J <= r1 + 1; ++J) {
-        double qc[M];
-#pragma unroll
-        for (int m = 0; m < M; m++) qc[m] = qo[m * L.fs + L.off(ic, J)];
-        double vC[M];
-
-        // ------------------------------------------------ x-row J (flux2, ixy=1)
-        if (J <= r1) {
-            double ql[M];
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                ql[m] = shup(qc[m]);
-                if (lane == 0) ql[m] = qo[m * L.fs + L.off(icl, J)];
-            }
-            double wx[MW][M], sx[MW], amx[M], apx[M];
-            typename S::Cache cx;
-            S::template rpn<1, 2>(a.P, ql, qc, wx, sx, amx, apx, cx);
-            if (xcfl_lane) {
-#pragma unroll
-                for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, cfl_term(dtdx, sx[mw]));
-            }
-            double cq[M], fa[M];
-#pragma unroll
-            for (int m = 0; m < M; m++) cq[m] = fa[m] = 0.0;
-            if (second) {
-                double dotr[MW], wn2[MW];
-#pragma unroll
-                for (int mw = 0; mw < MW; mw++) {
-                    double dr = 0.0, nn = 0.0;
-#pragma unroll
-                    for (int m = 0; m < M; m++) {
-                        double wr = shdn(wx[mw][m]);
-                        nn = nn + wx[mw][m] * wx[mw][m];
-                        dr = dr + wx[mw][m] * wr;
-                    }
-                    dotr[mw] = dr;
-                    wn2[mw] = nn;
-                }
-#pragma unroll
-                for (int mw = 0; mw < MW; mw++) {
-                    double dl = shup(dotr[mw]);
-                    if (a.lim[mw] != 0 && wn2[mw] != 0.0) {
-                        double r = (sx[mw] > 0.0 ? dl : dotr[mw]) / wn2[mw];
-                        double phi = philim(r, a.lim[mw]);
-#pragma unroll
-                        for (int m = 0; m < M; m++) wx[mw][m] = phi * wx[mw][m];
-                    }
-                }
-#pragma unroll
-                for (int mw = 0; mw < MW; mw++) {
-                    double sa = fabs(sx[mw]);
-                    double sw = a.use_fwave ? copysign(1.0, sx[mw]) : sa;
-                    double coef = sw * (1.0 - sa * dtdx);
-#pragma unroll
-                    for (int m = 0; m < M; m++) cq[m] = cq[m] + coef * wx[mw][m];
-                }
-#pragma unroll
-                for (int m = 0; m < M; m++) fa[m] = 0.5 * cq[m];
-            }
-            double G1[M], G2[M];
-            if (a.trans > 0) {
-                double amt[M], apt[M];
-#pragma unroll
-                for (int m = 0; m < M; m++) {
-                    amt[m] = tr2 ? amx[m] + cq[m] : amx[m];
-                    apt[m] = tr2 ? apx[m] - cq[m] : apx[m];
-                }
-                double bma[M], bpa[M], bmp[M], bpp[M];
-                S::template rpt<1, 2>(a.P, cx, amt, bma, bpa);
-                S::template rpt<1, 2>(a.P, cx, apt, bmp, bpp);
-#pragma unroll
-                for (int m = 0; m < M; m++) {
-                    double g1n = shdn(-0.5 * dtdx * bma[m]);
-                    double g2n = shdn(-0.5 * dtdx * bpa[m]);
-                    G1[m] = g1n - 0.5 * dtdx * bmp[m];
-                    G2[m] = g2n - 0.5 * dtdx * bpp[m];
-                }
-            } else {
-#pragma unroll
-                for (int m = 0; m < M; m++) G1[m] = G2[m] = 0.0;
-            }
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                double amn = shdn(amx[m]);
-                double fan = shdn(fa[m]);
-                double qadd = (0.0 - dtdx * apx[m]) - dtdx * amn;
-                double v = qc[m] + dtdy * g2prev[m];
-                v = v + qadd - dtdx * (fan - fa[m]) - dtdy * (G2[m] - G1[m]);
-                vC[m] = v;
-                vB[m] = vB[m] - dtdy * G1[m];
-                g2prev[m] = G2[m];
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < M; m++) vC[m] = 0.0;
-        }
-
-        // ------------------------------ y-interface J (flux2, ixy=2) on column i
-        double wy[MW][M], sy[MW], amy[M], apy[M];
-        typename S::Cache cy;
-        S::template rpn<2, 1>(a.P, qprev, qc, wy, sy, amy, apy, cy);
-        if (ycfl_lane && J >= 1 && J <= my + 1) {
-#pragma unroll
-            for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, cfl_term(dtdy, sy[mw]));
-        }
-        double dotry[MW];
-#pragma unroll
-        for (int mw = 0; mw < MW; mw++) {
-            double dr = 0.0;
-#pragma unroll
-            for (int m = 0; m < M; m++) dr = dr + wyp[mw][m] * wy[mw][m];
-            dotry[mw] = dr;
-        }
-        if (J - 1 >= r0) {
-            // limit interface J-1 (held in wyp) with its neighbours J-2 and J
-            double cq[M], fa[M];
-#pragma unroll
-            for (int m = 0; m < M; m++) cq[m] = fa[m] = 0.0;
-            if (second) {
-#pragma unroll
-                for (int mw = 0; mw < MW; mw++) {
-                    double nn = 0.0;
-#pragma unroll
-                    for (int m = 0; m < M; m++) nn = nn + wyp[mw][m] * wyp[mw][m];
-                    if (a.lim[mw] != 0 && nn != 0.0) {
-                        double r = (syp[mw] > 0.0 ? dotly[mw] : dotry[mw]) / nn;
-                        double phi = philim(r, a.lim[mw]);
-#pragma unroll
-                        for (int m = 0; m < M; m++) wyp[mw][m] = phi * wyp[mw][m];
-                    }
-                }
-#pragma unroll
-                for (int mw = 0; mw < MW; mw++) {
-                    double sa = fabs(syp[mw]);
-                    double sw = a.use_fwave ? copysign(1.0, syp[mw]) : sa;
-                    double coef = sw * (1.0 - sa * dtdy);
-#pragma unroll
-                    for (int m = 0; m < M; m++) cq[m] = cq[m] + coef * wyp[mw][m];
-                }
-#pragma unroll
-                for (int m = 0; m < M; m++) fa[m] = 0.5 * cq[m];
-            }
-            double bma[M], bpa[M], bmp[M], bpp[M];
-            if (a.trans > 0) {
-                double amt[M], apt[M];
-#pragma unroll
-                for (int m = 0; m < M; m++) {
-                    amt[m] = tr2 ? amyp[m] + cq[m] : amyp[m];
-                    apt[m] = tr2 ? apyp[m] - cq[m] : apyp[m];
-                }
-                S::template rpt<2, 1>(a.P, cyp, amt, bma, bpa);
-                S::template rpt<2, 1>(a.P, cyp, apt, bmp, bpp);
-            } else {
-#pragma unroll
-                for (int m = 0; m < M; m++) bma[m] = bpa[m] = bmp[m] = bpp[m] = 0.0;
-            }
-            if (J - 2 >= r0) {
-                // cell (i, J-2): its y-sweep terms are now complete
-#pragma unroll
-                for (int m = 0; m < M; m++) {
-                    double qadd = qa_part[m] - dtdy * amyp[m];
-                    double dF = fa[m] - fa_part[m];
-                    double G1 = a.trans > 0 ? (-0.5 * dtdy * bma[m]) - gp1[m] : 0.0;
-                    double G2 = a.trans > 0 ? (-0.5 * dtdy * bpa[m]) - gp2[m] : 0.0;
-                    double g2l = shup(G2);
-                    double g1r = shdn(G1);
-                    double v = vA[m] + dtdx * g2l;
-                    v = v + (qadd - dtdy * dF - dtdx * (G2 - G1));
-                    v = v - dtdx * g1r;
-                    if (out_lane) qn[m * L.fs + L.off(i, J - 2)] = v;
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                qa_part[m] = 0.0 - dtdy * apyp[m];
-                fa_part[m] = fa[m];
-                gp1[m] = 0.5 * dtdy * bmp[m];
-                gp2[m] = 0.5 * dtdy * bpp[m];
-            }
-        }
-        // ------------------------------------------------------ shift the pipeline
-#pragma unroll
-        for (int mw = 0; mw < MW; mw++) {
-            dotly[mw] = dotry[mw];
-            syp[mw] = sy[mw];
-#pragma unroll
-            for (int m = 0; m < M; m++) wyp[mw][m] = wy[mw][m];
-        }
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-            amyp[m] = amy[m];
-            apyp[m] = apy[m];
-            vA[m] = vB[m];
-            vB[m] = vC[m];
-            qprev[m] = qc[m];
-        }
-        cyp = cy;
-    }
-    publish_cfl(cfl, a.cfl_bits);
-}
 
 // ------------------------------------------------------------------ bc2
 // Every ghost cell is written from the interior in one pass: the row index
@@ -441,6 +123,7 @@ struct clawgpu_ctx {
     double* hcfl;  // pinned
     hipStream_t stream;
     int seglen;
+    int arith;
     int last_hip_error;
 };
 
@@ -470,12 +153,12 @@ static int solver_shape(int solver, int* meqn, int* mwaves)
 {
     switch (solver) {
     case CLAWGPU_SOLVER_ACOUSTICS:
-        *meqn = Acoustics2::M;
-        *mwaves = Acoustics2::MW;
+        *meqn = 3;
+        *mwaves = 2;
         return 0;
     case CLAWGPU_SOLVER_EULER:
-        *meqn = Euler2::M;
-        *mwaves = Euler2::MW;
+        *meqn = 4;
+        *mwaves = 3;
         return 0;
     default: return -1;
     }
@@ -524,7 +207,8 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     c->L.pitch = ((mx + 2 * mbc) + 15) / 16 * 16;
     c->L.fs = ((long)c->L.pitch * (my + 2 * mbc) + 31) / 32 * 32;
     c->cur = 0;
-    c->seglen = 64;
+    c->seglen = 32;
+    c->arith = ARITH_EXACT;
     size_t qbytes = (size_t)c->L.fs * meqn * sizeof(double);
     size_t abytes = (size_t)(mx + 2 * mbc) * (my + 2 * mbc) * meqn * sizeof(double);
     hipError_t e = hipSuccess;
@@ -657,18 +341,16 @@ static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const i
     a.trans = method[2];
     a.use_fwave = use_fwave ? 1 : 0;
     for (int k = 0; k < kMaxWaves; k++) a.lim[k] = k < c->mwaves ? mthlim[k] : 0;
-    a.nstrips = (c->mx + kOut - 1) / kOut;
+    a.nstrips = (c->mx + kStripOut - 1) / kStripOut;
     a.seglen = c->seglen;
     a.nsegs = (c->my + a.seglen - 1) / a.seglen;
     a.cfl_bits = c->dcfl;
     a.P = c->P;
     long waves = (long)a.nstrips * a.nsegs;
     int nb = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    if (c->solver == CLAWGPU_SOLVER_ACOUSTICS)
-        hipLaunchKernelGGL(step2_unsplit_kernel<Acoustics2>, dim3(nb), dim3(256), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL(step2_unsplit_kernel<Euler2>, dim3(nb), dim3(256), 0, c->stream, a);
-    HIPCHK(c, hipGetLastError());
+    int e = c->arith == ARITH_FAST ? launch_step2_fast(c->solver, a, nb, c->stream)
+                                   : launch_step2_exact(c->solver, a, nb, c->stream);
+    HIPCHK(c, (hipError_t)e);
     return CLAWGPU_OK;
 }
 
@@ -784,6 +466,14 @@ extern "C" int clawgpu_set_segment_rows(clawgpu_ctx* c, int rows)
 {
     if (!c || rows < 1) return set_err(CLAWGPU_ERR_ARG, "rows must be >= 1");
     c->seglen = rows;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_set_arith(clawgpu_ctx* c, int mode)
+{
+    if (!c || (mode != CLAWGPU_ARITH_EXACT && mode != CLAWGPU_ARITH_FAST))
+        return set_err(CLAWGPU_ERR_ARG, "mode must be CLAWGPU_ARITH_EXACT or CLAWGPU_ARITH_FAST");
+    c->arith = mode;
     return CLAWGPU_OK;
 }
 

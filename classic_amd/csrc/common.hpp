@@ -1,0 +1,49 @@
+// Types shared by the host API and the step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace clawgpu {
+
+constexpr int kMaxWaves = 8;
+constexpr int kWavesPerBlock = 4;
+constexpr int kStripOut = 60;  // output columns per 64-lane strip (lanes 2..61)
+
+struct SolverParams {
+    double rho, bulk, cc, zz;  // acoustics: density, bulk modulus, c, impedance
+    double gamma, gamma1;      // Euler: gamma and gamma - 1
+    int efix;                  // Euler: Harten-Hyman entropy fix
+};
+
+// Field-plane layout of a grid with mbc ghost cells: element (m, i, j) at
+// m*fs + (j+mbc-1)*pitch + (i+mbc-1), 1-based Fortran cell indices.
+struct Layout2 {
+    int mx, my, mbc, pitch;
+    long fs;
+    __host__ __device__ long off(int i, int j) const
+    {
+        return (long)(j + mbc - 1) * pitch + (i + mbc - 1);
+    }
+};
+
+struct StepArgs {
+    const double* qold;
+    double* qnew;
+    Layout2 L;
+    double dtdx, dtdy;
+    int order, trans, use_fwave;
+    int lim[kMaxWaves];
+    int nstrips, seglen, nsegs;
+    unsigned long long* cfl_bits;  // max Courant number of the step, as raw bits
+    SolverParams P;
+};
+
+enum { ARITH_EXACT = 0, ARITH_FAST = 1 };
+
+// Launch the unsplit step kernel for solver id `solver`; returns a
+// hipError_t value.  exact: reference rounding (FMA contraction off,
+// correctly rounded division); fast: FMA contraction and reciprocal reuse.
+int launch_step2_exact(int solver, const StepArgs& a, int nblocks, hipStream_t s);
+int launch_step2_fast(int solver, const StepArgs& a, int nblocks, hipStream_t s);
+
+}  // namespace clawgpu

@@ -4,44 +4,106 @@
 // (src/2d/claw2.f:312-402; call sites src/2d/flux2.f90:110-111, :204-205,
 // :220-221).  A GPU cannot call host Fortran procedures, so each solver is a
 // compile-time policy selected by a solver id.  One call solves ONE interface
-// between a left state qa (cell i-1) and a right state qb (cell i); the
-// normal momentum sits in slot MU and the transverse one in slot MV (0-based),
-// which is how the reference's ixy argument is expressed here.
+// between a left cell L (cell i-1) and a right cell R (cell i); the normal
+// momentum sits in slot MU and the transverse one in slot MV (0-based), which
+// is how the reference's ixy argument is expressed here.
+//
+// Quantities that depend on one cell only (sqrt(rho), pressure, sound speed,
+// velocities, the Roe weights q/sqrt(rho)) are evaluated once per cell in
+// cell() and shared by the cell's two interfaces in each direction.
 //
 // The arithmetic is written from LeVeque, "Finite Volume Methods for
 // Hyperbolic Problems": constant-coefficient acoustics (ch. 18/21) and the
 // Roe solver for the Euler equations with the Harten-Hyman entropy fix
-// (sec. 15.3).  Expression order is fixed so that results are reproducible
-// bit for bit by the CPU restatement that the tests compare against.
+// (sec. 15.3).  With ExactMath every operation is the one the CPU
+// restatement performs, in the same order, so (compiled without FMA
+// contraction) results agree bit for bit; FastMath reuses reciprocals and
+// is compiled with contraction, agreeing to a few ulps per operation.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include "common.hpp"
+
 namespace clawgpu {
 
-struct SolverParams {
-    double rho, bulk, cc, zz;  // acoustics: density, bulk modulus, c, impedance
-    double gamma, gamma1;      // Euler: gamma and gamma - 1
-    int efix;                  // Euler: Harten-Hyman entropy fix
+struct ExactMath {
+    static constexpr bool fast = false;
+    __device__ static inline double div(double a, double b) { return a / b; }
+    __device__ static inline double rcp(double b) { return 1.0 / b; }
+    __device__ static inline double sqrt_(double a) { return sqrt(a); }
+};
+
+struct FastMath {
+    static constexpr bool fast = true;
+    // hardware reciprocal refined by two Newton steps (about 1 ulp)
+    __device__ static inline double rcp(double b)
+    {
+        double r = __builtin_amdgcn_rcp(b);
+        double e = fma(-b, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-b, r, 1.0);
+        return fma(r, e, r);
+    }
+    __device__ static inline double div(double a, double b)
+    {
+        double r = rcp(b);
+        double q = a * r;
+        return fma(fma(-b, q, a), r, q);
+    }
+    // hardware reciprocal square root, one coupled Newton step for sqrt and
+    // 1/(2 sqrt), then a residual correction (a > 0; about 1 ulp)
+    __device__ static inline double sqrt_(double a)
+    {
+        double y = __builtin_amdgcn_rsq(a);
+        double g = a * y;
+        double h = 0.5 * y;
+        double r = fma(-g, h, 0.5);
+        g = fma(g, r, g);
+        h = fma(h, r, h);
+        double d = fma(-g, g, a);
+        return fma(d, h, g);
+    }
 };
 
 // Constant-coefficient acoustics, q = (p, u, v).  Waves r1 = (-Z, 1, 0) at
 // speed -c and r2 = (Z, 1, 0) at speed +c along the normal direction.
+template <class Mth>
 struct Acoustics2 {
+    using Math = Mth;
     static constexpr int M = 3;
     static constexpr int MW = 2;
+    struct Cell {
+        double q[M];
+    };
+    static constexpr int XSHIFT = 3;  // leading doubles of Cell an x-interface needs
     struct Cache {};
 
+    __device__ static inline Cell cell(const SolverParams&, const double* q)
+    {
+        Cell c;
+#pragma unroll
+        for (int m = 0; m < M; m++) c.q[m] = q[m];
+        return c;
+    }
+
     template <int MU, int MV>
-    __device__ static inline void rpn(const SolverParams& P, const double* qa, const double* qb,
+    __device__ static inline void rpn(const SolverParams& P, const Cell& L, const Cell& R,
                                       double w[MW][M], double s[MW], double am[M], double ap[M],
                                       Cache&)
     {
         const double zz = P.zz, cc = P.cc;
-        double dp = qb[0] - qa[0];
-        double du = qb[MU] - qa[MU];
-        double a1 = (-dp + zz * du) / (2.0 * zz);
-        double a2 = (dp + zz * du) / (2.0 * zz);
+        double dp = R.q[0] - L.q[0];
+        double du = R.q[MU] - L.q[MU];
+        double a1, a2;
+        if constexpr (Mth::fast) {
+            double h = 0.5 / zz;
+            a1 = (-dp + zz * du) * h;
+            a2 = (dp + zz * du) * h;
+        } else {
+            a1 = (-dp + zz * du) / (2.0 * zz);
+            a2 = (dp + zz * du) / (2.0 * zz);
+        }
         w[0][0] = -a1 * zz;
         w[0][MU] = a1;
         w[0][MV] = 0.0;
@@ -64,8 +126,15 @@ struct Acoustics2 {
                                       double bm[M], double bp[M])
     {
         const double zz = P.zz, cc = P.cc;
-        double a1 = (-asdq[0] + zz * asdq[MV]) / (2.0 * zz);
-        double a2 = (asdq[0] + zz * asdq[MV]) / (2.0 * zz);
+        double a1, a2;
+        if constexpr (Mth::fast) {
+            double h = 0.5 / zz;
+            a1 = (-asdq[0] + zz * asdq[MV]) * h;
+            a2 = (asdq[0] + zz * asdq[MV]) * h;
+        } else {
+            a1 = (-asdq[0] + zz * asdq[MV]) / (2.0 * zz);
+            a2 = (asdq[0] + zz * asdq[MV]) / (2.0 * zz);
+        }
         bm[0] = cc * a1 * zz;
         bm[MU] = 0.0;
         bm[MV] = -cc * a1;
@@ -78,51 +147,106 @@ struct Acoustics2 {
 // Euler equations, q = (rho, rho u, rho v, E), ideal gas.  Three waves: the
 // u-a acoustic wave, the entropy and shear waves lumped at speed u, and the
 // u+a acoustic wave.
+template <class Mth>
 struct Euler2 {
+    using Math = Mth;
     static constexpr int M = 4;
     static constexpr int MW = 3;
+    struct Cell {
+        double q[M];
+        double sr;     // sqrt(rho)
+        double rw[2];  // q(2)/sqrt(rho), q(3)/sqrt(rho): Roe weights of the momenta
+        double hw;     // (E + p)/sqrt(rho): Roe weight of the enthalpy
+        double c;      // sound speed
+        double vel[2]; // q(2)/rho, q(3)/rho
+    };
+    // an x-interface needs all but vel[1] of its left cell (Cell order)
+    static constexpr int XSHIFT = 10;
     struct Cache {
-        double u, v, enth, a, g1a2, euv, u2v2;
+        double u, v, enth, a, g1a2, euv, u2v2, h2a;  // h2a = 1/(2a) (fast math only)
     };
 
     __device__ static inline double pressure(const double* q, double g1)
     {
-        return g1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+        if constexpr (Mth::fast)
+            return g1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) * Mth::rcp(q[0]));
+        else
+            return g1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
     }
 
-    template <int MU, int MV>
-    __device__ static inline void roe(const SolverParams& P, const double* qa, const double* qb,
-                                      Cache& r)
+    __device__ static inline Cell cell(const SolverParams& P, const double* q)
     {
+        Cell c;
+#pragma unroll
+        for (int m = 0; m < M; m++) c.q[m] = q[m];
         const double g1 = P.gamma1;
-        double rhsqrtl = sqrt(qa[0]);
-        double rhsqrtr = sqrt(qb[0]);
-        double pl = pressure(qa, g1);
-        double pr = pressure(qb, g1);
-        double rhsq2 = rhsqrtl + rhsqrtr;
-        r.u = (qa[MU] / rhsqrtl + qb[MU] / rhsqrtr) / rhsq2;
-        r.v = (qa[MV] / rhsqrtl + qb[MV] / rhsqrtr) / rhsq2;
-        r.enth = ((qa[3] + pl) / rhsqrtl + (qb[3] + pr) / rhsqrtr) / rhsq2;
-        r.u2v2 = r.u * r.u + r.v * r.v;
-        double a2 = g1 * (r.enth - 0.5 * r.u2v2);
-        r.a = sqrt(a2);
-        r.g1a2 = g1 / a2;
-        r.euv = r.enth - r.u2v2;
+        c.sr = Mth::sqrt_(q[0]);
+        if constexpr (Mth::fast) {
+            double ri = Mth::rcp(q[0]);
+            double p = g1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) * ri);
+            double isr = c.sr * ri;
+            c.rw[0] = q[1] * isr;
+            c.rw[1] = q[2] * isr;
+            c.hw = (q[3] + p) * isr;
+            c.c = Mth::sqrt_(P.gamma * p * ri);
+            c.vel[0] = q[1] * ri;
+            c.vel[1] = q[2] * ri;
+        } else {
+            double p = g1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+            c.rw[0] = q[1] / c.sr;
+            c.rw[1] = q[2] / c.sr;
+            c.hw = (q[3] + p) / c.sr;
+            c.c = sqrt(P.gamma * p / q[0]);
+            c.vel[0] = q[1] / q[0];
+            c.vel[1] = q[2] / q[0];
+        }
+        return c;
     }
 
     template <int MU, int MV>
-    __device__ static inline void rpn(const SolverParams& P, const double* qa, const double* qb,
+    __device__ static inline void rpn(const SolverParams& P, const Cell& L, const Cell& R,
                                       double w[MW][M], double s[MW], double am[M], double ap[M],
                                       Cache& r)
     {
-        roe<MU, MV>(P, qa, qb, r);
+        const double g1 = P.gamma1;
+        const double* qa = L.q;
+        const double* qb = R.q;
+        // Roe averages (FVMHP 15.3.2)
+        double rhsq2 = L.sr + R.sr;
+        if constexpr (Mth::fast) {
+            double inv = Mth::rcp(rhsq2);
+            r.u = (L.rw[MU - 1] + R.rw[MU - 1]) * inv;
+            r.v = (L.rw[MV - 1] + R.rw[MV - 1]) * inv;
+            r.enth = (L.hw + R.hw) * inv;
+        } else {
+            r.u = (L.rw[MU - 1] + R.rw[MU - 1]) / rhsq2;
+            r.v = (L.rw[MV - 1] + R.rw[MV - 1]) / rhsq2;
+            r.enth = (L.hw + R.hw) / rhsq2;
+        }
+        r.u2v2 = r.u * r.u + r.v * r.v;
+        double a2 = g1 * (r.enth - 0.5 * r.u2v2);
+        r.a = Mth::sqrt_(a2);
+        if constexpr (Mth::fast) {
+            double ia2 = Mth::rcp(a2);
+            r.g1a2 = g1 * ia2;
+            r.h2a = 0.5 * r.a * ia2;
+        } else {
+            r.g1a2 = g1 / a2;
+            r.h2a = 0.0;
+        }
+        r.euv = r.enth - r.u2v2;
+
         double d1 = qb[0] - qa[0];
         double d2 = qb[MU] - qa[MU];
         double d3 = qb[MV] - qa[MV];
         double d4 = qb[3] - qa[3];
         double a3 = r.g1a2 * (r.euv * d1 + r.u * d2 + r.v * d3 - d4);
-        double a2 = d3 - r.v * d1;
-        double a4 = (d2 + (r.a - r.u) * d1 - r.a * a3) / (2.0 * r.a);
+        double a2s = d3 - r.v * d1;
+        double a4;
+        if constexpr (Mth::fast)
+            a4 = (d2 + (r.a - r.u) * d1 - r.a * a3) * r.h2a;
+        else
+            a4 = (d2 + (r.a - r.u) * d1 - r.a * a3) / (2.0 * r.a);
         double a1 = d1 - a3 - a4;
         w[0][0] = a1;
         w[0][MU] = a1 * (r.u - r.a);
@@ -131,8 +255,8 @@ struct Euler2 {
         s[0] = r.u - r.a;
         w[1][0] = a3;
         w[1][MU] = a3 * r.u;
-        w[1][MV] = a3 * r.v + a2;
-        w[1][3] = a3 * 0.5 * r.u2v2 + a2 * r.v;
+        w[1][MV] = a3 * r.v + a2s;
+        w[1][3] = a3 * 0.5 * r.u2v2 + a2s * r.v;
         s[1] = r.u;
         w[2][0] = a4;
         w[2][MU] = a4 * (r.u + r.a);
@@ -159,20 +283,24 @@ struct Euler2 {
 
         // Harten-Hyman: left-going flux difference with a transonic split of
         // the 1- or 3-wave; the right-going part is the rest of sum(s*W).
-        const double gamma = P.gamma, g1 = P.gamma1;
+        const double gamma = P.gamma;
         double amv[M] = {0.0, 0.0, 0.0, 0.0};
         do {
-            double rhoa = qa[0];
-            double pa = pressure(qa, g1);
-            double ca = sqrt(gamma * pa / rhoa);
-            double s0 = qa[MU] / rhoa - ca;
+            double s0 = L.vel[MU - 1] - L.c;  // u-c in the left cell
             if (s0 >= 0.0 && s[0] > 0.0) break;
             double qm[M];
 #pragma unroll
             for (int m = 0; m < M; m++) qm[m] = qa[m] + w[0][m];
-            double pm = pressure(qm, g1);
-            double cm = sqrt(gamma * pm / qm[0]);
-            double s1 = qm[MU] / qm[0] - cm;
+            double s1;
+            if constexpr (Mth::fast) {
+                double ri = Mth::rcp(qm[0]);
+                double pm = g1 * (qm[3] - 0.5 * (qm[1] * qm[1] + qm[2] * qm[2]) * ri);
+                s1 = qm[MU] * ri - Mth::sqrt_(gamma * pm * ri);
+            } else {
+                double pm = pressure(qm, g1);
+                double cm = sqrt(gamma * pm / qm[0]);
+                s1 = qm[MU] / qm[0] - cm;
+            }
             double sfract;
             if (s0 < 0.0 && s1 > 0.0)
                 sfract = s0 * (s1 - s[0]) / (s1 - s0);
@@ -185,16 +313,20 @@ struct Euler2 {
             if (s[1] >= 0.0) break;
 #pragma unroll
             for (int m = 0; m < M; m++) amv[m] = amv[m] + s[1] * w[1][m];
-            double rhob = qb[0];
-            double pb = pressure(qb, g1);
-            double cb = sqrt(gamma * pb / rhob);
-            double s3 = qb[MU] / rhob + cb;
+            double s3 = R.vel[MU - 1] + R.c;  // u+c in the right cell
             double qn[M];
 #pragma unroll
             for (int m = 0; m < M; m++) qn[m] = qb[m] - w[2][m];
-            double pn = pressure(qn, g1);
-            double cn = sqrt(gamma * pn / qn[0]);
-            double s2 = qn[MU] / qn[0] + cn;
+            double s2;
+            if constexpr (Mth::fast) {
+                double ri = Mth::rcp(qn[0]);
+                double pn = g1 * (qn[3] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2]) * ri);
+                s2 = qn[MU] * ri + Mth::sqrt_(gamma * pn * ri);
+            } else {
+                double pn = pressure(qn, g1);
+                double cn = sqrt(gamma * pn / qn[0]);
+                s2 = qn[MU] / qn[0] + cn;
+            }
             if (s2 < 0.0 && s3 > 0.0)
                 sfract = s2 * (s3 - s[2]) / (s3 - s2);
             else if (s[2] < 0.0)
@@ -223,7 +355,11 @@ struct Euler2 {
         double d1 = asdq[0], d2 = asdq[MU], d3 = asdq[MV], d4 = asdq[3];
         double a3 = r.g1a2 * (r.euv * d1 + r.u * d2 + r.v * d3 - d4);
         double a2 = d2 - r.u * d1;
-        double a4 = (d3 + (r.a - r.v) * d1 - r.a * a3) / (2.0 * r.a);
+        double a4;
+        if constexpr (Mth::fast)
+            a4 = (d3 + (r.a - r.v) * d1 - r.a * a3) * r.h2a;
+        else
+            a4 = (d3 + (r.a - r.v) * d1 - r.a * a3) / (2.0 * r.a);
         double a1 = d1 - a3 - a4;
         double wb[3][M], sb[3];
         wb[0][0] = a1;
@@ -257,12 +393,13 @@ struct Euler2 {
 
 // Wave limiter phi(r) (src/2d/inlinelimiter.f90:71-97): 1 minmod,
 // 2 superbee, 3 van Leer, 4 monotonized centred, 5 Beam-Warming.
+template <class Mth>
 __device__ static inline double philim(double r, int lim)
 {
     switch (lim) {
     case 1: return fmax(0.0, fmin(1.0, r));
     case 2: return fmax(0.0, fmax(fmin(1.0, 2.0 * r), fmin(2.0, r)));
-    case 3: return (r + fabs(r)) / (1.0 + fabs(r));
+    case 3: return Mth::div(r + fabs(r), 1.0 + fabs(r));
     case 4: {
         double c = (1.0 + r) / 2.0;
         return fmax(0.0, fmin(c, fmin(2.0, 2.0 * r)));

@@ -1,0 +1,405 @@
+// Fused 2D wave-propagation step for gfx950 (FP64): kernel template.
+//
+// Replaces, for one unsplit time step, the reference's step2 -> flux2 ->
+// rpn2/limiter/rpt2 path (src/2d/step2.f90:2-251, src/2d/flux2.f90:5-235,
+// src/2d/inlinelimiter.f90:25-105) with one kernel launch.
+//
+// Work decomposition ("column sweep").  Every 64-lane wavefront owns a strip
+// of 64 adjacent columns and a segment of rows.  Lane l holds column
+// i = 60*strip + l - 1; lanes 2..61 produce output cells, lanes 0, 1, 62, 63
+// are halo.  The wave walks down its segment one row J per iteration:
+//   * x-interfaces of row J are solved one per lane (interface i between
+//     cells i-1 and i); the left cell's data, the limiter's neighbouring
+//     waves and the contributions owed to the left neighbour cell come from
+//     adjacent lanes through cross-lane shifts;
+//   * the y-interface between rows J-1 and J is solved per lane; the limiter
+//     needs the interfaces above and below, which the lane still holds in
+//     registers from the previous iteration, so the y pipeline lags one row
+//     and finishes cell (i, J-2) at iteration J.
+// Every cell update is accumulated in registers in the reference's order
+// (x-sweep rows j-1, j, j+1, then y-sweep columns i-1, i, i+1, with the
+// parenthesisation of step2.f90:141-145 and :224-228).  q is read once from
+// HBM (plus a 4-column / 3-row halo) and written once; the next row is
+// prefetched while the current one is processed, and the row loop is
+// unrolled by two so that the y pipeline alternates between two register
+// sets instead of copying one into the other.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "riemann2d.hpp"
+
+#ifndef CLAWGPU_DPP
+#define CLAWGPU_DPP 1
+#endif
+#ifndef CLAWGPU_MIN_WAVES
+#define CLAWGPU_MIN_WAVES 1
+#endif
+
+namespace clawgpu {
+
+// Whole-wave shift by one lane: up(v) gives lane l the value of lane l-1,
+// dn(v) the value of lane l+1 (lanes with no source get 0 or their own).
+__device__ inline double lane_up(double v)
+{
+#if CLAWGPU_DPP
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);  // wave_shr:1
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+#else
+    return __shfl_up(v, 1);
+#endif
+}
+
+__device__ inline double lane_dn(double v)
+{
+#if CLAWGPU_DPP
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);  // wave_shl:1
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+#else
+    return __shfl_down(v, 1);
+#endif
+}
+
+template <class T>
+__device__ inline T lane_up_t(const T& v)
+{
+    T r;
+    const double* s = reinterpret_cast<const double*>(&v);
+    double* d = reinterpret_cast<double*>(&r);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / sizeof(double)); k++) d[k] = lane_up(s[k]);
+    return r;
+}
+
+// Max of a non-negative double over the wave, then one atomic per wave.
+__device__ inline void publish_cfl(double cfl, unsigned long long* bits)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) cfl = fmax(cfl, __shfl_xor(cfl, d));
+    if ((threadIdx.x & 63) == 0) atomicMax(bits, (unsigned long long)__double_as_longlong(cfl));
+}
+
+// Courant contribution max(dtdx*s, -dtdx*s) (flux2.f90:129-130); a NaN
+// speed becomes +inf so that the host refuses the step instead of
+// silently dropping it from the max.
+__device__ inline double cfl_term(double dtdx, double s)
+{
+    double t = fmax(dtdx * s, -(dtdx * s));
+    return (t == t) ? t : __builtin_huge_val();
+}
+
+// Method flags fixed at compile time (CfgFix) or read from the arguments
+// (CfgAny).  The fixed variant serves the common case method(2)=2,
+// method(3)=2, one limiter for every wave, no f-waves.
+struct CfgAny {
+    static constexpr bool fixed = false;
+    static constexpr int order = 0, trans = 0, lim = 0;
+};
+template <int ORDER, int TRANS, int LIM>
+struct CfgFix {
+    static constexpr bool fixed = true;
+    static constexpr int order = ORDER, trans = TRANS, lim = LIM;
+};
+
+template <class S, class Cfg>
+__global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(StepArgs a)
+{
+    using Mth = typename S::Math;
+    using Cell = typename S::Cell;
+    using Cache = typename S::Cache;
+    constexpr int M = S::M, MW = S::MW;
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (wid >= a.nstrips * a.nsegs) return;  // uniform per wave
+    const int strip = wid % a.nstrips;
+    const int seg = wid / a.nstrips;
+    const Layout2 L = a.L;
+    const int mx = L.mx, my = L.my, mbc = L.mbc;
+    const int i = kStripOut * strip + lane - 1;
+    const int ic = i < mx + mbc ? i : mx + mbc;  // clamped column for loads
+    const int r0 = 1 + seg * a.seglen;
+    const int r1 = min(r0 + a.seglen, my + 1);  // output rows r0 .. r1-1
+    const bool out_lane = lane >= 2 && lane <= kStripOut + 1 && i <= mx;
+    const bool xcfl_lane = lane >= 1 && i >= 1 && i <= mx + 1;
+    const bool ycfl_lane = i >= 0 && i <= mx + 1;
+    const double dtdx = a.dtdx, dtdy = a.dtdy;
+    const bool second = Cfg::fixed ? Cfg::order == 2 : a.order == 2;
+    const int trn = Cfg::fixed ? Cfg::trans : a.trans;
+    const bool tr2 = second && trn == 2;
+    const bool trans = trn > 0;
+    const bool fwave = Cfg::fixed ? false : a.use_fwave != 0;
+    const double* __restrict__ qo = a.qold + L.off(ic, 0);
+    double* __restrict__ qn = a.qnew + L.off(i, 0);
+    const long pitch = L.pitch, fs = L.fs;
+    const int rowmax = my + mbc;
+    const SolverParams P = a.P;
+    auto limw = [&](int mw) { return Cfg::fixed ? Cfg::lim : a.lim[mw]; };
+
+    // y-direction pipeline state of one interface (between rows J-1, J)
+    struct YS {
+        Cell cell;  // cell (i, J)
+        double w[MW][M], s[MW], am[M], ap[M], dotl[MW];
+        Cache cache;
+    };
+    YS ysA, ysB;
+    {
+        double q0[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) q0[m] = qo[m * fs + (r0 - 2) * pitch];
+        ysA.cell = S::cell(P, q0);
+#pragma unroll
+        for (int mw = 0; mw < MW; mw++) {
+            ysA.s[mw] = ysA.dotl[mw] = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; m++) ysA.w[mw][m] = 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) ysA.am[m] = ysA.ap[m] = 0.0;
+    }
+    double qnx[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) qnx[m] = qo[m * fs + (r0 - 1) * pitch];
+
+    double vA[M], vB[M], g2prev[M];
+    double qa_part[M], fa_part[M], gp1[M], gp2[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        vA[m] = vB[m] = g2prev[m] = 0.0;
+        qa_part[m] = fa_part[m] = gp1[m] = gp2[m] = 0.0;
+    }
+    double cfl = 0.0;
+
+    // One row of the sweep.  yp holds interface J-1 (and cell J-1), yc
+    // receives interface J (and cell J); the caller alternates the two
+    // states so that nothing is copied between iterations.
+    auto row = [&](YS& yp, YS& yc, const int J) {
+        double qc[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) qc[m] = qnx[m];
+        {
+            const long jn = min(J + 1, rowmax);
+#pragma unroll
+            for (int m = 0; m < M; m++) qnx[m] = qo[m * fs + jn * pitch];
+        }
+        yc.cell = S::cell(P, qc);
+        const Cell& cc = yc.cell;
+        double vC[M];
+
+        // ------------------------------------------------ x-row J (flux2, ixy=1)
+        if (J <= r1) {
+            Cell cl;
+            {
+                const double* src = reinterpret_cast<const double*>(&cc);
+                double* dst = reinterpret_cast<double*>(&cl);
+#pragma unroll
+                for (int k = 0; k < S::XSHIFT; k++) dst[k] = lane_up(src[k]);
+            }
+            double wx[MW][M], sx[MW], amx[M], apx[M];
+            Cache cx;
+            S::template rpn<1, 2>(P, cl, cc, wx, sx, amx, apx, cx);
+            if (xcfl_lane) {
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, cfl_term(dtdx, sx[mw]));
+            }
+            double cq[M], fa[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) cq[m] = fa[m] = 0.0;
+            if (second) {
+                double dotr[MW], wn2[MW];
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) {
+                    double dr = 0.0, nn = 0.0;
+#pragma unroll
+                    for (int m = 0; m < M; m++) {
+                        double wr = lane_dn(wx[mw][m]);
+                        nn = nn + wx[mw][m] * wx[mw][m];
+                        dr = dr + wx[mw][m] * wr;
+                    }
+                    dotr[mw] = dr;
+                    wn2[mw] = nn;
+                }
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) {
+                    double dl = lane_up(dotr[mw]);
+                    if (limw(mw) != 0 && wn2[mw] != 0.0) {
+                        double r = Mth::div(sx[mw] > 0.0 ? dl : dotr[mw], wn2[mw]);
+                        double phi = philim<Mth>(r, limw(mw));
+#pragma unroll
+                        for (int m = 0; m < M; m++) wx[mw][m] = phi * wx[mw][m];
+                    }
+                }
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) {
+                    double sa = fabs(sx[mw]);
+                    double sw = fwave ? copysign(1.0, sx[mw]) : sa;
+                    double coef = sw * (1.0 - sa * dtdx);
+#pragma unroll
+                    for (int m = 0; m < M; m++) cq[m] = cq[m] + coef * wx[mw][m];
+                }
+#pragma unroll
+                for (int m = 0; m < M; m++) fa[m] = 0.5 * cq[m];
+            }
+            double G1[M], G2[M];
+            if (trans) {
+                double amt[M], apt[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    amt[m] = tr2 ? amx[m] + cq[m] : amx[m];
+                    apt[m] = tr2 ? apx[m] - cq[m] : apx[m];
+                }
+                double bma[M], bpa[M], bmp[M], bpp[M];
+                S::template rpt<1, 2>(P, cx, amt, bma, bpa);
+                S::template rpt<1, 2>(P, cx, apt, bmp, bpp);
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    double g1n = lane_dn(-0.5 * dtdx * bma[m]);
+                    double g2n = lane_dn(-0.5 * dtdx * bpa[m]);
+                    G1[m] = g1n - 0.5 * dtdx * bmp[m];
+                    G2[m] = g2n - 0.5 * dtdx * bpp[m];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < M; m++) G1[m] = G2[m] = 0.0;
+            }
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                double amn = lane_dn(amx[m]);
+                double fan = lane_dn(fa[m]);
+                double qadd = (0.0 - dtdx * apx[m]) - dtdx * amn;
+                double v = qc[m] + dtdy * g2prev[m];
+                v = v + qadd - dtdx * (fan - fa[m]) - dtdy * (G2[m] - G1[m]);
+                vC[m] = v;
+                vB[m] = vB[m] - dtdy * G1[m];
+                g2prev[m] = G2[m];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < M; m++) vC[m] = 0.0;
+        }
+
+        // ------------------------------ y-interface J (flux2, ixy=2) on column i
+        S::template rpn<2, 1>(P, yp.cell, cc, yc.w, yc.s, yc.am, yc.ap, yc.cache);
+        if (ycfl_lane && J >= 1 && J <= my + 1) {
+#pragma unroll
+            for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, cfl_term(dtdy, yc.s[mw]));
+        }
+#pragma unroll
+        for (int mw = 0; mw < MW; mw++) {
+            double dr = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; m++) dr = dr + yp.w[mw][m] * yc.w[mw][m];
+            yc.dotl[mw] = dr;  // dotr of interface J-1 = dotl of interface J
+        }
+        if (J - 1 >= r0) {
+            // limit interface J-1 (held in yp) with its neighbours J-2 and J
+            double cq[M], fa[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) cq[m] = fa[m] = 0.0;
+            if (second) {
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) {
+                    double nn = 0.0;
+#pragma unroll
+                    for (int m = 0; m < M; m++) nn = nn + yp.w[mw][m] * yp.w[mw][m];
+                    if (limw(mw) != 0 && nn != 0.0) {
+                        double r = Mth::div(yp.s[mw] > 0.0 ? yp.dotl[mw] : yc.dotl[mw], nn);
+                        double phi = philim<Mth>(r, limw(mw));
+#pragma unroll
+                        for (int m = 0; m < M; m++) yp.w[mw][m] = phi * yp.w[mw][m];
+                    }
+                }
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) {
+                    double sa = fabs(yp.s[mw]);
+                    double sw = fwave ? copysign(1.0, yp.s[mw]) : sa;
+                    double coef = sw * (1.0 - sa * dtdy);
+#pragma unroll
+                    for (int m = 0; m < M; m++) cq[m] = cq[m] + coef * yp.w[mw][m];
+                }
+#pragma unroll
+                for (int m = 0; m < M; m++) fa[m] = 0.5 * cq[m];
+            }
+            double bma[M], bpa[M], bmp[M], bpp[M];
+            if (trans) {
+                double amt[M], apt[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    amt[m] = tr2 ? yp.am[m] + cq[m] : yp.am[m];
+                    apt[m] = tr2 ? yp.ap[m] - cq[m] : yp.ap[m];
+                }
+                S::template rpt<2, 1>(P, yp.cache, amt, bma, bpa);
+                S::template rpt<2, 1>(P, yp.cache, apt, bmp, bpp);
+            } else {
+#pragma unroll
+                for (int m = 0; m < M; m++) bma[m] = bpa[m] = bmp[m] = bpp[m] = 0.0;
+            }
+            if (J - 2 >= r0) {
+                // cell (i, J-2): its y-sweep terms are now complete
+                const bool store = out_lane && J - 2 < r1;
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    double qadd = qa_part[m] - dtdy * yp.am[m];
+                    double dF = fa[m] - fa_part[m];
+                    double G1 = trans ? (-0.5 * dtdy * bma[m]) - gp1[m] : 0.0;
+                    double G2 = trans ? (-0.5 * dtdy * bpa[m]) - gp2[m] : 0.0;
+                    double g2l = lane_up(G2);
+                    double g1r = lane_dn(G1);
+                    double v = vA[m] + dtdx * g2l;
+                    v = v + (qadd - dtdy * dF - dtdx * (G2 - G1));
+                    v = v - dtdx * g1r;
+                    if (store) qn[m * fs + (long)(J - 2) * pitch] = v;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                qa_part[m] = 0.0 - dtdy * yp.ap[m];
+                fa_part[m] = fa[m];
+                gp1[m] = 0.5 * dtdy * bmp[m];
+                gp2[m] = 0.5 * dtdy * bpp[m];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            vA[m] = vB[m];
+            vB[m] = vC[m];
+        }
+    };
+
+    // rows r0-1 .. r1+1, padded to an even count (a padded last row only
+    // recomputes in-bounds data and stores nothing)
+    const int nrow = r1 - r0 + 3;
+    const int jend = r0 - 1 + nrow + (nrow & 1);
+    for (int J = r0 - 1; J < jend; J += 2) {
+        row(ysA, ysB, J);
+        row(ysB, ysA, J + 1);
+    }
+    publish_cfl(cfl, a.cfl_bits);
+}
+
+template <class S>
+static void launch_cfg(const StepArgs& a, int nblocks, hipStream_t s)
+{
+    bool mc = a.order == 2 && a.trans == 2 && !a.use_fwave;
+    for (int k = 0; k < S::MW; k++) mc = mc && a.lim[k] == 4;
+    if (mc)
+        hipLaunchKernelGGL((step2_unsplit_kernel<S, CfgFix<2, 2, 4>>), dim3(nblocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((step2_unsplit_kernel<S, CfgAny>), dim3(nblocks), dim3(256), 0, s, a);
+}
+
+template <class Mth>
+static int launch_step2_impl(int solver, const StepArgs& a, int nblocks, hipStream_t s)
+{
+    if (solver == 1)
+        launch_cfg<Acoustics2<Mth>>(a, nblocks, s);
+    else
+        launch_cfg<Euler2<Mth>>(a, nblocks, s);
+    return (int)hipGetLastError();
+}
+
+}  // namespace clawgpu

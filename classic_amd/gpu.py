@@ -43,7 +43,7 @@ def solver_params(solver_id, rho=1.0, bulk=4.0, gamma=1.4, efix=True):
 class Grid2D:
     """Device-resident single grid (one clawgpu context)."""
 
-    def __init__(self, solver_id, mx, my, params, mbc=2, maux=0):
+    def __init__(self, solver_id, mx, my, params, mbc=2, maux=0, arith=0):
         self.lib = _lib.load()
         meqn, mwaves = SOLVER_SHAPE[solver_id]
         self.solver_id, self.mx, self.my, self.mbc = solver_id, mx, my, mbc
@@ -53,6 +53,8 @@ class Grid2D:
         check(self.lib.clawgpu_create(C.byref(h), meqn, mwaves, maux, mbc, mx, my, solver_id,
                                       _dp(p), p.size))
         self.h = h
+        if arith:
+            self.set_arith(arith)
 
     def close(self):
         if getattr(self, "h", None):
@@ -118,6 +120,10 @@ class Grid2D:
     def set_segment_rows(self, rows):
         check(self.lib.clawgpu_set_segment_rows(self.h, int(rows)))
 
+    def set_arith(self, mode):
+        """0 = exact reference arithmetic (default), 1 = fast (FMA)."""
+        check(self.lib.clawgpu_set_arith(self.h, int(mode)))
+
     def time_steps(self, dx, dy, dt, method, mthlim, mthbc, nsteps, use_fwave=False):
         """nsteps fixed-dt steps timed with HIP events on the library's
         stream; returns (total_ms, step_kernel_ms, cfl_max)."""
@@ -131,13 +137,14 @@ class Grid2D:
         return ms.value, kms.value, cfl.value
 
 
-def step2(solver_id, params, qold, dx, dy, dt, method, mthlim, use_fwave=False, mbc=2):
+def step2(solver_id, params, qold, dx, dy, dt, method, mthlim, use_fwave=False, mbc=2,
+          arith=0):
     """One step with the reference step2 contract on host arrays: returns
     (qnew, cfl).  qold must have its ghost cells set; ghost cells of qnew are
     copied from qold (the reference leaves them partially updated)."""
     qold = np.ascontiguousarray(qold, dtype=np.float64)
     my2, mx2, meqn = qold.shape
-    with Grid2D(solver_id, mx2 - 2 * mbc, my2 - 2 * mbc, params, mbc=mbc) as g:
+    with Grid2D(solver_id, mx2 - 2 * mbc, my2 - 2 * mbc, params, mbc=mbc, arith=arith) as g:
         g.set_q(qold)
         cfl = g.step2(dx, dy, dt, method, mthlim, use_fwave)
         qn = g.get_q()
