@@ -521,3 +521,44 @@ extern "C" int clawgpu_time_steps(clawgpu_ctx* c, double dx, double dy, double d
     }
     return CLAWGPU_OK;
 }
+
+// Solver-level evaluation for tests (no reference counterpart): n interfaces
+// with host arrays ql, qr, asdq (n*meqn) -> wave (n*mwaves*meqn), s
+// (n*mwaves), amdq, apdq, bmasdq, bpasdq (n*meqn), all in the reference's
+// per-interface layout.
+extern "C" int clawgpu_solver_probe(clawgpu_ctx* c, int ixy, int n, const double* ql,
+                                    const double* qr, const double* asdq, double* wave, double* s,
+                                    double* amdq, double* apdq, double* bmasdq, double* bpasdq)
+{
+    if (!c || n < 1 || !ql || !qr || !asdq || !wave || !s || !amdq || !apdq || !bmasdq || !bpasdq)
+        return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    if (ixy != 1 && ixy != 2) return set_err(CLAWGPU_ERR_ARG, "ixy must be 1 or 2");
+    const int M = c->meqn, W = c->mwaves;
+    size_t nq = (size_t)n * M, nw = (size_t)n * W;
+    double* d = nullptr;
+    size_t tot = 3 * nq + nw * M + nw + 4 * nq;
+    HIPCHK(c, hipMalloc((void**)&d, tot * sizeof(double)));
+    double *dql = d, *dqr = d + nq, *das = d + 2 * nq, *dw = d + 3 * nq, *ds = dw + nw * M;
+    double *dam = ds + nw, *dap = dam + nq, *dbm = dap + nq, *dbp = dbm + nq;
+    int rc = CLAWGPU_OK;
+    hipError_t e;
+    if ((e = hipMemcpy(dql, ql, nq * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(dqr, qr, nq * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(das, asdq, nq * 8, hipMemcpyHostToDevice)) != hipSuccess) {
+        rc = set_err(CLAWGPU_ERR_HIP, hipGetErrorString(e));
+    } else {
+        int le = c->arith == ARITH_FAST
+                     ? launch_probe_fast(c->solver, n, ixy, c->P, dql, dqr, das, dw, ds, dam, dap, dbm, dbp, c->stream)
+                     : launch_probe_exact(c->solver, n, ixy, c->P, dql, dqr, das, dw, ds, dam, dap, dbm, dbp, c->stream);
+        if (le != 0 || (e = hipStreamSynchronize(c->stream)) != hipSuccess ||
+            (e = hipMemcpy(wave, dw, nw * M * 8, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(s, ds, nw * 8, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(amdq, dam, nq * 8, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(apdq, dap, nq * 8, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(bmasdq, dbm, nq * 8, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(bpasdq, dbp, nq * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+            rc = set_err(CLAWGPU_ERR_HIP, "solver probe failed");
+    }
+    (void)hipFree(d);
+    return rc;
+}

@@ -46,4 +46,11 @@ enum { ARITH_EXACT = 0, ARITH_FAST = 1 };
 int launch_step2_exact(int solver, const StepArgs& a, int nblocks, hipStream_t s);
 int launch_step2_fast(int solver, const StepArgs& a, int nblocks, hipStream_t s);
 
+int launch_probe_exact(int solver, int n, int ixy, const SolverParams& P, const double* ql,
+                       const double* qr, const double* asdq, double* wave, double* s,
+                       double* amdq, double* apdq, double* bm, double* bp, hipStream_t st);
+int launch_probe_fast(int solver, int n, int ixy, const SolverParams& P, const double* ql,
+                      const double* qr, const double* asdq, double* wave, double* s,
+                      double* amdq, double* apdq, double* bm, double* bp, hipStream_t st);
+
 }  // namespace clawgpu

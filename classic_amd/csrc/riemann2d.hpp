@@ -264,6 +264,55 @@ struct Euler2 {
         w[2][3] = a4 * (r.enth + r.u * r.a);
         s[2] = r.u + r.a;
 
+        if constexpr (Mth::fast) {
+            if (P.efix) {
+                // Harten-Hyman as below, with the transonic tests done on
+                // signs: s1 = u_m - c_m > 0 iff rho u_m > 0 and
+                // (rho u_m)^2 > gamma rho_m p_m, which needs no sqrt or
+                // division; the sonic speeds are evaluated only when a
+                // transonic rarefaction is present.
+                const double gamma = P.gamma, gg1 = P.gamma * g1;
+                const double s0 = L.vel[MU - 1] - L.c;
+                const bool super = s0 >= 0.0 && s[0] > 0.0;
+                double qm0 = qa[0] + w[0][0], qmu = qa[MU] + w[0][MU];
+                double qmv = qa[MV] + w[0][MV], qme = qa[3] + w[0][3];
+                double km = qmu * qmu;
+                double tm = gg1 * (qme * qm0 - 0.5 * (km + qmv * qmv));
+                bool tr1 = s0 < 0.0 && qmu > 0.0 && tm >= 0.0 && km > tm;
+                double f1 = s[0] < 0.0 ? s[0] : 0.0;
+                if (tr1) {
+                    double ri = Mth::rcp(qm0);
+                    double pm = g1 * (qme - 0.5 * (km + qmv * qmv) * ri);
+                    double s1 = qmu * ri - Mth::sqrt_(gamma * pm * ri);
+                    f1 = s0 * (s1 - s[0]) / (s1 - s0);
+                }
+                const bool left2 = !super && s[1] < 0.0;
+                const double s3 = R.vel[MU - 1] + R.c;
+                double qn0 = qb[0] - w[2][0], qnu = qb[MU] - w[2][MU];
+                double qnv = qb[MV] - w[2][MV], qne = qb[3] - w[2][3];
+                double kn = qnu * qnu;
+                double tn = gg1 * (qne * qn0 - 0.5 * (kn + qnv * qnv));
+                bool tr3 = left2 && s3 > 0.0 && qnu < 0.0 && tn >= 0.0 && kn > tn;
+                double f3 = s[2] < 0.0 ? s[2] : 0.0;
+                if (tr3) {
+                    double ri = Mth::rcp(qn0);
+                    double pn = g1 * (qne - 0.5 * (kn + qnv * qnv) * ri);
+                    double s2 = qnu * ri + Mth::sqrt_(gamma * pn * ri);
+                    f3 = s2 * (s3 - s[2]) / (s3 - s2);
+                }
+                f1 = super ? 0.0 : f1;
+                const double f2 = left2 ? s[1] : 0.0;
+                f3 = left2 ? f3 : 0.0;
+                const double g1c = s[0] - f1, g2c = s[1] - f2, g3c = s[2] - f3;
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    am[m] = f1 * w[0][m] + f2 * w[1][m] + f3 * w[2][m];
+                    ap[m] = g1c * w[0][m] + g2c * w[1][m] + g3c * w[2][m];
+                }
+                return;
+            }
+        }
+
         if (!P.efix) {
 #pragma unroll
             for (int m = 0; m < M; m++) {
@@ -352,6 +401,36 @@ struct Euler2 {
     __device__ static inline void rpt(const SolverParams&, const Cache& r, const double* asdq,
                                       double bm[M], double bp[M])
     {
+        if constexpr (Mth::fast) {
+            // the same split written on the eigen-coefficients: with
+            // c_p = min(s_p, 0) * alpha_p (resp. max for bpasdq)
+            double d1 = asdq[0], d2 = asdq[MU], d3 = asdq[MV], d4 = asdq[3];
+            double a3 = r.g1a2 * (r.euv * d1 + r.u * d2 + r.v * d3 - d4);
+            double a2 = d2 - r.u * d1;
+            double a4 = (d3 + (r.a - r.v) * d1 - r.a * a3) * r.h2a;
+            double a1 = d1 - a3 - a4;
+            const double l1 = r.v - r.a, l2 = r.v, l3 = r.v + r.a;
+            const double va = r.v * r.a, hk = 0.5 * r.u2v2;
+            {
+                double m1 = fmin(l1, 0.0) * a1, m2 = fmin(l2, 0.0), m3 = fmin(l3, 0.0) * a4;
+                double S = m1 + m3, D = m3 - m1, e2 = m2 * a3, s2 = m2 * a2;
+                double Se = S + e2;
+                bm[0] = Se;
+                bm[MU] = Se * r.u + s2;
+                bm[MV] = Se * r.v + D * r.a;
+                bm[3] = S * r.enth + D * va + e2 * hk + s2 * r.u;
+            }
+            {
+                double m1 = fmax(l1, 0.0) * a1, m2 = fmax(l2, 0.0), m3 = fmax(l3, 0.0) * a4;
+                double S = m1 + m3, D = m3 - m1, e2 = m2 * a3, s2 = m2 * a2;
+                double Se = S + e2;
+                bp[0] = Se;
+                bp[MU] = Se * r.u + s2;
+                bp[MV] = Se * r.v + D * r.a;
+                bp[3] = S * r.enth + D * va + e2 * hk + s2 * r.u;
+            }
+            return;
+        }
         double d1 = asdq[0], d2 = asdq[MU], d3 = asdq[MV], d4 = asdq[3];
         double a3 = r.g1a2 * (r.euv * d1 + r.u * d2 + r.v * d3 - d4);
         double a2 = d2 - r.u * d1;

@@ -8,4 +8,10 @@ int launch_step2_exact(int solver, const StepArgs& a, int nblocks, hipStream_t s
 {
     return launch_step2_impl<ExactMath>(solver, a, nblocks, s);
 }
+int launch_probe_exact(int solver, int n, int ixy, const SolverParams& P, const double* ql,
+                      const double* qr, const double* asdq, double* wave, double* s, double* amdq,
+                      double* apdq, double* bm, double* bp, hipStream_t st)
+{
+    return launch_probe_impl<ExactMath>(solver, n, ixy, P, ql, qr, asdq, wave, s, amdq, apdq, bm, bp, st);
+}
 }  // namespace clawgpu

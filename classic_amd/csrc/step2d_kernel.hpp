@@ -381,6 +381,56 @@ __global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(S
     publish_cfl(cfl, a.cfl_bits);
 }
 
+// Solver-level evaluation for tests: interface k between states ql[k]
+// (left, reference layout q(meqn)) and qr[k]; outputs wave(meqn, mwaves),
+// s(mwaves), amdq, apdq, and the transverse split of asdq[k] (imp = 1 uses
+// amdq-like data, the split does not depend on imp) into bmasdq, bpasdq.
+template <class S>
+__global__ void solver_probe_kernel(int n, int ixy, SolverParams P, const double* ql,
+                                    const double* qr, const double* asdq, double* wave,
+                                    double* s, double* amdq, double* apdq, double* bm,
+                                    double* bp)
+{
+    constexpr int M = S::M, MW = S::MW;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    typename S::Cell L = S::cell(P, ql + k * M), R = S::cell(P, qr + k * M);
+    double w[MW][M], sp[MW], am[M], ap[M], b0[M], b1[M];
+    typename S::Cache c;
+    if (ixy == 1) {
+        S::template rpn<1, 2>(P, L, R, w, sp, am, ap, c);
+        S::template rpt<1, 2>(P, c, asdq + k * M, b0, b1);
+    } else {
+        S::template rpn<2, 1>(P, L, R, w, sp, am, ap, c);
+        S::template rpt<2, 1>(P, c, asdq + k * M, b0, b1);
+    }
+    for (int mw = 0; mw < MW; mw++) {
+        s[k * MW + mw] = sp[mw];
+        for (int m = 0; m < M; m++) wave[(k * MW + mw) * M + m] = w[mw][m];
+    }
+    for (int m = 0; m < M; m++) {
+        amdq[k * M + m] = am[m];
+        apdq[k * M + m] = ap[m];
+        bm[k * M + m] = b0[m];
+        bp[k * M + m] = b1[m];
+    }
+}
+
+template <class Mth>
+static int launch_probe_impl(int solver, int n, int ixy, const SolverParams& P, const double* ql,
+                             const double* qr, const double* asdq, double* wave, double* s,
+                             double* amdq, double* apdq, double* bm, double* bp, hipStream_t st)
+{
+    int nb = (n + 127) / 128;
+    if (solver == 1)
+        hipLaunchKernelGGL(solver_probe_kernel<Acoustics2<Mth>>, dim3(nb), dim3(128), 0, st, n, ixy,
+                           P, ql, qr, asdq, wave, s, amdq, apdq, bm, bp);
+    else
+        hipLaunchKernelGGL(solver_probe_kernel<Euler2<Mth>>, dim3(nb), dim3(128), 0, st, n, ixy, P,
+                           ql, qr, asdq, wave, s, amdq, apdq, bm, bp);
+    return (int)hipGetLastError();
+}
+
 template <class S>
 static void launch_cfg(const StepArgs& a, int nblocks, hipStream_t s)
 {

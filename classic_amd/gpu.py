@@ -117,6 +117,21 @@ class Grid2D:
         check(rc)
         return dict(info=info.value, tend=te.value, dtv=dtv_a, cflv=cflv_a, nv=nv)
 
+    def solver_probe(self, ixy, ql, qr, asdq):
+        """Device rpn2/rpt2 on interfaces ql[k] | qr[k] (arrays (n, meqn));
+        returns wave (n, mwaves, meqn), s (n, mwaves), amdq, apdq, bmasdq,
+        bpasdq (n, meqn)."""
+        ql = np.ascontiguousarray(ql, dtype=np.float64)
+        qr = np.ascontiguousarray(qr, dtype=np.float64)
+        asdq = np.ascontiguousarray(asdq, dtype=np.float64)
+        n = ql.shape[0]
+        wave = np.empty((n, self.mwaves, self.meqn))
+        s = np.empty((n, self.mwaves))
+        out = [np.empty((n, self.meqn)) for _ in range(4)]
+        check(self.lib.clawgpu_solver_probe(self.h, ixy, n, _dp(ql), _dp(qr), _dp(asdq), _dp(wave),
+                                            _dp(s), *[_dp(o) for o in out]))
+        return (wave, s, *out)
+
     def set_segment_rows(self, rows):
         check(self.lib.clawgpu_set_segment_rows(self.h, int(rows)))
 

@@ -98,6 +98,16 @@ int clawgpu_time_steps(clawgpu_ctx* ctx, double dx, double dy, double dt, const 
                        const int* mthlim, const int* mthbc, int use_fwave, int nsteps,
                        float* total_ms, float* step_kernel_ms, double* cfl_max);
 
+/* Solver-level evaluation for tests (no reference counterpart): the device
+ * rpn2 on n interfaces between ql[k] and qr[k] (each meqn doubles) and the
+ * device rpt2 split of asdq[k] with that interface's eigenstructure, in the
+ * context's solver and arithmetic mode.  Outputs follow the reference's
+ * per-interface layout: wave(meqn, mwaves), s(mwaves), amdq, apdq, bmasdq,
+ * bpasdq (meqn each).  ixy = 1 for an x-slice, 2 for a y-slice. */
+int clawgpu_solver_probe(clawgpu_ctx* ctx, int ixy, int n, const double* ql, const double* qr,
+                         const double* asdq, double* wave, double* s, double* amdq, double* apdq,
+                         double* bmasdq, double* bpasdq);
+
 #ifdef __cplusplus
 }
 #endif

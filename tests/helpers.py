@@ -8,13 +8,22 @@ def interior(q, mbc=2):
     return q[mbc:-mbc, mbc:-mbc]
 
 
+FIELD_FLOOR = 1e-3
+
+
 def max_rel_err(a, b, mbc=2):
-    """Per-field max |a-b| / max|b| over the interior, maximised over fields."""
+    """Max relative error per field over the interior, maximised over fields:
+    max|a_m - b_m| / scale_m with scale_m = max|b_m|.  A field that stays
+    below FIELD_FLOOR times the largest field (e.g. the transverse momentum
+    of a one-dimensional problem, which is zero up to round-off) has no
+    scale of its own; it is measured against FIELD_FLOOR times the largest
+    field instead, otherwise the ratio of two round-off noises is reported."""
     a = interior(a, mbc)
     b = interior(b, mbc)
+    top = max(np.abs(b[..., m]).max() for m in range(b.shape[-1]))
     worst = 0.0
     for m in range(a.shape[-1]):
-        scale = np.abs(b[..., m]).max()
+        scale = max(np.abs(b[..., m]).max(), FIELD_FLOOR * top)
         d = np.abs(a[..., m] - b[..., m]).max()
         worst = max(worst, d / scale if scale > 0 else d)
     return worst

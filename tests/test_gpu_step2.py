@@ -259,3 +259,47 @@ def test_fast_vs_exact_full_size_200_steps():
     dterr = abs(o1["dtv"][4] - o0["dtv"][4]) / o0["dtv"][4]
     print("fast vs exact 4096^2 x 200:", err, dterr)
     assert err <= TOL and dterr <= TOL
+
+
+def _toro1(mx, my, along_y=False):
+    """Toro's test 1 (left rho=1, u=0.75, p=1; right rho=0.125, u=0, p=0.1):
+    the left rarefaction is transonic, so the entropy fix's transonic branch
+    runs at the interfaces inside the fan."""
+    g = 1.4
+    q = np.zeros((my + 4, mx + 4, 4))
+    n = my + 4 if along_y else mx + 4
+    x = (np.arange(n) - 1.5) / (n - 4)
+    left = x < 0.3
+    rho = np.where(left, 1.0, 0.125)
+    u = np.where(left, 0.75, 0.0)
+    p = np.where(left, 1.0, 0.1)
+    E = p / (g - 1) + 0.5 * rho * u * u
+    if along_y:
+        q[..., 0] = rho[:, None]
+        q[..., 2] = (rho * u)[:, None]
+        q[..., 3] = E[:, None]
+    else:
+        q[..., 0] = rho[None, :]
+        q[..., 1] = (rho * u)[None, :]
+        q[..., 3] = E[None, :]
+    return q, 1.0 / mx, 1.0 / my
+
+
+@pytest.mark.parametrize("along_y", [False, True])
+@pytest.mark.parametrize("arith", [0, 1])
+def test_transonic_rarefaction(along_y, arith):
+    mx, my = (120, 40) if not along_y else (40, 120)
+    q, dx, dy = _toro1(mx, my, along_y)
+    method, mthlim, mthbc = (1, 2, 2, 0, 0, 0, 0), [4, 4, 4], (1, 1, 1, 1)
+    p = _oracle_problem(SOLVER_EULER, mx, my, method, mthlim, mthbc)
+    r = O.claw2(p, q, dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 60)
+    with gpu.Grid2D(SOLVER_EULER, mx, my, [1.4, 1.0], arith=arith) as g:
+        g.set_q(q)
+        out = g.claw2(dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 60, method, mthlim, mthbc)
+        q_g = g.get_q()
+    assert out["nv"][1] == r.nv[1] == 60
+    err = max_rel_err(q_g, r.q)
+    print("toro1", along_y, arith, err)
+    if arith == 0:
+        assert np.array_equal(q_g[2:-2, 2:-2], r.q[2:-2, 2:-2])
+    assert err <= TOL
