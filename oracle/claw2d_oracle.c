@@ -409,11 +409,15 @@ static int orc_flux2(int ixy, int maxm, int meqn, int mwaves, int maux, int mbc,
     }
 
     double cfl = 0.0;
+    int nan_speed = 0;
     for (int i = 1; i <= mx + 1; i++)
         for (int mw = 1; mw <= mwaves; mw++) {
             cfl = fmax(cfl, fmax(DT(i) * SS(s, mw, i), -DT(i - 1) * SS(s, mw, i)));
+            if (SS(s, mw, i) != SS(s, mw, i)) nan_speed = 1;
         }
-    *cfl1d = cfl;
+    /* Not in the reference: a NaN speed is reported as a NaN Courant number
+     * (fmax would drop it) so that the time loop can stop, see orc_claw2. */
+    *cfl1d = nan_speed ? NAN : cfl;
 
     if (method[1] != 1) {
         if (limit)
@@ -537,7 +541,7 @@ static int orc_step2_w(int maxm, int meqn, int mwaves, int maux, int mbc, int mx
                       sw->aux3, method, mthlim, qadd, fadd, gadd, &cfl1d, &sw->fw, rpn2, rpt2,
                       use_fwave) != 0)
             return -1;
-        cfl = fmax(cfl, cfl1d);
+        cfl = (cfl1d != cfl1d || cfl != cfl) ? NAN : fmax(cfl, cfl1d);
         if (index_capa == 0) {
             for (int i = 1; i <= mx; i++)
                 for (int m = 1; m <= meqn; m++) {
@@ -579,7 +583,7 @@ static int orc_step2_w(int maxm, int meqn, int mwaves, int maux, int mbc, int mx
                       sw->aux3, method, mthlim, qadd, fadd, gadd, &cfl1d, &sw->fw, rpn2, rpt2,
                       use_fwave) != 0)
             return -1;
-        cfl = fmax(cfl, cfl1d);
+        cfl = (cfl1d != cfl1d || cfl != cfl) ? NAN : fmax(cfl, cfl1d);
         if (index_capa == 0) {
             for (int j = 1; j <= my; j++)
                 for (int m = 1; m <= meqn; m++) {
@@ -652,7 +656,7 @@ static int orc_step2ds_w(int maxm, int meqn, int mwaves, int maux, int mbc, int 
                           sw->aux2, sw->aux3, method, mthlim, qadd, fadd, gadd, &cfl1d, &sw->fw,
                           rpn2, rpt2, use_fwave) != 0)
                 return -1;
-            cfl = fmax(cfl, cfl1d);
+            cfl = (cfl1d != cfl1d || cfl != cfl) ? NAN : fmax(cfl, cfl1d);
             for (int i = 1; i <= mx; i++)
                 for (int m = 1; m <= meqn; m++) {
                     if (index_capa == 0)
@@ -689,7 +693,7 @@ static int orc_step2ds_w(int maxm, int meqn, int mwaves, int maux, int mbc, int 
                           sw->aux2, sw->aux3, method, mthlim, qadd, fadd, gadd, &cfl1d, &sw->fw,
                           rpn2, rpt2, use_fwave) != 0)
                 return -1;
-            cfl = fmax(cfl, cfl1d);
+            cfl = (cfl1d != cfl1d || cfl != cfl) ? NAN : fmax(cfl, cfl1d);
             for (int j = 1; j <= my; j++)
                 for (int m = 1; m <= meqn; m++) {
                     if (index_capa == 0)
@@ -719,14 +723,14 @@ static int orc_dimsp2_w(int maxm, int meqn, int mwaves, int maux, int mbc, int m
     if (orc_step2ds_w(maxm, meqn, mwaves, maux, mbc, mx, my, qold, qnew, aux, dx, dy, dtx,
                       method, mthlim, &cflx, 1, use_fwave, rpn2, rpt2, sw) != 0)
         return -1;
-    if (cflx > cflv[0]) {
+    if (cflx > cflv[0] || cflx != cflx) {
         *cfl = cflx;
         return 0;
     }
     if (orc_step2ds_w(maxm, meqn, mwaves, maux, mbc, mx, my, qnew, qnew, aux, dx, dy, dt,
                       method, mthlim, &cfly, 2, use_fwave, rpn2, rpt2, sw) != 0)
         return -1;
-    *cfl = fmax(cflx, cfly);
+    *cfl = (cflx != cflx || cfly != cfly) ? NAN : fmax(cflx, cfly);
     if (method[2] == -2) {
         if (cfly > cflv[0]) {
             *cfl = cfly;
@@ -735,7 +739,7 @@ static int orc_dimsp2_w(int maxm, int meqn, int mwaves, int maux, int mbc, int m
         if (orc_step2ds_w(maxm, meqn, mwaves, maux, mbc, mx, my, qnew, qnew, aux, dx, dy, dt2,
                           method, mthlim, &cflx, 1, use_fwave, rpn2, rpt2, sw) != 0)
             return -1;
-        *cfl = fmax(*cfl, cflx);
+        *cfl = (cflx != cflx || *cfl != *cfl) ? NAN : fmax(*cfl, cflx);
     }
     return 0;
 }
