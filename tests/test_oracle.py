@@ -98,7 +98,8 @@ def test_golden_bc2():
 
 
 @pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref not present")
-@pytest.mark.parametrize("seed,mthbc", [(6, (1, 3, 2, 2)), (7, (1, 1, 1, 1)), (2, (3, 3, 3, 3))])
+@pytest.mark.parametrize("seed,mthbc", [(6, (1, 3, 2, 2)), (7, (1, 1, 1, 1)), (3, (3, 3, 3, 3)),
+                                        (4, (2, 2, 3, 3))])
 def test_live_reference_agreement(seed, mthbc):
     """Against the reference build itself.  The configurations keep the Roe
     solver in physical states: the reference spins forever in its retake
