@@ -41,6 +41,7 @@ _SIGS = {
     "clawgpu_get_q": ([_V, _D], _i),
     "clawgpu_bc2": ([_V, _I], _i),
     "clawgpu_step2": ([_V, _d, _d, _d, _I, _I, _i, _D], _i),
+    "clawgpu_dimsp2": ([_V, _d, _d, _d, _I, _I, _D, _i, _D], _i),
     "clawgpu_claw2": ([_V, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
     "clawgpu_set_segment_rows": ([_V, _i], _i),
     "clawgpu_set_arith": ([_V, _i], _i),

@@ -12,6 +12,7 @@
 #include <new>
 
 #include "common.hpp"
+#include "split2d_kernel.hpp"
 #include "../../include/clawgpu.h"
 
 namespace clawgpu {
@@ -58,7 +59,8 @@ __global__ void bc2_kernel(BcArgs a)
     const int nx = L.mx + 2 * L.mbc, ny = L.my + 2 * L.mbc;
     const int ng = 2 * L.mbc * nx + 2 * L.mbc * L.my;  // ghost rows + ghost columns of interior rows
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0 && a.cfl_bits) *a.cfl_bits = 0ull;
+    if (t == 0 && a.cfl_bits)
+        for (int k = 0; k < 4; k++) a.cfl_bits[k] = 0ull;
     if (t >= ng) return;
     int i, j;
     if (t < 2 * L.mbc * nx) {
@@ -116,7 +118,7 @@ struct clawgpu_ctx {
     int meqn, mwaves, maux, mbc, mx, my, solver;
     SolverParams P;
     Layout2 L;
-    double* dq[2];
+    double* dq[3];  // q and two work copies: qold/qnew of a step, sweeps of a split step
     int cur;
     double* staging;
     unsigned long long* dcfl;
@@ -215,15 +217,17 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc((void**)&c->dq[0], qbytes)) != hipSuccess ||
         (e = hipMalloc((void**)&c->dq[1], qbytes)) != hipSuccess ||
+        (e = hipMalloc((void**)&c->dq[2], qbytes)) != hipSuccess ||
         (e = hipMalloc((void**)&c->staging, abytes)) != hipSuccess ||
-        (e = hipMalloc((void**)&c->dcfl, sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipHostMalloc((void**)&c->hcfl, sizeof(double), hipHostMallocDefault)) != hipSuccess) {
+        (e = hipMalloc((void**)&c->dcfl, 4 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipHostMalloc((void**)&c->hcfl, 4 * sizeof(double), hipHostMallocDefault)) != hipSuccess) {
         snprintf(g_errbuf, sizeof g_errbuf, "allocation failed: %s", hipGetErrorString(e));
         clawgpu_destroy(c);
         return CLAWGPU_ERR_HIP;
     }
     (void)hipMemsetAsync(c->dq[0], 0, qbytes, c->stream);
     (void)hipMemsetAsync(c->dq[1], 0, qbytes, c->stream);
+    (void)hipMemsetAsync(c->dq[2], 0, qbytes, c->stream);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
         snprintf(g_errbuf, sizeof g_errbuf, "init failed: %s", hipGetErrorString(e));
         clawgpu_destroy(c);
@@ -239,6 +243,7 @@ extern "C" int clawgpu_destroy(clawgpu_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->dq[0]) (void)hipFree(c->dq[0]);
     if (c->dq[1]) (void)hipFree(c->dq[1]);
+    if (c->dq[2]) (void)hipFree(c->dq[2]);
     if (c->staging) (void)hipFree(c->staging);
     if (c->dcfl) (void)hipFree(c->dcfl);
     if (c->hcfl) (void)hipHostFree(c->hcfl);
@@ -333,7 +338,7 @@ static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const i
 {
     StepArgs a;
     a.qold = c->dq[c->cur];
-    a.qnew = c->dq[1 - c->cur];
+    a.qnew = c->dq[(c->cur + 1) % 3];
     a.L = c->L;
     a.dtdx = dt / dx;
     a.dtdy = dt / dy;
@@ -373,7 +378,97 @@ extern "C" int clawgpu_step2(clawgpu_ctx* c, double dx, double dy, double dt, co
     if (rc) return rc;
     rc = read_cfl(c, cfl);
     if (rc) return rc;
-    c->cur = 1 - c->cur;
+    c->cur = (c->cur + 1) % 3;
+    return CLAWGPU_OK;
+}
+
+// One dimensionally split step (dimsp2, src/2d/dimsp2.f:4-98) from dq[cur]:
+// Godunov x(dt) -> y(dt), or Strang x(dt/2) -> y(dt) -> x(dt/2).  Every
+// sweep runs; each writes its own Courant number (slots 0, 1, 2) and
+// split_cfl() applies dimsp2's early-abort rule to them, which gives the
+// reference's cfl for the step (a step the reference abandons half way is
+// rejected, so the extra sweep's result is never used).  *result receives
+// the buffer holding the new q.
+static int enqueue_sweep(clawgpu_ctx* c, int dir, int from, int to, double dtd, const int* method,
+                         const int* mthlim, int use_fwave, int slot)
+{
+    SweepArgs a;
+    a.qin = c->dq[from];
+    a.qout = c->dq[to];
+    a.L = c->L;
+    a.dtd = dtd;
+    a.order = method[1];
+    a.use_fwave = use_fwave ? 1 : 0;
+    for (int k = 0; k < kMaxWaves; k++) a.lim[k] = k < c->mwaves ? mthlim[k] : 0;
+    a.seglen = c->seglen;
+    if (dir == 1) {
+        a.nstrips = (c->mx + kStripOut - 1) / kStripOut;
+        a.nsegs = (c->my + 2 * c->mbc + a.seglen - 1) / a.seglen;
+    } else {
+        a.nstrips = (c->mx + 2 * c->mbc + 63) / 64;
+        a.nsegs = (c->my + a.seglen - 1) / a.seglen;
+    }
+    a.cfl_bits = c->dcfl + slot;
+    a.P = c->P;
+    long waves = (long)a.nstrips * a.nsegs;
+    int nb = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    int e = c->arith == ARITH_FAST ? launch_sweep_fast(c->solver, dir, a, nb, c->stream)
+                                   : launch_sweep_exact(c->solver, dir, a, nb, c->stream);
+    HIPCHK(c, (hipError_t)e);
+    return CLAWGPU_OK;
+}
+
+static int enqueue_split(clawgpu_ctx* c, double dx, double dy, double dt, const int* method,
+                         const int* mthlim, int use_fwave, int* result)
+{
+    const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
+    const bool strang = method[2] == -2;
+    const double dtx = strang ? dt / 2.0 : dt;
+    int rc = enqueue_sweep(c, 1, A, B, dtx / dx, method, mthlim, use_fwave, 0);
+    if (!rc) rc = enqueue_sweep(c, 2, B, C, dt / dy, method, mthlim, use_fwave, 1);
+    if (!rc && strang) rc = enqueue_sweep(c, 1, C, B, dtx / dx, method, mthlim, use_fwave, 2);
+    *result = strang ? B : C;
+    return rc;
+}
+
+static double split_cfl(const double* v, bool strang, double cflmax)
+{
+    double cflx = v[0], cfly = v[1];
+    if (cflx > cflmax) return cflx;
+    double cfl = fmax(cflx, cfly);
+    if (strang) {
+        if (cfly > cflmax) return cfly;
+        cfl = fmax(cfl, v[2]);
+    }
+    return cfl;
+}
+
+static int read_cfl4(clawgpu_ctx* c, double* v)
+{
+    HIPCHK(c, hipMemcpyAsync(c->hcfl, c->dcfl, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; k++) v[k] = c->hcfl[k];
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_dimsp2(clawgpu_ctx* c, double dx, double dy, double dt, const int* method,
+                              const int* mthlim, const double* cflv, int use_fwave, double* cfl)
+{
+    if (!c || !method || !mthlim || !cflv || !cfl) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    if (method[2] != -1 && method[2] != -2) return set_err(CLAWGPU_ERR_ARG, "method(3) must be -1 or -2");
+    if (method[1] != 1 && method[1] != 2) return set_err(CLAWGPU_ERR_ARG, "method(2) must be 1 or 2");
+    if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported");
+    for (int mw = 0; mw < c->mwaves; mw++)
+        if (mthlim[mw] < 0 || mthlim[mw] > 5) return set_err(CLAWGPU_ERR_ARG, "invalid limiter method");
+    HIPCHK(c, hipMemsetAsync(c->dcfl, 0, 4 * sizeof(unsigned long long), c->stream));
+    int res;
+    int rc = enqueue_split(c, dx, dy, dt, method, mthlim, use_fwave, &res);
+    if (rc) return rc;
+    double v[4];
+    rc = read_cfl4(c, v);
+    if (rc) return rc;
+    *cfl = split_cfl(v, method[2] == -2, cflv[0]);
+    c->cur = res;
     return CLAWGPU_OK;
 }
 
@@ -387,8 +482,12 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
 {
     if (!c || !tend || !dtv || !cflv || !nv || !method || !mthlim || !mthbc || !info_out)
         return set_err(CLAWGPU_ERR_ARG, "null argument");
-    int rc = check_method(c, method, mthlim);
+    const bool split = method[2] < 0;
+    int rc = split ? CLAWGPU_OK : check_method(c, method, mthlim);
     if (rc) return rc;
+    if (split && method[2] != -1 && method[2] != -2)
+        return set_err(CLAWGPU_ERR_ARG, "method(3) must be -2, -1, 0, 1 or 2");
+    if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported");
     if (check_bc(mthbc) != 0)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides)");
     int info = 0;
@@ -418,10 +517,20 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
         for (;;) {
             rc = enqueue_bc(c, c->dq[c->cur], mthbc, true);
             if (rc) return rc;
-            rc = enqueue_step(c, dx, dy, dt, method, mthlim, use_fwave);
-            if (rc) return rc;
-            rc = read_cfl(c, &cfl);
-            if (rc) return rc;
+            int res = (c->cur + 1) % 3;
+            if (split) {
+                rc = enqueue_split(c, dx, dy, dt, method, mthlim, use_fwave, &res);
+                if (rc) return rc;
+                double v[4];
+                rc = read_cfl4(c, v);
+                if (rc) return rc;
+                cfl = split_cfl(v, method[2] == -2, cflv[0]);
+            } else {
+                rc = enqueue_step(c, dx, dy, dt, method, mthlim, use_fwave);
+                if (rc) return rc;
+                rc = read_cfl(c, &cfl);
+                if (rc) return rc;
+            }
             if (!std::isfinite(cfl)) {
                 *info_out = -5;
                 return set_err(CLAWGPU_ERR_NONFINITE, "non-finite wave speed (Courant number)");
@@ -429,7 +538,7 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
             t = told + dt;
             if (cfl <= cflv[0]) {
                 cflmax = fmax(cfl, cflmax);
-                c->cur = 1 - c->cur;  // accept: the new buffer becomes q
+                c->cur = res;  // accept: the new buffer becomes q
                 break;
             }
             t = told;  // reject: dq[cur] still holds the step's initial data
@@ -501,7 +610,7 @@ extern "C" int clawgpu_time_steps(clawgpu_ctx* c, double dx, double dy, double d
         rc = enqueue_step(c, dx, dy, dt, method, mthlim, use_fwave);
         if (rc) return rc;
         HIPCHK(c, hipEventRecord(ev[3 + 2 * n], c->stream));
-        c->cur = 1 - c->cur;
+        c->cur = (c->cur + 1) % 3;
     }
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
     HIPCHK(c, hipEventSynchronize(ev[1]));

@@ -40,6 +40,10 @@ struct StepArgs {
 
 enum { ARITH_EXACT = 0, ARITH_FAST = 1 };
 
+struct SweepArgs;
+int launch_sweep_exact(int solver, int dir, const SweepArgs& a, int nblocks, hipStream_t s);
+int launch_sweep_fast(int solver, int dir, const SweepArgs& a, int nblocks, hipStream_t s);
+
 // Launch the unsplit step kernel for solver id `solver`; returns a
 // hipError_t value.  exact: reference rounding (FMA contraction off,
 // correctly rounded division); fast: FMA contraction and reciprocal reuse.

@@ -5,6 +5,7 @@
 // 1e-12 relative per field over whole runs.
 #pragma clang fp contract(fast)
 #include "step2d_kernel.hpp"
+#include "split2d_kernel.hpp"
 
 namespace clawgpu {
 int launch_step2_fast(int solver, const StepArgs& a, int nblocks, hipStream_t s)
@@ -16,5 +17,9 @@ int launch_probe_fast(int solver, int n, int ixy, const SolverParams& P, const d
                       double* apdq, double* bm, double* bp, hipStream_t st)
 {
     return launch_probe_impl<FastMath>(solver, n, ixy, P, ql, qr, asdq, wave, s, amdq, apdq, bm, bp, st);
+}
+int launch_sweep_fast(int solver, int dir, const SweepArgs& a, int nblocks, hipStream_t s)
+{
+    return launch_sweep_impl<FastMath>(solver, dir, a, nblocks, s);
 }
 }  // namespace clawgpu

@@ -99,6 +99,15 @@ class Grid2D:
                                      C.byref(cfl)))
         return cfl.value
 
+    def dimsp2(self, dx, dy, dt, method, mthlim, cflv=(1.0, 0.9), use_fwave=False):
+        """One dimensionally split step (method(3) = -1 or -2); returns cfl."""
+        cfl = C.c_double(0.0)
+        cv = np.ascontiguousarray(np.asarray(cflv, dtype=np.float64))
+        check(self.lib.clawgpu_dimsp2(self.h, dx, dy, dt, _ip(_ivec(method, 7)),
+                                      _ip(_ivec(mthlim, self.mwaves)), _dp(cv),
+                                      int(bool(use_fwave)), C.byref(cfl)))
+        return cfl.value
+
     def claw2(self, dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc,
               use_fwave=False):
         """claw2 time loop on the device q; returns dict of the outputs."""

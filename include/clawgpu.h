@@ -77,8 +77,16 @@ int clawgpu_bc2(clawgpu_ctx* ctx, const int* mthbc);
 int clawgpu_step2(clawgpu_ctx* ctx, double dx, double dy, double dt, const int* method,
                   const int* mthlim, int use_fwave, double* cfl);
 
+/* One dimensionally split step, the contract of dimsp2
+ * (src/2d/dimsp2.f:4-98) with step2ds (src/2d/step2ds.f90:2-275):
+ * method(3) = -1 Godunov (x then y) or -2 Strang (x dt/2, y dt, x dt/2); the
+ * x-sweeps cover the ghost rows and the y-sweep the ghost columns, as in the
+ * reference.  *cfl follows dimsp2's early-abort rule with cflv(1). */
+int clawgpu_dimsp2(clawgpu_ctx* ctx, double dx, double dy, double dt, const int* method,
+                   const int* mthlim, const double* cflv, int use_fwave, double* cfl);
+
 /* The claw2 time loop (src/2d/claw2.f:5-8 signature, :494-784 semantics) on
- * the device q: bc2, step, Courant check with retake on rejection, variable
+ * the device q: bc2, step2 or dimsp2 (method(3) < 0), Courant check with retake on rejection, variable
  * or fixed dt, and the outputs tend, dtv(3:5), cflv(3:4), nv(2) and info
  * (0, 2, 3, 5, 6, 11, 12 as in :288-300).  info = -5 is returned (with
  * CLAWGPU_ERR_NONFINITE) when a wave speed is NaN, where the reference would
