@@ -46,6 +46,14 @@ _SIGS = {
     "clawgpu_set_segment_rows": ([_V, _i], _i),
     "clawgpu_set_arith": ([_V, _i], _i),
     "clawgpu_solver_probe": ([_V, _i, _i, _D, _D, _D, _D, _D, _D, _D, _D, _D], _i),
+    "clawgpu_learned_set_params": ([_V, C.POINTER(C.c_float), _i], _i),
+    "clawgpu_learned_features": ([_V, _d, _d, _d], _i),
+    "clawgpu_learned_phi": ([_V, _i], _i),
+    "clawgpu_learned_table_size": ([_V, C.POINTER(C.c_long)], _i),
+    "clawgpu_learned_read": ([_V, _i, _V, C.c_long], _i),
+    "clawgpu_learned_write": ([_V, _i, _V, C.c_long], _i),
+    "clawgpu_learned_set_target": ([_V, _D], _i),
+    "clawgpu_learned_grad": ([_V, _d, _d, _d, _I, _I, _i, C.POINTER(C.c_float), _D], _i),
     "clawgpu_time_steps": ([_V, _d, _d, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
 }
 

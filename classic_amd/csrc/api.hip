@@ -114,38 +114,14 @@ __global__ void planes_to_aos(const double* __restrict__ src, double* __restrict
 // =====================================================================
 using namespace clawgpu;
 
-struct clawgpu_ctx {
-    int meqn, mwaves, maux, mbc, mx, my, solver;
-    SolverParams P;
-    Layout2 L;
-    double* dq[3];  // q and two work copies: qold/qnew of a step, sweeps of a split step
-    int cur;
-    double* staging;
-    unsigned long long* dcfl;
-    double* hcfl;  // pinned
-    hipStream_t stream;
-    int seglen;
-    int arith;
-    int last_hip_error;
-};
+#include "ctx.hpp"
 
-static thread_local char g_errbuf[256];
+thread_local char g_errbuf[256];
 
-static int set_err(int code, const char* msg)
-{
-    snprintf(g_errbuf, sizeof g_errbuf, "%s", msg);
-    return code;
-}
-
-#define HIPCHK(ctx, call)                                                              \
-    do {                                                                               \
-        hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess) {                                                        \
-            snprintf(g_errbuf, sizeof g_errbuf, "%s: %s", #call, hipGetErrorString(e_)); \
-            if (ctx) (ctx)->last_hip_error = (int)e_;                                  \
-            return CLAWGPU_ERR_HIP;                                                    \
-        }                                                                              \
-    } while (0)
+// learned limiter (learned.hip)
+void learned_release(clawgpu_ctx* c);
+int learned_prepare(clawgpu_ctx* c, double dx, double dy, double dt);
+bool learned_ready(clawgpu_ctx* c);
 
 extern "C" const char* clawgpu_last_error(void) { return g_errbuf; }
 
@@ -241,6 +217,7 @@ extern "C" int clawgpu_destroy(clawgpu_ctx* c)
 {
     if (!c) return CLAWGPU_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    learned_release(c);
     if (c->dq[0]) (void)hipFree(c->dq[0]);
     if (c->dq[1]) (void)hipFree(c->dq[1]);
     if (c->dq[2]) (void)hipFree(c->dq[2]);
@@ -326,9 +303,20 @@ static int check_method(const clawgpu_ctx* c, const int* method, const int* mthl
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "method(3) must be 0, 1 or 2 for the unsplit step");
     if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported by this solver");
     if (c->mwaves > kMaxWaves) return set_err(CLAWGPU_ERR_UNSUPPORTED, "too many waves");
-    for (int mw = 0; mw < c->mwaves; mw++)
-        if (mthlim[mw] < 0 || mthlim[mw] > 5) return set_err(CLAWGPU_ERR_ARG, "invalid limiter method");
+    for (int mw = 0; mw < c->mwaves; mw++) {
+        if (mthlim[mw] == kLimTable && !learned_ready(const_cast<clawgpu_ctx*>(c)))
+            return set_err(CLAWGPU_ERR_ARG, "limiter 6 (learned) needs clawgpu_learned_set_params");
+        if (mthlim[mw] < 0 || mthlim[mw] > kLimTable)
+            return set_err(CLAWGPU_ERR_ARG, "invalid limiter method");
+    }
     return CLAWGPU_OK;
+}
+
+static bool uses_table(const clawgpu_ctx* c, const int* mthlim)
+{
+    for (int mw = 0; mw < c->mwaves; mw++)
+        if (mthlim[mw] == kLimTable) return true;
+    return false;
 }
 
 // Enqueue one unsplit step from dq[cur] into dq[1-cur]; the Courant
@@ -351,6 +339,8 @@ static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const i
     a.nsegs = (c->my + a.seglen - 1) / a.seglen;
     a.cfl_bits = c->dcfl;
     a.P = c->P;
+    a.phix = c->phix;
+    a.phiy = c->phiy;
     long waves = (long)a.nstrips * a.nsegs;
     int nb = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     int e = c->arith == ARITH_FAST ? launch_step2_fast(c->solver, a, nb, c->stream)
@@ -374,6 +364,10 @@ extern "C" int clawgpu_step2(clawgpu_ctx* c, double dx, double dy, double dt, co
     int rc = check_method(c, method, mthlim);
     if (rc) return rc;
     HIPCHK(c, hipMemsetAsync(c->dcfl, 0, sizeof(unsigned long long), c->stream));
+    if (uses_table(c, mthlim)) {
+        rc = learned_prepare(c, dx, dy, dt);
+        if (rc) return rc;
+    }
     rc = enqueue_step(c, dx, dy, dt, method, mthlim, use_fwave);
     if (rc) return rc;
     rc = read_cfl(c, cfl);
@@ -487,6 +481,8 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
     if (rc) return rc;
     if (split && method[2] != -1 && method[2] != -2)
         return set_err(CLAWGPU_ERR_ARG, "method(3) must be -2, -1, 0, 1 or 2");
+    if (split && uses_table(c, mthlim))
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "the learned limiter runs with the unsplit step only");
     if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported");
     if (check_bc(mthbc) != 0)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides)");
@@ -526,6 +522,10 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
                 if (rc) return rc;
                 cfl = split_cfl(v, method[2] == -2, cflv[0]);
             } else {
+                if (uses_table(c, mthlim)) {
+                    rc = learned_prepare(c, dx, dy, dt);
+                    if (rc) return rc;
+                }
                 rc = enqueue_step(c, dx, dy, dt, method, mthlim, use_fwave);
                 if (rc) return rc;
                 rc = read_cfl(c, &cfl);
@@ -670,4 +670,17 @@ extern "C" int clawgpu_solver_probe(clawgpu_ctx* c, int ixy, int n, const double
     }
     (void)hipFree(d);
     return rc;
+}
+
+// entry points used by the learned-limiter training (learned.hip)
+int enqueue_step_public(clawgpu_ctx* c, double dx, double dy, double dt, const int* method,
+                        const int* mthlim, int use_fwave)
+{
+    return enqueue_step(c, dx, dy, dt, method, mthlim, use_fwave);
+}
+
+int enqueue_bc_public(clawgpu_ctx* c, const int* mthbc)
+{
+    if (check_bc(mthbc) != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "bad mthbc");
+    return enqueue_bc(c, c->dq[c->cur], mthbc, true);
 }

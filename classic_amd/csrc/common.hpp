@@ -36,7 +36,23 @@ struct StepArgs {
     int nstrips, seglen, nsegs;
     unsigned long long* cfl_bits;  // max Courant number of the step, as raw bits
     SolverParams P;
+    // limiter 6: phi read from per-interface tables (learned limiter);
+    // entry (i, J, mw) at ((J + 1) * (mx + 4) + (i + 1)) * mwaves + mw for
+    // i in [-1, mx+2], J in [-1, my+2]; x: interface between cells (i-1, J)
+    // and (i, J); y: between (i, J-1) and (i, J)
+    const double* phix;
+    const double* phiy;
 };
+
+// limiter id of the table-driven (learned) limiter
+constexpr int kLimTable = 6;
+
+__host__ __device__ inline long phi_index(int i, int J, int mx, int my, int mw, int mwaves)
+{
+    i = i < -1 ? -1 : (i > mx + 2 ? mx + 2 : i);
+    J = J < -1 ? -1 : (J > my + 2 ? my + 2 : J);
+    return ((long)(J + 1) * (mx + 4) + (i + 1)) * mwaves + mw;
+}
 
 enum { ARITH_EXACT = 0, ARITH_FAST = 1 };
 

@@ -1,0 +1,50 @@
+// Device context shared by the host-side translation units (api.hip,
+// learned.hip): the grid, its device buffers and stream, and the error
+// reporting helpers of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "common.hpp"
+#include "../../include/clawgpu.h"
+
+using namespace clawgpu;
+
+struct clawgpu_ctx {
+    int meqn, mwaves, maux, mbc, mx, my, solver;
+    SolverParams P;
+    Layout2 L;
+    double* dq[3];  // q and two work copies: qold/qnew of a step, sweeps of a split step
+    int cur;
+    double* staging;
+    unsigned long long* dcfl;
+    double* hcfl;  // pinned
+    hipStream_t stream;
+    int seglen;
+    int arith;
+    // learned limiter state (allocated on first use)
+    double *phix = nullptr, *phiy = nullptr;  // phi tables read by the step kernel
+    void* learned = nullptr;                   // LearnedState (learned.hip)
+    int last_hip_error;
+};
+
+extern thread_local char g_errbuf[256];
+
+inline int set_err(int code, const char* msg)
+{
+    snprintf(g_errbuf, sizeof g_errbuf, "%s", msg);
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            snprintf(g_errbuf, sizeof g_errbuf, "%s: %s", #call, hipGetErrorString(e_)); \
+            if (ctx) (ctx)->last_hip_error = (int)e_;                                  \
+            return CLAWGPU_ERR_HIP;                                                    \
+        }                                                                              \
+    } while (0)
+

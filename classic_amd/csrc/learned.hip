@@ -1,0 +1,729 @@
+// Learned flux limiter: features, BF16-MFMA inference, FP32 training kernels.
+//
+// A small network (5 -> 32 -> 32 -> 1, GELU) replaces phi(theta) of the
+// wave limiter (src/2d/inlinelimiter.f90:71-97).  For every interface wave
+// the pre-pass computes five features -- the upwind ratio theta (the
+// reference's r), the downwind ratio, the signed Courant number s dt/dx,
+// the wave strength |W| relative to the two states, and s relative to the
+// fastest wave of the interface -- and the network's output is clipped to
+// the Sweby TVD region 0 <= phi <= min(2 theta, 2) (phi = 0 for theta <= 0);
+// wherever the clip would change the output the MC value is used instead.
+// The resulting phi tables are read by the fused step kernel (limiter 6).
+//
+// Inference runs as batched BF16 GEMMs on the matrix cores with FP32
+// accumulation: each wavefront takes 16 waves at a time and computes the
+// layers transposed (hidden x waves), H1^T = W1^T X^T and H2^T = W2^T H1^T,
+// with v_mfma_f32_16x16x16_bf16, so that the accumulator layout of one layer
+// (lane l holds rows 4*(l/16)+r of column l%16) is already the B-operand
+// layout of the next and no data leaves the registers between layers.
+//
+// Training (FP32): the forward/backward kernel evaluates the network per
+// wave and reduces the parameter gradients per block through LDS (the
+// dW2 = H1^T dZ2 outer-product sum is a small GEMM over the block's waves),
+// then adds one partial per parameter per block to global memory.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <new>
+
+#include "ctx.hpp"
+#include "riemann2d.hpp"
+
+namespace clawgpu {
+
+constexpr int kNF = 8;  // features stored per wave (5 used)
+constexpr int kH = 32;
+constexpr int oW1 = 0, oB1 = 5 * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
+              oB3 = oW3 + kH, kNParams = oB3 + 1;  // 1281
+constexpr float kThetaClip = 3.0f;
+
+struct LearnedState {
+    long ntab;        // interface slots per direction: (mx+4)*(my+4)
+    long nw;          // waves per direction: ntab*mwaves
+    float* feat;      // [2][nw][kNF]
+    double* theta;    // [2][nw] upwind ratio
+    float* g;         // [2][nw][meqn] gradient coefficients dt/dx * 0.5|s|(1-|s|dt/dx) W
+    float* dphi;      // [2][nw] dL/dphi
+    float* params;    // [kNParams]
+    float* grad;      // [kNParams]
+    double* target;   // q* in field planes
+    double* lossacc;  // [1]
+    bool have_params;
+};
+
+__device__ inline unsigned short f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (unsigned short)(u >> 16);
+}
+
+__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+
+__device__ inline float gelu_d(float x)
+{
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) +
+           x * 0.3989422804014327f * expf(-0.5f * x * x);
+}
+
+__device__ inline double mc_phi(double th)
+{
+    double c = (1.0 + th) / 2.0;
+    return fmax(0.0, fmin(c, fmin(2.0, 2.0 * th)));
+}
+
+// TVD clip with MC fallback
+__device__ inline double clip_phi(double raw, double th, bool* inside)
+{
+    double hi = th > 0.0 ? fmin(2.0 * th, 2.0) : 0.0;
+    bool in = raw >= 0.0 && raw <= hi && th > 0.0;
+    if (inside) *inside = in;
+    return in ? raw : mc_phi(th);
+}
+
+struct FeatArgs {
+    const double* q;
+    Layout2 L;
+    double dtdx, dtdy;
+    SolverParams P;
+    float* feat;
+    double* theta;
+    float* g;
+    double* phix;  // MC fallback written here for slots without a wave
+    double* phiy;
+    long ntab;
+};
+
+// One thread per (direction, table slot): solves the interface and its two
+// neighbours along the direction, and writes features, theta, gradient
+// coefficients and (as a default) the MC value of phi.
+template <class S>
+__global__ void features_kernel(FeatArgs a)
+{
+    using Mth = typename S::Math;
+    constexpr int M = S::M, MW = S::MW;
+    const Layout2 L = a.L;
+    const int mx = L.mx, my = L.my;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * a.ntab) return;
+    const int dir = (int)(t / a.ntab);
+    const long slot = t % a.ntab;
+    const int i = (int)(slot % (mx + 4)) - 1, J = (int)(slot / (mx + 4)) - 1;
+    const long wbase = ((long)dir * a.ntab + slot) * MW;
+    double* phit = dir == 0 ? a.phix : a.phiy;
+    bool ok = dir == 0 ? (i >= 1 && i <= mx + 1 && J >= 0 && J <= my + 1)
+                       : (J >= 1 && J <= my + 1 && i >= 0 && i <= mx + 1);
+    for (int mw = 0; mw < MW; mw++) {
+        for (int k = 0; k < kNF; k++) a.feat[(wbase + mw) * kNF + k] = 0.0f;
+        for (int m = 0; m < M; m++) a.g[(wbase + mw) * M + m] = 0.0f;
+        a.theta[wbase + mw] = 0.0;
+        phit[slot * MW + mw] = 0.0;
+    }
+    if (!ok) return;
+    // cells k-2 .. k+1 along the direction around interface k
+    double qs[4][M];
+    for (int c = 0; c < 4; c++) {
+        int ii = dir == 0 ? i - 2 + c : i, jj = dir == 0 ? J : J - 2 + c;
+        for (int m = 0; m < M; m++) qs[c][m] = a.q[m * L.fs + L.off(ii, jj)];
+    }
+    double w[3][MW][M], s[3][MW], am[M], ap[M];
+    typename S::Cache cc;
+    for (int k = 0; k < 3; k++) {
+        typename S::Cell cl = S::cell(a.P, qs[k]), cr = S::cell(a.P, qs[k + 1]);
+        if (dir == 0)
+            S::template rpn<1, 2>(a.P, cl, cr, w[k], s[k], am, ap, cc);
+        else
+            S::template rpn<2, 1>(a.P, cl, cr, w[k], s[k], am, ap, cc);
+    }
+    const double dtd = dir == 0 ? a.dtdx : a.dtdy;
+    double smax = 0.0, nl = 0.0, nr = 0.0;
+    for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[1][mw]));
+    for (int m = 0; m < M; m++) {
+        nl += qs[1][m] * qs[1][m];
+        nr += qs[2][m] * qs[2][m];
+    }
+    const double qn = sqrt(nl) + sqrt(nr) + 1e-300;
+    for (int mw = 0; mw < MW; mw++) {
+        double wn2 = 0.0, dl = 0.0, dr = 0.0;
+        for (int m = 0; m < M; m++) {
+            wn2 = wn2 + w[1][mw][m] * w[1][mw][m];
+            dl = dl + w[0][mw][m] * w[1][mw][m];
+            dr = dr + w[1][mw][m] * w[2][mw][m];
+        }
+        if (wn2 == 0.0) continue;
+        const double sp = s[1][mw];
+        const double th = Mth::div(sp > 0.0 ? dl : dr, wn2);
+        const double td = Mth::div(sp > 0.0 ? dr : dl, wn2);
+        float* f = a.feat + (wbase + mw) * kNF;
+        f[0] = (float)(fmin(fmax(th, -kThetaClip), kThetaClip) / kThetaClip);
+        f[1] = (float)(fmin(fmax(td, -kThetaClip), kThetaClip) / kThetaClip);
+        f[2] = (float)(sp * dtd);
+        f[3] = (float)(sqrt(wn2) / qn);
+        f[4] = (float)(sp / (smax + 1e-300));
+        a.theta[wbase + mw] = th;
+        const double c = dtd * 0.5 * fabs(sp) * (1.0 - fabs(sp) * dtd);
+        for (int m = 0; m < M; m++) a.g[(wbase + mw) * M + m] = (float)(c * w[1][mw][m]);
+        phit[slot * MW + mw] = mc_phi(th);
+    }
+}
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct InferArgs {
+    const float* feat;
+    const double* theta;
+    const float* params;
+    double* phix;
+    double* phiy;
+    long nw;  // waves per direction
+    long ntiles;
+};
+
+// BF16-MFMA inference, 16 waves per wavefront iteration; see the file head
+// for the layout.  Lanes 0..15 write the 16 results of a tile.
+__global__ __launch_bounds__(256) void mlp_infer_mfma_kernel(InferArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int grp = lane >> 4, col = lane & 15;
+    const long wave_id = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long nwave = (long)gridDim.x * 4;
+    const float* P = a.params;
+    // weight fragments in the A-operand layout: A[row=col][k=4*grp+j]
+    bf16x4 w1f[2], w2f[2][2];
+    float b1v[2][4], b2v[2][4], w3v[2][4];
+    for (int t = 0; t < 2; t++) {
+        for (int j = 0; j < 4; j++) {
+            int k = 4 * grp + j;  // input feature
+            float v = k < 5 ? P[oW1 + k * kH + 16 * t + col] : 0.0f;
+            w1f[t][j] = (short)f2bf(v);
+            for (int kt = 0; kt < 2; kt++)
+                w2f[t][kt][j] = (short)f2bf(P[oW2 + (16 * kt + 4 * grp + j) * kH + 16 * t + col]);
+        }
+        for (int r = 0; r < 4; r++) {
+            int h = 16 * t + 4 * grp + r;
+            b1v[t][r] = P[oB1 + h];
+            b2v[t][r] = P[oB2 + h];
+            w3v[t][r] = P[oW3 + h];
+        }
+    }
+    const float b3 = P[oB3];
+    const long nall = 2 * a.nw;
+    for (long tile = wave_id; tile < a.ntiles; tile += nwave) {
+        const long n = tile * 16 + col;  // wave handled by this lane's column
+        const bool nok = n < nall;
+        bf16x4 x;
+        for (int j = 0; j < 4; j++) {
+            int k = 4 * grp + j;
+            float v = (nok && k < kNF) ? a.feat[n * kNF + k] : 0.0f;
+            x[j] = (short)f2bf(v);
+        }
+        f32x4 h1[2];
+        for (int t = 0; t < 2; t++) {
+            f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+            z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1f[t], x, z, 0, 0, 0);
+            for (int r = 0; r < 4; r++) h1[t][r] = gelu(z[r] + b1v[t][r]);
+        }
+        bf16x4 hb[2];
+        for (int kt = 0; kt < 2; kt++)
+            for (int j = 0; j < 4; j++) hb[kt][j] = (short)f2bf(h1[kt][j]);
+        float part = 0.0f;
+        for (int t = 0; t < 2; t++) {
+            f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+            z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2f[t][0], hb[0], z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2f[t][1], hb[1], z, 0, 0, 0);
+            for (int r = 0; r < 4; r++) part += w3v[t][r] * gelu(z[r] + b2v[t][r]);
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (grp == 0 && nok) {
+            const double th = a.theta[n];
+            double* tab = n < a.nw ? a.phix : a.phiy;
+            const long idx = n < a.nw ? n : n - a.nw;
+            if (a.feat[n * kNF + 3] > 0.0f)  // nonzero wave; others keep the MC default
+                tab[idx] = clip_phi((double)(part + b3), th, nullptr);
+        }
+    }
+}
+
+// FP32 per-wave forward pass (training path and reference for the MFMA
+// kernel); returns the raw output and keeps the pre-activations.
+__device__ inline float mlp_fwd_f32(const float* P, const float* f, float* z1, float* h1, float* z2,
+                                    float* h2)
+{
+    for (int h = 0; h < kH; h++) {
+        float acc = P[oB1 + h];
+        for (int k = 0; k < 5; k++) acc += f[k] * P[oW1 + k * kH + h];
+        z1[h] = acc;
+        h1[h] = gelu(acc);
+    }
+    float out = P[oB3];
+    for (int h = 0; h < kH; h++) {
+        float acc = P[oB2 + h];
+        for (int k = 0; k < kH; k++) acc += h1[k] * P[oW2 + k * kH + h];
+        z2[h] = acc;
+        h2[h] = gelu(acc);
+        out += P[oW3 + h] * h2[h];
+    }
+    return out;
+}
+
+struct TrainArgs {
+    const float* feat;
+    const double* theta;
+    const float* dphi;  // upstream dL/dphi (mode 1)
+    const float* params;
+    float* grad;
+    double* lossacc;
+    double* phix;  // mode 2: write phi tables from the FP32 forward
+    double* phiy;
+    long nw;
+    int mode;      // 0: regress MC (loss (out - MC)^2); 1: backprop dphi; 2: forward only
+    float scale;   // 1 / number of samples (mode 0)
+};
+
+constexpr int kTB = 128;  // waves per training block
+
+// FP32 forward (+ backward): thread per wave; the block's per-wave vectors go
+// through LDS and every thread reduces a slice of the parameter gradient
+// over the block's waves (dW2 is a 32x32 sum of outer products).
+__global__ __launch_bounds__(kTB) void mlp_train_kernel(TrainArgs a)
+{
+    __shared__ float sP[kNParams];
+    __shared__ float sF[kTB][5];
+    __shared__ float sH1[kTB][kH];
+    __shared__ float sD1[kTB][kH];
+    __shared__ float sH2[kTB][kH];
+    __shared__ float sD2[kTB][kH];
+    __shared__ float sD3[kTB];
+    __shared__ double sLoss[kTB];
+    for (int k = threadIdx.x; k < kNParams; k += kTB) sP[k] = a.params[k];
+    __syncthreads();
+    const long nall = 2 * a.nw;
+    const long n = (long)blockIdx.x * kTB + threadIdx.x;
+    float f[5] = {0, 0, 0, 0, 0}, z1[kH], h1[kH], z2[kH], h2[kH];
+    float d3 = 0.0f;
+    double loss = 0.0;
+    bool live = false;
+    if (n < nall) {
+        const float* fp = a.feat + n * kNF;
+        live = fp[3] > 0.0f;  // nonzero wave
+        for (int k = 0; k < 5; k++) f[k] = fp[k];
+    }
+    float out = mlp_fwd_f32(sP, f, z1, h1, z2, h2);
+    if (live) {
+        const double th = a.theta[n];
+        if (a.mode == 0) {
+            double e = (double)out - mc_phi(th);
+            loss = e * e * a.scale;
+            d3 = (float)(2.0 * e) * a.scale;
+        } else if (a.mode == 1) {
+            bool inside;
+            clip_phi((double)out, th, &inside);
+            d3 = inside ? a.dphi[n] : 0.0f;
+        } else {
+            double* tab = n < a.nw ? a.phix : a.phiy;
+            const long idx = n < a.nw ? n : n - a.nw;
+            tab[idx] = clip_phi((double)out, th, nullptr);
+        }
+    }
+    if (a.mode == 2) return;
+    // backward of this wave
+    float dz2[kH], dz1[kH];
+    for (int h = 0; h < kH; h++) dz2[h] = d3 * sP[oW3 + h] * gelu_d(z2[h]);
+    for (int k = 0; k < kH; k++) {
+        float acc = 0.0f;
+        for (int h = 0; h < kH; h++) acc += sP[oW2 + k * kH + h] * dz2[h];
+        dz1[k] = acc * gelu_d(z1[k]);
+    }
+    const int tid = threadIdx.x;
+    for (int k = 0; k < 5; k++) sF[tid][k] = live ? f[k] : 0.0f;
+    for (int h = 0; h < kH; h++) {
+        sH1[tid][h] = h1[h];
+        sD1[tid][h] = live ? dz1[h] : 0.0f;
+        sH2[tid][h] = h2[h];
+        sD2[tid][h] = live ? dz2[h] : 0.0f;
+    }
+    sD3[tid] = live ? d3 : 0.0f;
+    sLoss[tid] = loss;
+    __syncthreads();
+    for (int p = tid; p < kNParams; p += kTB) {
+        float acc = 0.0f;
+        if (p < oB1) {  // W1[k][h]
+            int k = p / kH, h = p % kH;
+            for (int b = 0; b < kTB; b++) acc += sF[b][k] * sD1[b][h];
+        } else if (p < oW2) {
+            int h = p - oB1;
+            for (int b = 0; b < kTB; b++) acc += sD1[b][h];
+        } else if (p < oB2) {  // W2[k][h]
+            int k = (p - oW2) / kH, h = (p - oW2) % kH;
+            for (int b = 0; b < kTB; b++) acc += sH1[b][k] * sD2[b][h];
+        } else if (p < oW3) {
+            int h = p - oB2;
+            for (int b = 0; b < kTB; b++) acc += sD2[b][h];
+        } else if (p < oB3) {
+            int h = p - oW3;
+            for (int b = 0; b < kTB; b++) acc += sH2[b][h] * sD3[b];
+        } else {
+            for (int b = 0; b < kTB; b++) acc += sD3[b];
+        }
+        if (acc != 0.0f) atomicAdd(&a.grad[p], acc);
+    }
+    if (tid == 0) {
+        double l = 0.0;
+        for (int b = 0; b < kTB; b++) l += sLoss[b];
+        if (l != 0.0) atomicAdd(a.lossacc, l);
+    }
+}
+
+struct DphiArgs {
+    const double* qnew;
+    const double* target;
+    Layout2 L;
+    int meqn;
+    int mwaves;
+    const float* g;
+    float* dphi;
+    long ntab;
+    double* lossacc;
+    double scale;  // 1 / (mx * my)
+};
+
+// dL/dphi of the one-step loss L = scale * sum over interior cells of
+// |qnew - q*|^2, through the normal correction flux: a wave of x-interface
+// i adds +g to cell (i, J) and -g to cell (i-1, J) per unit phi.
+__global__ void dphi_kernel(DphiArgs a)
+{
+    const Layout2 L = a.L;
+    const int mx = L.mx, my = L.my, M = a.meqn, MW = a.mwaves;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * a.ntab) return;
+    const int dir = (int)(t / a.ntab);
+    const long slot = t % a.ntab;
+    const int i = (int)(slot % (mx + 4)) - 1, J = (int)(slot / (mx + 4)) - 1;
+    const int ir = i, jr = J;
+    const int il = dir == 0 ? i - 1 : i, jl = dir == 0 ? J : J - 1;
+    auto res = [&](int ii, int jj, int m) -> double {
+        if (ii < 1 || ii > mx || jj < 1 || jj > my) return 0.0;
+        long o = m * L.fs + L.off(ii, jj);
+        return a.qnew[o] - a.target[o];
+    };
+    for (int mw = 0; mw < MW; mw++) {
+        const long w = ((long)dir * a.ntab + slot) * MW + mw;
+        double d = 0.0;
+        for (int m = 0; m < M; m++) d += (double)a.g[w * M + m] * (res(ir, jr, m) - res(il, jl, m));
+        a.dphi[w] = (float)(2.0 * a.scale * d);
+    }
+    // loss: one thread per interior cell (dir 0 slots with i, J interior)
+    if (dir == 0 && i >= 1 && i <= mx && J >= 1 && J <= my) {
+        double l = 0.0;
+        for (int m = 0; m < M; m++) {
+            double e = res(i, J, m);
+            l += e * e;
+        }
+        atomicAdd(a.lossacc, l * a.scale);
+    }
+}
+
+}  // namespace clawgpu
+
+using namespace clawgpu;
+
+static LearnedState* learned_state(clawgpu_ctx* c)
+{
+    if (c->learned) return (LearnedState*)c->learned;
+    LearnedState* s = new (std::nothrow) LearnedState();
+    if (!s) return nullptr;
+    s->ntab = (long)(c->mx + 4) * (c->my + 4);
+    s->nw = s->ntab * c->mwaves;
+    size_t tabb = (size_t)s->nw * sizeof(double);
+    size_t qbytes = (size_t)c->L.fs * c->meqn * sizeof(double);
+    bool ok = hipMalloc((void**)&c->phix, tabb) == hipSuccess &&
+              hipMalloc((void**)&c->phiy, tabb) == hipSuccess &&
+              hipMalloc((void**)&s->feat, 2 * s->nw * kNF * sizeof(float)) == hipSuccess &&
+              hipMalloc((void**)&s->theta, 2 * s->nw * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&s->g, 2 * s->nw * c->meqn * sizeof(float)) == hipSuccess &&
+              hipMalloc((void**)&s->dphi, 2 * s->nw * sizeof(float)) == hipSuccess &&
+              hipMalloc((void**)&s->params, kNParams * sizeof(float)) == hipSuccess &&
+              hipMalloc((void**)&s->grad, kNParams * sizeof(float)) == hipSuccess &&
+              hipMalloc((void**)&s->target, qbytes) == hipSuccess &&
+              hipMalloc((void**)&s->lossacc, sizeof(double)) == hipSuccess;
+    if (!ok) {
+        delete s;
+        return nullptr;
+    }
+    s->have_params = false;
+    c->learned = s;
+    return s;
+}
+
+void learned_release(clawgpu_ctx* c)
+{
+    LearnedState* s = (LearnedState*)c->learned;
+    if (c->phix) (void)hipFree(c->phix);
+    if (c->phiy) (void)hipFree(c->phiy);
+    c->phix = c->phiy = nullptr;
+    if (!s) return;
+    (void)hipFree(s->feat);
+    (void)hipFree(s->theta);
+    (void)hipFree(s->g);
+    (void)hipFree(s->dphi);
+    (void)hipFree(s->params);
+    (void)hipFree(s->grad);
+    (void)hipFree(s->target);
+    (void)hipFree(s->lossacc);
+    delete s;
+    c->learned = nullptr;
+}
+
+static int enqueue_features(clawgpu_ctx* c, LearnedState* s, double dx, double dy, double dt)
+{
+    FeatArgs a;
+    a.q = c->dq[c->cur];
+    a.L = c->L;
+    a.dtdx = dt / dx;
+    a.dtdy = dt / dy;
+    a.P = c->P;
+    a.feat = s->feat;
+    a.theta = s->theta;
+    a.g = s->g;
+    a.phix = c->phix;
+    a.phiy = c->phiy;
+    a.ntab = s->ntab;
+    int nb = (int)((2 * s->ntab + 127) / 128);
+    // the same wave arithmetic as the step kernel, so that theta (and the MC
+    // default of phi) are the step kernel's own values
+    if (c->solver == CLAWGPU_SOLVER_ACOUSTICS) {
+        if (c->arith == ARITH_FAST)
+            hipLaunchKernelGGL(features_kernel<Acoustics2<FastMath>>, dim3(nb), dim3(128), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(features_kernel<Acoustics2<ExactMath>>, dim3(nb), dim3(128), 0, c->stream, a);
+    } else {
+        if (c->arith == ARITH_FAST)
+            hipLaunchKernelGGL(features_kernel<Euler2<FastMath>>, dim3(nb), dim3(128), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(features_kernel<Euler2<ExactMath>>, dim3(nb), dim3(128), 0, c->stream, a);
+    }
+    HIPCHK(c, hipGetLastError());
+    return CLAWGPU_OK;
+}
+
+static int enqueue_phi(clawgpu_ctx* c, LearnedState* s, int mode)
+{
+    if (mode == 2) return CLAWGPU_OK;  // MC: already written by the feature pass
+    if (!s->have_params) return set_err(CLAWGPU_ERR_ARG, "learned limiter parameters not set");
+    if (mode == 0) {
+        InferArgs a;
+        a.feat = s->feat;
+        a.theta = s->theta;
+        a.params = s->params;
+        a.phix = c->phix;
+        a.phiy = c->phiy;
+        a.nw = s->nw;
+        a.ntiles = (2 * s->nw + 15) / 16;
+        long nb = (a.ntiles + 3) / 4;
+        if (nb > 4096) nb = 4096;
+        hipLaunchKernelGGL(mlp_infer_mfma_kernel, dim3((int)nb), dim3(256), 0, c->stream, a);
+    } else {
+        TrainArgs a{};
+        a.feat = s->feat;
+        a.theta = s->theta;
+        a.params = s->params;
+        a.grad = s->grad;
+        a.lossacc = s->lossacc;
+        a.phix = c->phix;
+        a.phiy = c->phiy;
+        a.nw = s->nw;
+        a.mode = 2;
+        int nb = (int)((2 * s->nw + kTB - 1) / kTB);
+        hipLaunchKernelGGL(mlp_train_kernel, dim3(nb), dim3(kTB), 0, c->stream, a);
+    }
+    HIPCHK(c, hipGetLastError());
+    return CLAWGPU_OK;
+}
+
+// Called by the claw2 loop before each step attempt when every wave family
+// uses limiter 6: features of the current q at this dt, then phi by the
+// BF16 MFMA network with the TVD clip / MC fallback.
+int learned_prepare(clawgpu_ctx* c, double dx, double dy, double dt)
+{
+    LearnedState* s = learned_state(c);
+    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    int rc = enqueue_features(c, s, dx, dy, dt);
+    if (rc) return rc;
+    return enqueue_phi(c, s, 0);
+}
+
+extern "C" int clawgpu_learned_set_params(clawgpu_ctx* c, const float* params, int n)
+{
+    if (!c || !params || n != kNParams) return set_err(CLAWGPU_ERR_ARG, "expected 1281 parameters");
+    LearnedState* s = learned_state(c);
+    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    HIPCHK(c, hipMemcpy(s->params, params, kNParams * sizeof(float), hipMemcpyHostToDevice));
+    s->have_params = true;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_learned_features(clawgpu_ctx* c, double dx, double dy, double dt)
+{
+    if (!c) return set_err(CLAWGPU_ERR_ARG, "null context");
+    LearnedState* s = learned_state(c);
+    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    int rc = enqueue_features(c, s, dx, dy, dt);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_learned_phi(clawgpu_ctx* c, int mode)
+{
+    if (!c || !c->learned) return set_err(CLAWGPU_ERR_ARG, "features not computed");
+    if (mode < 0 || mode > 2) return set_err(CLAWGPU_ERR_ARG, "mode must be 0, 1 or 2");
+    int rc = enqueue_phi(c, (LearnedState*)c->learned, mode);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_learned_read(clawgpu_ctx* c, int what, void* out, long n)
+{
+    if (!c || !c->learned || !out) return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    LearnedState* s = (LearnedState*)c->learned;
+    const void* src = nullptr;
+    size_t bytes = 0;
+    switch (what) {
+    case 0: src = s->feat; bytes = 2 * s->nw * kNF * sizeof(float); break;
+    case 1: src = c->phix; bytes = s->nw * sizeof(double); break;
+    case 2: src = c->phiy; bytes = s->nw * sizeof(double); break;
+    case 3: src = s->theta; bytes = 2 * s->nw * sizeof(double); break;
+    case 4: src = s->g; bytes = 2 * s->nw * c->meqn * sizeof(float); break;
+    case 5: src = s->dphi; bytes = 2 * s->nw * sizeof(float); break;
+    default: return set_err(CLAWGPU_ERR_ARG, "unknown table");
+    }
+    if ((size_t)n < bytes) return set_err(CLAWGPU_ERR_ARG, "output buffer too small");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost));
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_learned_write(clawgpu_ctx* c, int what, const void* in, long n)
+{
+    // overwrite a table (features 0 or theta 3 or dphi 5) from the host: for
+    // the gradient checks of the training kernels
+    if (!c || !in) return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    LearnedState* s = learned_state(c);
+    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    void* dst = nullptr;
+    size_t bytes = 0;
+    switch (what) {
+    case 0: dst = s->feat; bytes = 2 * s->nw * kNF * sizeof(float); break;
+    case 3: dst = s->theta; bytes = 2 * s->nw * sizeof(double); break;
+    case 5: dst = s->dphi; bytes = 2 * s->nw * sizeof(float); break;
+    default: return set_err(CLAWGPU_ERR_ARG, "table not writable");
+    }
+    if ((size_t)n != bytes) return set_err(CLAWGPU_ERR_ARG, "size mismatch");
+    HIPCHK(c, hipMemcpy(dst, in, bytes, hipMemcpyHostToDevice));
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_learned_table_size(clawgpu_ctx* c, long* waves_per_direction)
+{
+    if (!c || !waves_per_direction) return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    *waves_per_direction = (long)(c->mx + 4) * (c->my + 4) * c->mwaves;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_learned_set_target(clawgpu_ctx* c, const double* q)
+{
+    if (!c || !q) return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    LearnedState* s = learned_state(c);
+    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    // reuse the upload path: stage into the current buffer's spare copy
+    int keep = c->cur;
+    c->cur = (keep + 2) % 3;
+    int rc = clawgpu_set_q(c, q);
+    c->cur = keep;
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(s->target, c->dq[(keep + 2) % 3], (size_t)c->L.fs * c->meqn * sizeof(double),
+                        hipMemcpyDeviceToDevice));
+    return CLAWGPU_OK;
+}
+
+bool learned_ready(clawgpu_ctx* c)
+{
+    return c->learned && ((LearnedState*)c->learned)->have_params;
+}
+
+int enqueue_step_public(clawgpu_ctx* c, double dx, double dy, double dt, const int* method,
+                        const int* mthlim, int use_fwave);
+int enqueue_bc_public(clawgpu_ctx* c, const int* mthbc);
+
+// Gradient of a training loss with respect to the network parameters.
+//   mode 0: regression onto MC, L = mean over live waves of (out - phi_MC)^2
+//   mode 1: one-step loss of the unsplit step (method, mthlim ignored: all
+//           waves use the network through the FP32 forward, clipped) from the
+//           device q against the target set by clawgpu_learned_set_target,
+//           L = mean over interior cells of |q_new - q*|^2; the gradient
+//           flows through the normal correction fluxes.
+// grad (host, 1281 floats) and loss receive the results; q is unchanged.
+extern "C" int clawgpu_learned_grad(clawgpu_ctx* c, double dx, double dy, double dt,
+                                    const int* method, const int* mthbc, int mode, float* grad,
+                                    double* loss)
+{
+    if (!c || !grad || !loss || (mode != 0 && mode != 1) || !method || !mthbc)
+        return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    LearnedState* s = learned_state(c);
+    if (!s || !s->have_params) return set_err(CLAWGPU_ERR_ARG, "parameters not set");
+    HIPCHK(c, hipMemsetAsync(s->grad, 0, kNParams * sizeof(float), c->stream));
+    HIPCHK(c, hipMemsetAsync(s->lossacc, 0, sizeof(double), c->stream));
+    int rc;
+    if (mode == 1) {
+        rc = enqueue_bc_public(c, mthbc);
+        if (rc) return rc;
+        rc = enqueue_features(c, s, dx, dy, dt);
+        if (rc) return rc;
+        rc = enqueue_phi(c, s, 1);  // FP32 forward, clipped
+        if (rc) return rc;
+        int lim[kMaxWaves];
+        for (int k = 0; k < kMaxWaves; k++) lim[k] = kLimTable;
+        int keep = c->cur;
+        rc = enqueue_step_public(c, dx, dy, dt, method, lim, 0);
+        if (rc) return rc;
+        int res = (keep + 1) % 3;
+        c->cur = keep;  // q is not advanced
+        DphiArgs d;
+        d.qnew = c->dq[res];
+        d.target = s->target;
+        d.L = c->L;
+        d.meqn = c->meqn;
+        d.mwaves = c->mwaves;
+        d.g = s->g;
+        d.dphi = s->dphi;
+        d.ntab = s->ntab;
+        d.lossacc = s->lossacc;
+        d.scale = 1.0 / ((double)c->mx * c->my);
+        int nb = (int)((2 * s->ntab + 127) / 128);
+        hipLaunchKernelGGL(dphi_kernel, dim3(nb), dim3(128), 0, c->stream, d);
+        HIPCHK(c, hipGetLastError());
+    }
+    TrainArgs a{};
+    a.feat = s->feat;
+    a.theta = s->theta;
+    a.dphi = s->dphi;
+    a.params = s->params;
+    a.grad = s->grad;
+    a.lossacc = s->lossacc;
+    a.nw = s->nw;
+    a.mode = mode;
+    if (mode == 0) {
+        // mean over live waves: count them on the host from theta/features
+        a.scale = 1.0f;
+    }
+    int nb = (int)((2 * s->nw + kTB - 1) / kTB);
+    hipLaunchKernelGGL(mlp_train_kernel, dim3(nb), dim3(kTB), 0, c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(grad, s->grad, kNParams * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(loss, s->lossacc, sizeof(double), hipMemcpyDeviceToHost));
+    return CLAWGPU_OK;
+}

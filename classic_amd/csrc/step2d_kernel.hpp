@@ -227,8 +227,13 @@ __global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(S
                 for (int mw = 0; mw < MW; mw++) {
                     double dl = lane_up(dotr[mw]);
                     if (limw(mw) != 0 && wn2[mw] != 0.0) {
-                        double r = Mth::div(sx[mw] > 0.0 ? dl : dotr[mw], wn2[mw]);
-                        double phi = philim<Mth>(r, limw(mw));
+                        double phi;
+                        if (!Cfg::fixed && limw(mw) == kLimTable) {
+                            phi = a.phix[phi_index(ic, J, mx, my, mw, MW)];
+                        } else {
+                            double r = Mth::div(sx[mw] > 0.0 ? dl : dotr[mw], wn2[mw]);
+                            phi = philim<Mth>(r, limw(mw));
+                        }
 #pragma unroll
                         for (int m = 0; m < M; m++) wx[mw][m] = phi * wx[mw][m];
                     }
@@ -307,8 +312,13 @@ __global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(S
 #pragma unroll
                     for (int m = 0; m < M; m++) nn = nn + yp.w[mw][m] * yp.w[mw][m];
                     if (limw(mw) != 0 && nn != 0.0) {
-                        double r = Mth::div(yp.s[mw] > 0.0 ? yp.dotl[mw] : yc.dotl[mw], nn);
-                        double phi = philim<Mth>(r, limw(mw));
+                        double phi;
+                        if (!Cfg::fixed && limw(mw) == kLimTable) {
+                            phi = a.phiy[phi_index(ic, J - 1, mx, my, mw, MW)];
+                        } else {
+                            double r = Mth::div(yp.s[mw] > 0.0 ? yp.dotl[mw] : yc.dotl[mw], nn);
+                            phi = philim<Mth>(r, limw(mw));
+                        }
 #pragma unroll
                         for (int m = 0; m < M; m++) yp.w[mw][m] = phi * yp.w[mw][m];
                     }

@@ -116,6 +116,35 @@ int clawgpu_solver_probe(clawgpu_ctx* ctx, int ixy, int n, const double* ql, con
                          const double* asdq, double* wave, double* s, double* amdq, double* apdq,
                          double* bmasdq, double* bpasdq);
 
+/* Learned flux limiter (no reference counterpart: it replaces phi(theta) of
+ * src/2d/inlinelimiter.f90:71-97 as limiter method 6).  A 5-32-32-1 GELU
+ * network, parameters as 1281 floats in the order W1[5][32], b1[32],
+ * W2[32][32], b2[32], W3[32], b3, evaluated per interface wave on the matrix
+ * cores in BF16 with FP32 accumulation, clipped to the Sweby TVD region with
+ * MC where the clip triggers.  With the parameters set, mthlim(mw) = 6 in
+ * clawgpu_step2 / clawgpu_claw2 (unsplit) uses it.  The remaining calls
+ * expose the pre-pass and the training gradients:
+ *   features: 5 features, theta and gradient coefficients of every interface
+ *             wave of the device q (ghost cells set) at this dt, MC defaults
+ *             in the phi tables;
+ *   phi:      mode 0 BF16-MFMA network, 1 FP32 network, 2 MC;
+ *   read/write: tables 0 features (float, 8 per wave), 1 phi x, 2 phi y
+ *             (double), 3 theta (double), 4 gradient coefficients (float,
+ *             meqn per wave), 5 dL/dphi (float); waves per direction from
+ *             table_size (slot (i, J) for i in [-1, mx+2], J in [-1, my+2]);
+ *   grad:     parameter gradient (host, 1281 floats) and loss of mode 0
+ *             (sum over waves of (out - phi_MC)^2) or mode 1 (one unsplit
+ *             step against the target q, mean squared error per cell). */
+int clawgpu_learned_set_params(clawgpu_ctx* ctx, const float* params, int n);
+int clawgpu_learned_features(clawgpu_ctx* ctx, double dx, double dy, double dt);
+int clawgpu_learned_phi(clawgpu_ctx* ctx, int mode);
+int clawgpu_learned_table_size(clawgpu_ctx* ctx, long* waves_per_direction);
+int clawgpu_learned_read(clawgpu_ctx* ctx, int what, void* out, long nbytes);
+int clawgpu_learned_write(clawgpu_ctx* ctx, int what, const void* in, long nbytes);
+int clawgpu_learned_set_target(clawgpu_ctx* ctx, const double* q);
+int clawgpu_learned_grad(clawgpu_ctx* ctx, double dx, double dy, double dt, const int* method,
+                         const int* mthbc, int mode, float* grad, double* loss);
+
 #ifdef __cplusplus
 }
 #endif

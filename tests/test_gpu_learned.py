@@ -1,0 +1,146 @@
+"""Learned limiter (limiter 6): the table path, the BF16 MFMA inference, the
+FP32 training gradients against the numpy oracle, and what training gains
+over the plain MC limiter on held-out data."""
+import numpy as np
+import pytest
+
+from classic_amd import SOLVER_EULER
+from classic_amd import learned as Lm
+from classic_amd import problems as P
+from oracle import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+METHOD, BC = (1, 2, 2, 0, 0, 0, 0), (1, 1, 1, 1)
+
+
+def _fallback_params():
+    """Network output -1 everywhere: outside the TVD region, so MC is used."""
+    p = np.zeros(Lm.NPARAMS, np.float32)
+    p[-1] = -1.0
+    return p
+
+
+def _state(n=96, seed=3, steps=10):
+    q, dx, dy = P.euler_quadrants(n, n, seed=seed)
+    with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0]) as g:
+        g.set_q(q)
+        g.claw2(dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], steps, METHOD, [4, 4, 4], BC)
+        return g.get_q(), dx, dy
+
+
+@pytest.mark.parametrize("arith", [0, 1])
+def test_table_limiter_with_mc_fallback_equals_mc(arith):
+    """With every network output outside the TVD region the table holds the
+    MC values: the step is identical to limiter 4 -- bit for bit in exact
+    arithmetic, where the pre-pass computes theta like the step kernel."""
+    q, dx, dy = _state()
+    n = q.shape[0] - 4
+    res = []
+    for lim in (4, 6):
+        with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith) as g:
+            g.set_params(_fallback_params())
+            g.set_q(q)
+            out = g.claw2(dx, dy, 0.0, 1e9, [0.2 * dx, 1e99], [1.0, 0.9], 5, METHOD, [lim] * 3, BC)
+            res.append((g.get_q(), out["dtv"].copy()))
+    if arith == 0:
+        np.testing.assert_array_equal(res[0][0][2:-2, 2:-2], res[1][0][2:-2, 2:-2])
+        np.testing.assert_array_equal(res[0][1], res[1][1])
+    else:  # the pre-pass rounds theta without FMA, the fast kernel with it
+        from tests.helpers import max_rel_err
+        assert max_rel_err(res[1][0], res[0][0]) <= 1e-13
+
+
+def _tables(g):
+    nw = g.nwaves()
+    return g.read(0), g.read(3), np.concatenate([g.read(1), g.read(2)]), nw
+
+
+def test_inference_fp32_and_mfma_vs_oracle():
+    q, dx, dy = _state()
+    n = q.shape[0] - 4
+    params = Lm.init_params(7)
+    with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0]) as g:
+        g.set_params(params)
+        g.set_q(q)
+        g.bc2(BC)
+        g.features(dx, dy, 0.2 * dx)
+        g.phi(1)
+        feat, theta, phi32, nw = _tables(g)
+        g.phi(0)
+        phibf = np.concatenate([g.read(1), g.read(2)])
+    live = feat[:, 3] > 0
+    assert live.sum() > 1000
+    raw, _ = MO.forward(params.astype(np.float64), feat[live, :5].astype(np.float64))
+    ref, inside = MO.clip(raw, theta[live])
+    # FP32 network vs float64 oracle
+    d32 = np.abs(phi32[live] - ref)
+    assert np.quantile(d32, 0.999) < 1e-5, d32.max()
+    # BF16 matrix-core network: within BF16 rounding except where the clip
+    # decision flips at the region boundary
+    dbf = np.abs(phibf[live] - ref)
+    assert np.median(dbf) < 1e-2 and np.mean(dbf < 5e-2) > 0.99, (np.median(dbf), np.mean(dbf < 5e-2))
+    # waves without a nonzero wave keep the MC default
+    np.testing.assert_array_equal(phibf[~live], phi32[~live])
+
+
+def test_mc_regression_gradient_vs_oracle():
+    q, dx, dy = _state(n=64)
+    n = 64
+    params = Lm.init_params(3)
+    with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0]) as g:
+        g.set_params(params)
+        g.set_q(q)
+        g.bc2(BC)
+        g.features(dx, dy, 0.2 * dx)
+        gr, loss = g.grad(dx, dy, 0.2 * dx, 0)
+        feat, theta = g.read(0), g.read(3)
+    live = feat[:, 3] > 0
+    go, lo = MO.grad_mc_regression(params.astype(np.float64), feat[live, :5].astype(np.float64),
+                                   theta[live])
+    assert abs(loss - lo) <= 1e-4 * lo
+    np.testing.assert_allclose(gr, go, rtol=2e-3, atol=1e-4 * np.abs(go).max())
+
+
+def test_one_step_gradient_vs_oracle():
+    """Mode 1: the dL/dphi table written by the kernel, back-propagated by the
+    numpy oracle through the same clip, gives the kernel's parameter gradient."""
+    n = 64
+    snaps, dxc, dtc = Lm.fine_snapshots(n, seed=21, nsnap=1)
+    a, b = snaps[0]
+    params = Lm.init_params(5)
+    with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0], arith=1) as g:
+        g.set_params(params)
+        g.set_q(a)
+        g.set_target(b)
+        gr, loss = g.grad(dxc, dxc, dtc, 1)
+        feat, theta, dphi = g.read(0), g.read(3), g.read(5)
+    live = feat[:, 3] > 0
+    go = MO.grad_from_dphi(params.astype(np.float64), feat[live, :5].astype(np.float64),
+                           theta[live], dphi[live].astype(np.float64))
+    assert loss > 0
+    np.testing.assert_allclose(gr, go, rtol=5e-3, atol=1e-3 * np.abs(go).max())
+
+
+def test_training_gains_over_mc_on_held_out_data():
+    """Train briefly on 4x-coarsened 512^2 runs (3 seeds) and compare the
+    one-step error of learned vs MC on held-out seeds; the trained network
+    must not be worse than MC and the multi-step run must stay finite."""
+    params, hist = Lm.train(n_coarse=128, seeds=(31, 32, 33), nsnap=3, pre_iters=150, iters=200,
+                            lr=2e-3)
+    assert np.isfinite(hist["step"]).all()
+    held = []
+    with Lm.LearnedGrid(SOLVER_EULER, 128, 128, [1.4, 1.0], arith=1) as g:
+        for s in (41, 42):
+            snaps, dx, dtc = Lm.fine_snapshots(128, seed=s, nsnap=3)
+            for a, b in snaps:
+                mc = Lm.one_step_loss(g, a, b, dx, dtc, None)
+                ln = Lm.one_step_loss(g, a, b, dx, dtc, params)
+                held.append((mc, ln))
+    held = np.array(held)
+    gain = held[:, 0].sum() / held[:, 1].sum()
+    print("held-out one-step error MC / learned =", gain, "pre-train loss", hist["pre"][-1],
+          "train loss first/last", hist["step"][0], hist["step"][-1])
+    assert gain >= 1.0
+    q, el, steps = Lm.run(128, 41, 0.05, 6, params)
+    assert np.isfinite(q).all() and steps > 0

@@ -1,0 +1,38 @@
+"""The numpy oracle of the learned limiter's network: its hand-written
+gradients agree with central finite differences, and the clip keeps every
+output in the Sweby TVD region with MC outside it (CPU only)."""
+import numpy as np
+
+from oracle import mlp_oracle as MO
+
+
+def test_gradients_match_finite_differences():
+    rng = np.random.default_rng(0)
+    p = rng.standard_normal(MO.NPARAMS) * 0.3
+    X = rng.standard_normal((60, 5))
+    th = rng.uniform(-1, 3, 60)
+    g, _ = MO.grad_mc_regression(p, X, th)
+    dphi = rng.standard_normal(60)
+    g2 = MO.grad_from_dphi(p, X, th, dphi)
+    eps = 1e-6
+    for k in [0, 7, 100, 159, 170, 200, 700, 1215, 1230, 1250, 1280]:
+        e = np.zeros_like(p)
+        e[k] = eps
+        fd = (MO.grad_mc_regression(p + e, X, th)[1] - MO.grad_mc_regression(p - e, X, th)[1]) / (2 * eps)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, g[k])
+        # linear functional sum(dphi * phi) through the clip (inside waves only)
+        def lin(pp):
+            out, _ = MO.forward(pp, X)
+            _, inside = MO.clip(MO.forward(p, X)[0], th)
+            return float((dphi * np.where(inside, out, 0.0)).sum())
+        fd2 = (lin(p + e) - lin(p - e)) / (2 * eps)
+        assert abs(fd2 - g2[k]) <= 1e-6 * max(1.0, abs(fd2)), (k, fd2, g2[k])
+
+
+def test_clip_region_and_fallback():
+    th = np.array([-1.0, 0.0, 0.25, 0.5, 1.0, 2.0, 5.0])
+    for raw in (-0.5, 0.0, 0.3, 1.0, 1.9, 2.5):
+        phi, inside = MO.clip(np.full_like(th, raw), th)
+        hi = np.where(th > 0, np.minimum(2 * th, 2.0), 0.0)
+        assert np.all(phi >= 0) and np.all(phi <= hi + 1e-15)
+        np.testing.assert_array_equal(phi[~inside], MO.mc(th[~inside]))
