@@ -70,3 +70,41 @@ def euler_quadrants(mx=4096, my=4096, seed=0, gamma=1.4, mbc=2, split=(0.5, 0.5)
         q[mask, 2] = rho * v
         q[mask, 3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
     return q, dx, dy
+
+
+def layered_medium_aux(mx, my, mz, lower=(-1.0, -1.0, -1.0), upper=(1.0, 1.0, 1.0), mbc=2,
+                       below=(2.0, 2.0), above=(1.0, 1.0)):
+    """aux = (Z, c) of a two-layer medium split at z = 0, ghost cells
+    included: `below` for cell centres with z < 0, `above` otherwise (the
+    setaux of examples/acoustics_3d_heterogeneous/setaux.f:19-35).  Shape
+    (mz+2mbc, my+2mbc, mx+2mbc, 2)."""
+    zc, _ = cell_centers(lower[2], upper[2], mz, mbc)
+    aux = np.empty((mz + 2 * mbc, my + 2 * mbc, mx + 2 * mbc, 2))
+    lo = zc < 0.0
+    aux[lo] = below
+    aux[~lo] = above
+    return aux
+
+
+def acoustics3d_heterogeneous(n=20, mbc=2):
+    """The heterogeneous-acoustics example (examples/acoustics_3d_heterogeneous):
+    [-1,1]^3, q = 0, Z = c = 2 below z = 0 and Z = c = 1 above, driven by the
+    piston boundary condition at the top (mthbc(6) = 0).  Returns (q, aux)."""
+    q = np.zeros((n + 2 * mbc,) * 3 + (4,))
+    return q, layered_medium_aux(n, n, n, mbc=mbc)
+
+
+def acoustics3d_pulse(n=512, seed=0, mbc=2, width=0.01):
+    """3D acoustics on [-1,1]^3, q = (p, u, v, w): p = exp(-|x - x0|^2 / width^2)
+    with x0 drawn from default_rng(seed).uniform(-0.5, 0.5, 3), u = v = w = 0;
+    aux = (Z, c): rho = 1, c = 1 (Z = 1) for z < 0 and rho = 3, c = 2 (Z = 6)
+    above.  Returns (q, aux, x0)."""
+    x0 = np.random.default_rng(seed).uniform(-0.5, 0.5, 3)
+    xc, _ = cell_centers(-1.0, 1.0, n, mbc)
+    q = np.zeros((n + 2 * mbc,) * 3 + (4,))
+    ex = np.exp(-((xc - x0[0]) ** 2) / width ** 2)
+    ey = np.exp(-((xc - x0[1]) ** 2) / width ** 2)
+    ez = np.exp(-((xc - x0[2]) ** 2) / width ** 2)
+    q[..., 0] = ez[:, None, None] * ey[None, :, None] * ex[None, None, :]
+    aux = layered_medium_aux(n, n, n, mbc=mbc, below=(1.0, 1.0), above=(6.0, 2.0))
+    return q, aux, x0

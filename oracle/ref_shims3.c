@@ -1,0 +1,169 @@
+/*
+ * oracle/ref_shims3.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Glue that lets the reference 3D Fortran (compiled from /root/reference into
+ * oracle/_ref by oracle/Makefile) run with this project's variable-coefficient
+ * acoustics solvers:
+ *   - Fortran-ABI wrappers around the oracle's rpn3/rpt3/rptt3, passed to the
+ *     reference claw3/step3/dimsp3 as the external procedures of the plug-in
+ *     contract (src/3d/claw3.f:292-516);
+ *   - C entry points that size and zero the work array as src/3d/claw3ez.f
+ *     does and call the reference subroutines.  The piston boundary condition
+ *     is the example's own bc3 (examples/acoustics_3d_heterogeneous/bc3.f),
+ *     compiled with its symbol renamed to bc3piston_.
+ */
+#include <stdlib.h>
+
+#include "claw3d_oracle.h"
+
+extern void claw3_(int* meqn, int* mwaves, int* mbc, int* mx, int* my, int* mz, int* maux,
+                   double* q, double* aux, double* xlower, double* ylower, double* zlower,
+                   double* dx, double* dy, double* dz, double* tstart, double* tend, double* dtv,
+                   double* cflv, int* nv, int* method, int* mthlim, int* mthbc, double* work,
+                   int* mwork, int* use_fwave, int* info, void* bc3, void* rpn3, void* rpt3,
+                   void* rptt3, void* src3, void* b4step3);
+extern void step3_(int* maxm, int* meqn, int* mwaves, int* mbc, int* mx, int* my, int* mz,
+                   double* qold, double* qnew, double* aux, double* dx, double* dy, double* dz,
+                   double* dt, int* method, int* mthlim, double* cfl, double* qadd, double* fadd,
+                   double* gadd, double* hadd, double* q1d, double* dtdx1d, double* dtdy1d,
+                   double* dtdz1d, double* aux1, double* aux2, double* aux3, int* maux,
+                   double* work, int* mwork, int* use_fwave, void* rpn3, void* rpt3,
+                   void* rptt3);
+extern void dimsp3_(int* maxm, int* meqn, int* mwaves, int* mbc, int* mx, int* my, int* mz,
+                    double* qold, double* qnew, double* aux, double* dx, double* dy, double* dz,
+                    double* dt, int* method, int* mthlim, double* cfl, double* cflv, double* qadd,
+                    double* fadd, double* gadd, double* hadd, double* q1d, double* dtdx1d,
+                    double* dtdy1d, double* dtdz1d, double* aux1, double* aux2, double* aux3,
+                    int* maux, double* work, int* mwork, int* use_fwave, void* rpn3, void* rpt3,
+                    void* rptt3);
+extern void bc3_(int* meqn, int* mbc, int* mx, int* my, int* mz, double* xlower, double* ylower,
+                 double* zlower, double* dx, double* dy, double* dz, double* q, int* maux,
+                 double* aux, double* t, double* dt, int* mthbc);
+extern void bc3piston_(int* meqn, int* mbc, int* mx, int* my, int* mz, double* xlower,
+                       double* ylower, double* zlower, double* dx, double* dy, double* dz,
+                       double* q, int* maux, double* aux, double* t, double* dt, int* mthbc);
+extern void src3_(void);
+extern void b4step3_(void);
+
+static void f_rpn3(int* ixyz, int* maxm, int* meqn, int* mwaves, int* maux, int* mbc, int* mx,
+                   double* ql, double* qr, double* auxl, double* auxr, double* wave, double* s,
+                   double* amdq, double* apdq)
+{
+    orc_rpn3_vc_acoustics(*ixyz, *maxm, *meqn, *mwaves, *maux, *mbc, *mx, ql, qr, auxl, auxr,
+                          wave, s, amdq, apdq);
+}
+
+static void f_rpt3(int* ixyz, int* icoor, int* imp, int* maxm, int* meqn, int* mwaves,
+                   int* maux, int* mbc, int* mx, double* ql, double* qr, double* aux1,
+                   double* aux2, double* aux3, double* asdq, double* bmasdq, double* bpasdq)
+{
+    orc_rpt3_vc_acoustics(*ixyz, *icoor, *imp, *maxm, *meqn, *mwaves, *maux, *mbc, *mx, ql, qr,
+                          aux1, aux2, aux3, asdq, bmasdq, bpasdq);
+}
+
+static void f_rptt3(int* ixyz, int* icoor, int* imp, int* impt, int* maxm, int* meqn,
+                    int* mwaves, int* maux, int* mbc, int* mx, double* ql, double* qr,
+                    double* aux1, double* aux2, double* aux3, double* bsasdq, double* cmbsasdq,
+                    double* cpbsasdq)
+{
+    orc_rptt3_vc_acoustics(*ixyz, *icoor, *imp, *impt, *maxm, *meqn, *mwaves, *maux, *mbc, *mx,
+                           ql, qr, aux1, aux2, aux3, bsasdq, cmbsasdq, cpbsasdq);
+}
+
+static int maxm3(int mx, int my, int mz)
+{
+    int m = mx > my ? mx : my;
+    return m > mz ? m : mz;
+}
+
+int ref3_claw3(int meqn, int mwaves, int maux, int mbc, int mx, int my, int mz, double* q,
+               double* aux, const orc3_geom* g, double tstart, double* tend, double* dtv,
+               double* cflv, int* nv, int* method, int* mthlim, int* mthbc, int use_fwave,
+               int piston)
+{
+    int narray = method[4] < 2 ? 1 : 2;
+    int maxm = maxm3(mx, my, mz);
+    /* claw3ez.f:193-196 with one thread */
+    int mwork = (maxm + 2 * mbc) * (46 * meqn + mwaves + meqn * mwaves + 9 * maux + 3) +
+                narray * (mx + 2 * mbc) * (my + 2 * mbc) * (mz + 2 * mbc) * meqn;
+    double* work = (double*)calloc((size_t)mwork, sizeof(double));
+    double xl = g->xlower, yl = g->ylower, zl = g->zlower, dx = g->dx, dy = g->dy, dz = g->dz;
+    int info = 0, fw = use_fwave ? 1 : 0;
+    if (!work) return -3;
+    claw3_(&meqn, &mwaves, &mbc, &mx, &my, &mz, &maux, q, aux, &xl, &yl, &zl, &dx, &dy, &dz,
+           &tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, work, &mwork, &fw, &info,
+           piston ? (void*)bc3piston_ : (void*)bc3_, (void*)f_rpn3, (void*)f_rpt3,
+           (void*)f_rptt3, (void*)src3_, (void*)b4step3_);
+    free(work);
+    return info;
+}
+
+static int ref3_step_common(int split, int meqn, int mwaves, int maux, int mbc, int mx, int my,
+                            int mz, double* qold, double* qnew, double* aux, double dx,
+                            double dy, double dz, double dt, int* method, int* mthlim,
+                            double* cflv, int use_fwave, double* cfl)
+{
+    int maxm = maxm3(mx, my, mz);
+    int n = maxm + 2 * mbc;
+    int ma = maux > 0 ? maux : 1;
+    int mwork = n * (meqn * mwaves + mwaves + 40 * meqn) + 16;
+    size_t tot = (size_t)n * (15 * meqn + 3 + 9 * ma) + (size_t)mwork;
+    double* b = (double*)calloc(tot, sizeof(double));
+    int fw = use_fwave ? 1 : 0;
+    if (!b) return -3;
+    double* qadd = b;
+    double* fadd = qadd + (size_t)n * meqn;
+    double* gadd = fadd + (size_t)n * meqn;
+    double* hadd = gadd + (size_t)6 * n * meqn;
+    double* q1d = hadd + (size_t)6 * n * meqn;
+    double* dtdx1d = q1d + (size_t)n * meqn;
+    double* dtdy1d = dtdx1d + n;
+    double* dtdz1d = dtdy1d + n;
+    double* aux1 = dtdz1d + n;
+    double* aux2 = aux1 + (size_t)3 * n * ma;
+    double* aux3 = aux2 + (size_t)3 * n * ma;
+    double* work = aux3 + (size_t)3 * n * ma;
+    if (!split)
+        step3_(&maxm, &meqn, &mwaves, &mbc, &mx, &my, &mz, qold, qnew, aux, &dx, &dy, &dz, &dt,
+               method, mthlim, cfl, qadd, fadd, gadd, hadd, q1d, dtdx1d, dtdy1d, dtdz1d, aux1,
+               aux2, aux3, &maux, work, &mwork, &fw, (void*)f_rpn3, (void*)f_rpt3,
+               (void*)f_rptt3);
+    else
+        dimsp3_(&maxm, &meqn, &mwaves, &mbc, &mx, &my, &mz, qold, qnew, aux, &dx, &dy, &dz, &dt,
+                method, mthlim, cfl, cflv, qadd, fadd, gadd, hadd, q1d, dtdx1d, dtdy1d, dtdz1d,
+                aux1, aux2, aux3, &maux, work, &mwork, &fw, (void*)f_rpn3, (void*)f_rpt3,
+                (void*)f_rptt3);
+    free(b);
+    return 0;
+}
+
+int ref3_step3(int meqn, int mwaves, int maux, int mbc, int mx, int my, int mz, double* qold,
+               double* qnew, double* aux, double dx, double dy, double dz, double dt,
+               int* method, int* mthlim, int use_fwave, double* cfl)
+{
+    return ref3_step_common(0, meqn, mwaves, maux, mbc, mx, my, mz, qold, qnew, aux, dx, dy, dz,
+                            dt, method, mthlim, NULL, use_fwave, cfl);
+}
+
+int ref3_dimsp3(int meqn, int mwaves, int maux, int mbc, int mx, int my, int mz, double* qold,
+                double* qnew, double* aux, double dx, double dy, double dz, double dt,
+                int* method, int* mthlim, double* cflv, int use_fwave, double* cfl)
+{
+    return ref3_step_common(1, meqn, mwaves, maux, mbc, mx, my, mz, qold, qnew, aux, dx, dy, dz,
+                            dt, method, mthlim, cflv, use_fwave, cfl);
+}
+
+int ref3_bc3(int meqn, int mbc, int mx, int my, int mz, const orc3_geom* g, double* q, double t,
+             int* mthbc, int piston)
+{
+    double xl = g->xlower, yl = g->ylower, zl = g->zlower, dx = g->dx, dy = g->dy, dz = g->dz;
+    double dt = 0.0, dummy_aux[1] = {0.0};
+    int maux = 0;
+    if (piston)
+        bc3piston_(&meqn, &mbc, &mx, &my, &mz, &xl, &yl, &zl, &dx, &dy, &dz, q, &maux, dummy_aux,
+                   &t, &dt, mthbc);
+    else
+        bc3_(&meqn, &mbc, &mx, &my, &mz, &xl, &yl, &zl, &dx, &dy, &dz, q, &maux, dummy_aux, &t,
+             &dt, mthbc);
+    return 0;
+}
