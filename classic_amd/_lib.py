@@ -21,6 +21,7 @@ ERR_NONFINITE = -4
 
 SOLVER_ACOUSTICS = 1
 SOLVER_EULER = 2
+SOLVER_VC_ACOUSTICS = 3
 
 ARITH_EXACT = 0
 ARITH_FAST = 1
@@ -55,6 +56,17 @@ _SIGS = {
     "clawgpu_learned_set_target": ([_V, _D], _i),
     "clawgpu_learned_grad": ([_V, _d, _d, _d, _I, _I, _i, C.POINTER(C.c_float), _D], _i),
     "clawgpu_time_steps": ([_V, _d, _d, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
+    "clawgpu3_create": ([C.POINTER(_V), _i, _i, _i, _i, _i, _i, _i, _i], _i),
+    "clawgpu3_destroy": ([_V], _i),
+    "clawgpu3_set_q": ([_V, _D], _i),
+    "clawgpu3_get_q": ([_V, _D], _i),
+    "clawgpu3_set_aux": ([_V, _D], _i),
+    "clawgpu3_bc3": ([_V, _D, _d, _I], _i),
+    "clawgpu3_step3": ([_V, _d, _d, _d, _d, _I, _I, _i, _D], _i),
+    "clawgpu3_dimsp3": ([_V, _d, _d, _d, _d, _I, _I, _D, _i, _D], _i),
+    "clawgpu3_claw3": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
+    "clawgpu3_set_segment": ([_V, _i], _i),
+    "clawgpu3_time_steps": ([_V, _D, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
 }
 
 _lib = None
@@ -69,7 +81,7 @@ class ClawGpuError(RuntimeError):
 def declared_symbols():
     """Names of the functions declared in include/clawgpu.h."""
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(clawgpu_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(clawgpu3?_\w+)\s*\(", text, re.M)))
 
 
 def load():

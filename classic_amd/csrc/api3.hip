@@ -1,0 +1,593 @@
+// C-ABI of the 3D path: context, boundary conditions, step3 / dimsp3 and the
+// claw3 time loop on the device (declarations and reference citations in
+// include/clawgpu.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "ctx.hpp"
+#include "step3d.hpp"
+
+using namespace clawgpu::s3;
+
+struct clawgpu3_ctx {
+    int meqn, mwaves, maux, mbc, mx, my, mz, solver;
+    Layout3 L;
+    double* dq[3];    // q (field planes) and two work buffers
+    int cur;
+    double* daux;     // Z and c planes
+    double* staging;  // reference-layout copy for uploads and downloads
+    double cmax;      // max |c| over the cells whose interfaces the sweeps visit
+    double* exy;      // piston: exp(-500 r^2) per (i, j) column, from the host's libm
+    double exy_geom[4];
+    hipStream_t stream;
+    int seg;          // cells per segment of the y/z split sweeps
+    int last_hip_error;
+};
+
+namespace {
+
+__global__ void aos_to_planes3(const double* __restrict__ src, double* __restrict__ dst, long n,
+                               int ncomp, int take, long fs)
+{
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int m = 0; m < take; m++) dst[m * fs + c] = src[c * ncomp + m];
+}
+
+__global__ void planes_to_aos3(const double* __restrict__ src, double* __restrict__ dst, long n,
+                               int ncomp, long fs)
+{
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int m = 0; m < ncomp; m++) dst[c * ncomp + m] = src[m * fs + c];
+}
+
+struct Bc3 {
+    int code[6];
+    double t, et;            // time and exp(-500 t^2) (host libm)
+    const double* exy;       // exp(-500 (x^2 + y^2)) per column (host libm)
+};
+
+// bc3.f: faces x-lo, x-hi, y-lo, y-hi, z-lo, z-hi in that order, each over
+// the full extent of the other indices, so a cell outside in several
+// directions ends with the value of the last face: the source cell has every
+// outside index mapped (extrapolation, periodic, mirror), and each wall
+// negates its normal velocity.  Code 0 on z-hi is the piston wall of the
+// heterogeneous-acoustics example (examples/acoustics_3d_heterogeneous/bc3.f
+// labels 600-606): w = 2 s(x, y, t) - w(mirror).
+__global__ void bc3_kernel(double* __restrict__ q, Layout3 L, Bc3 b)
+{
+    const long n = (long)L.nx * L.ny * L.nz;
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int ii = (int)(e % L.nx), jj = (int)((e / L.nx) % L.ny), kk = (int)(e / ((long)L.nx * L.ny));
+    const int g[3] = {ii + 1 - L.mbc, jj + 1 - L.mbc, kk + 1 - L.mbc};
+    const int nn[3] = {L.mx, L.my, L.mz};
+    int s[3];
+    bool flip[3] = {false, false, false}, piston = false, ghost = false;
+    for (int d = 0; d < 3; d++) {
+        s[d] = g[d];
+        if (g[d] >= 1 && g[d] <= nn[d]) continue;
+        ghost = true;
+        const bool hi = g[d] > nn[d];
+        const int code = b.code[2 * d + (hi ? 1 : 0)];
+        if (code == 1) s[d] = hi ? nn[d] : 1;
+        else if (code == 2) s[d] = hi ? g[d] - nn[d] : g[d] + nn[d];
+        else {
+            s[d] = hi ? 2 * nn[d] + 1 - g[d] : 1 - g[d];
+            if (code == 3) flip[d] = true;
+            else piston = true;  // code 0, z-hi only (checked on the host)
+        }
+    }
+    if (!ghost) return;
+    const long src = L.off(s[0], s[1], s[2]);
+    double v[4];
+    for (int m = 0; m < 4; m++) v[m] = q[m * L.fs + src];
+    for (int d = 0; d < 3; d++)
+        if (flip[d]) v[1 + d] = -v[1 + d];
+    if (piston) {
+        double sp = 0.0;
+        if (b.t < 0.1) {
+            // s = -exp(-500 r^2) * exp(-500 t^2), both exponentials taken on
+            // the host so that they round as the reference's do
+            sp = -(b.exy[(long)jj * L.nx + ii] * b.et);
+            if (fabs(sp) < 1e-6) sp = 0.0;
+        }
+        v[3] = 2.0 * sp - v[3];
+    }
+    for (int m = 0; m < 4; m++) q[m * L.fs + e] = v[m];
+}
+
+// max |c| over cells with all indices in [0, n+1] (the cells adjacent to
+// the interfaces 1..n+1 of the slices 0..n+1 that step3/step3ds visit);
+// NaN wins.  Result as raw bits in out[0] (values are >= 0) and a NaN flag
+// in out[1].
+__global__ void cmax_kernel(const double* __restrict__ c, Layout3 L, unsigned long long* out)
+{
+    const long n = (long)(L.mx + 2) * (L.my + 2) * (L.mz + 2);
+    double m = 0.0;
+    bool nan = false;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % (L.mx + 2)), j = (int)((e / (L.mx + 2)) % (L.my + 2)),
+                  k = (int)(e / ((long)(L.mx + 2) * (L.my + 2)));
+        const double v = fabs(c[L.off(i, j, k)]);
+        if (v != v) nan = true;
+        else m = fmax(m, v);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        m = fmax(m, __shfl_xor(m, d));
+        nan = nan || __shfl_xor((int)nan, d);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], (unsigned long long)__double_as_longlong(m));
+        if (nan) atomicOr(&out[1], 1ull);
+    }
+}
+
+int check_method3(const int* method)
+{
+    int info = 0;
+    if (method[0] != 0 && method[0] != 1) info = 6;
+    if (method[1] != 1 && method[1] != 2) info = 6;
+    if (method[1] == 1 && method[2] >= 0 && method[2] != 0 && method[2] != 10 && method[2] != 11)
+        info = 6;
+    if (method[1] == 2 && method[2] >= 0 && method[2] != 0 && method[2] != 10 && method[2] != 11 &&
+        method[2] != 20 && method[2] != 21 && method[2] != 22)
+        info = 6;
+    return info;
+}
+
+int check_args3(clawgpu3_ctx* c, const int* method, const int* mthlim)
+{
+    if (method[2] < 0 && method[2] != -1)
+        return set_err(CLAWGPU_ERR_ARG, "method(3) < 0 must be -1 (dimsp3 is Godunov splitting only)");
+    if (check_method3(method) != 0) return set_err(CLAWGPU_ERR_ARG, "invalid method (chkmth info=6)");
+    if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported in 3D");
+    for (int w = 0; w < c->mwaves; w++)
+        if (mthlim[w] < 0 || mthlim[w] > 5) return set_err(CLAWGPU_ERR_ARG, "mthlim must be 0..5");
+    return CLAWGPU_OK;
+}
+
+int check_bc3(const int* mthbc)
+{
+    for (int f = 0; f < 6; f++) {
+        const int lo = f == 5 ? 0 : 1;
+        if (mthbc[f] < lo || mthbc[f] > 3) return -1;
+    }
+    for (int d = 0; d < 3; d++)
+        if ((mthbc[2 * d] == 2) != (mthbc[2 * d + 1] == 2)) return -1;
+    return 0;
+}
+
+// exp(-500 (x^2 + y^2)) of the piston at every (i, j) column, computed with
+// the host's exp (the reference's dexp) whenever the geometry changes
+int piston_table(clawgpu3_ctx* c, const double* geom)
+{
+    const double g4[4] = {geom[0], geom[1], geom[3], geom[4]};
+    if (c->exy && std::memcmp(g4, c->exy_geom, sizeof g4) == 0) return CLAWGPU_OK;
+    const int nx = c->L.nx, ny = c->L.ny;
+    double* h = new (std::nothrow) double[(size_t)nx * ny];
+    if (!h) return set_err(CLAWGPU_ERR_ARG, "out of host memory");
+    for (int jj = 0; jj < ny; jj++)
+        for (int ii = 0; ii < nx; ii++) {
+            const double xi = geom[0] + ((ii + 1 - c->mbc) - 0.5) * geom[3];
+            const double yj = geom[1] + ((jj + 1 - c->mbc) - 0.5) * geom[4];
+            const double r2 = (xi * xi + yj * yj);
+            h[(size_t)jj * nx + ii] = std::exp(-500.0 * r2);
+        }
+    hipError_t e = hipSuccess;
+    if (!c->exy) e = hipMalloc((void**)&c->exy, (size_t)nx * ny * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->exy, h, (size_t)nx * ny * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    delete[] h;
+    HIPCHK(c, e);
+    std::memcpy(c->exy_geom, g4, sizeof g4);
+    return CLAWGPU_OK;
+}
+
+int enqueue_bc3(clawgpu3_ctx* c, double* q, const double* geom, double t, const int* mthbc)
+{
+    Bc3 b;
+    for (int f = 0; f < 6; f++) b.code[f] = mthbc[f];
+    b.t = t;
+    b.et = 0.0;
+    b.exy = nullptr;
+    if (mthbc[5] == 0) {
+        int rc = piston_table(c, geom);
+        if (rc) return rc;
+        b.exy = c->exy;
+        b.et = std::exp(-(500.0 * (t * t)));
+    }
+    const long n = (long)c->L.nx * c->L.ny * c->L.nz;
+    hipLaunchKernelGGL(bc3_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, q,
+                       c->L, b);
+    HIPCHK(c, hipGetLastError());
+    return CLAWGPU_OK;
+}
+
+Args3 step_args(clawgpu3_ctx* c, const double* q, double* qn, double dx, double dy, double dz,
+                double dt, const int* method, const int* mthlim, int use_fwave)
+{
+    Args3 a;
+    a.q = q;
+    a.qn = qn;
+    a.aux = c->daux;
+    a.L = c->L;
+    a.afs = c->L.fs;
+    a.dtdx = dt / dx;
+    a.dtdy = dt / dy;
+    a.dtdz = dt / dz;
+    a.m2 = method[1];
+    a.m3 = method[2] / 10;
+    a.m4 = method[2] - 10 * a.m3;
+    a.fwave = use_fwave ? 1 : 0;
+    a.lim0 = mthlim[0];
+    a.lim1 = mthlim[1];
+    a.ntx = (c->mx + kOut - 1) / kOut;
+    a.nty = (c->my + kTileY - 1) / kTileY;
+    a.ntz = (c->mz + kTileZ - 1) / kTileZ;
+    return a;
+}
+
+int enqueue_step3(clawgpu3_ctx* c, int res, double dx, double dy, double dz, double dt,
+                  const int* method, const int* mthlim, int use_fwave)
+{
+    Args3 a = step_args(c, c->dq[c->cur], c->dq[res], dx, dy, dz, dt, method, mthlim, use_fwave);
+    const int e = launch_step3_tile(a, c->stream);
+    if (e) return set_err(CLAWGPU_ERR_HIP, "step3 kernel launch failed");
+    return CLAWGPU_OK;
+}
+
+int enqueue_sweep3(clawgpu3_ctx* c, int dir, const double* q, double* qn, double d,
+                   const int* method, const int* mthlim, int use_fwave)
+{
+    SweepArgs3 a;
+    a.q = q;
+    a.qn = qn;
+    a.aux = c->daux;
+    a.L = c->L;
+    a.afs = c->L.fs;
+    a.d = d;
+    a.m2 = method[1];
+    a.fwave = use_fwave ? 1 : 0;
+    a.lim0 = mthlim[0];
+    a.lim1 = mthlim[1];
+    const int n = dir == 1 ? c->my : c->mz;
+    a.seg = c->seg;
+    a.nsegs = (n + c->seg - 1) / c->seg;
+    a.nstrips = dir == 0 ? (c->mx + 2 + kOut - 1) / kOut : (c->L.nx + 63) / 64;
+    const int e = launch_sweep3(dir, a, c->stream);
+    if (e) return set_err(CLAWGPU_ERR_HIP, "sweep kernel launch failed");
+    return CLAWGPU_OK;
+}
+
+// dimsp3.f:5-85 on the device: x, y, z sweeps out of place, cur -> o1 -> o2
+// -> o1.  The Courant numbers are known before the sweeps (dt/dx * max|c|),
+// so the early returns of dimsp3 become a count of sweeps to run.
+int enqueue_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz, double dt, const int* method,
+                   const int* mthlim, const double* cflv, int use_fwave, int* res, double* cfl)
+{
+    const double cx = dt / dx * c->cmax, cy = dt / dy * c->cmax, cz = dt / dz * c->cmax;
+    const int o1 = (c->cur + 1) % 3, o2 = (c->cur + 2) % 3;
+    int rc = enqueue_sweep3(c, 0, c->dq[c->cur], c->dq[o1], dt / dx, method, mthlim, use_fwave);
+    if (rc) return rc;
+    *res = o1;
+    if (cx > cflv[0]) { *cfl = cx; return CLAWGPU_OK; }
+    rc = enqueue_sweep3(c, 1, c->dq[o1], c->dq[o2], dt / dy, method, mthlim, use_fwave);
+    if (rc) return rc;
+    *res = o2;
+    if (cy > cflv[0]) { *cfl = cy; return CLAWGPU_OK; }
+    rc = enqueue_sweep3(c, 2, c->dq[o2], c->dq[o1], dt / dz, method, mthlim, use_fwave);
+    if (rc) return rc;
+    *res = o1;
+    *cfl = fmax(cx, fmax(cy, cz));
+    return CLAWGPU_OK;
+}
+
+double unsplit_cfl(const clawgpu3_ctx* c, double dx, double dy, double dz, double dt)
+{
+    return fmax(dt / dx * c->cmax, fmax(dt / dy * c->cmax, dt / dz * c->cmax));
+}
+
+}  // namespace
+
+extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int maux, int mbc, int mx,
+                               int my, int mz, int solver_id)
+{
+    if (!out) return set_err(CLAWGPU_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (solver_id != CLAWGPU_SOLVER_VC_ACOUSTICS)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "the 3D path has the variable-coefficient acoustics solver only");
+    if (meqn != 4 || mwaves != 2 || maux < 2)
+        return set_err(CLAWGPU_ERR_ARG, "vc acoustics needs meqn=4, mwaves=2, maux>=2 (Z, c)");
+    if (mbc != 2) return set_err(CLAWGPU_ERR_UNSUPPORTED, "the 3D path needs mbc = 2");
+    if (mx < 1 || my < 1 || mz < 1) return set_err(CLAWGPU_ERR_ARG, "grid sizes must be >= 1");
+    clawgpu3_ctx* c = new (std::nothrow) clawgpu3_ctx();
+    if (!c) return set_err(CLAWGPU_ERR_ARG, "out of host memory");
+    c->meqn = meqn; c->mwaves = mwaves; c->maux = maux; c->mbc = mbc;
+    c->mx = mx; c->my = my; c->mz = mz; c->solver = solver_id;
+    c->L.mx = mx; c->L.my = my; c->L.mz = mz; c->L.mbc = mbc;
+    c->L.nx = mx + 2 * mbc; c->L.ny = my + 2 * mbc; c->L.nz = mz + 2 * mbc;
+    c->L.fs = (long)c->L.nx * c->L.ny * c->L.nz;
+    c->seg = 64;
+    c->cmax = 0.0;
+    const size_t planes = (size_t)c->L.fs * sizeof(double);
+    int rc = CLAWGPU_OK;
+    auto fail = [&](hipError_t e) {
+        snprintf(g_errbuf, sizeof g_errbuf, "device allocation failed: %s", hipGetErrorString(e));
+        rc = CLAWGPU_ERR_HIP;
+    };
+    hipError_t e;
+    for (int k = 0; k < 3 && rc == CLAWGPU_OK; k++)
+        if ((e = hipMalloc((void**)&c->dq[k], 4 * planes)) != hipSuccess) fail(e);
+    if (rc == CLAWGPU_OK && (e = hipMalloc((void**)&c->daux, 2 * planes)) != hipSuccess) fail(e);
+    if (rc == CLAWGPU_OK &&
+        (e = hipMalloc((void**)&c->staging, (size_t)(meqn > maux ? meqn : maux) * planes)) != hipSuccess)
+        fail(e);
+    if (rc == CLAWGPU_OK && (e = hipStreamCreate(&c->stream)) != hipSuccess) fail(e);
+    if (rc == CLAWGPU_OK && (e = hipMemsetAsync(c->daux, 0, 2 * planes, c->stream)) != hipSuccess) fail(e);
+    if (rc == CLAWGPU_OK && (e = hipStreamSynchronize(c->stream)) != hipSuccess) fail(e);
+    if (rc != CLAWGPU_OK) {
+        for (int k = 0; k < 3; k++) (void)hipFree(c->dq[k]);
+        (void)hipFree(c->daux);
+        (void)hipFree(c->staging);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_destroy(clawgpu3_ctx* c)
+{
+    if (!c) return CLAWGPU_OK;
+    (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 3; k++) (void)hipFree(c->dq[k]);
+    (void)hipFree(c->daux);
+    (void)hipFree(c->staging);
+    (void)hipFree(c->exy);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_set_q(clawgpu3_ctx* c, const double* q)
+{
+    if (!c || !q) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    const long n = c->L.fs;
+    HIPCHK(c, hipMemcpyAsync(c->staging, q, (size_t)n * c->meqn * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(aos_to_planes3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->staging, c->dq[c->cur], n, c->meqn, c->meqn, n);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_get_q(clawgpu3_ctx* c, double* q)
+{
+    if (!c || !q) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    const long n = c->L.fs;
+    hipLaunchKernelGGL(planes_to_aos3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->dq[c->cur], c->staging, n, c->meqn, n);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(q, c->staging, (size_t)n * c->meqn * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_set_aux(clawgpu3_ctx* c, const double* aux)
+{
+    if (!c || !aux) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    const long n = c->L.fs;
+    HIPCHK(c, hipMemcpyAsync(c->staging, aux, (size_t)n * c->maux * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(aos_to_planes3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->staging, c->daux, n, c->maux, 2, n);
+    HIPCHK(c, hipGetLastError());
+    unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->staging);
+    HIPCHK(c, hipMemsetAsync(bits, 0, 2 * sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(cmax_kernel, dim3(1024), dim3(256), 0, c->stream, c->daux + n, c->L, bits);
+    HIPCHK(c, hipGetLastError());
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(h, bits, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double m;
+    std::memcpy(&m, &h[0], sizeof m);
+    c->cmax = h[1] ? NAN : m;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_bc3(clawgpu3_ctx* c, const double* geom, double t, const int* mthbc)
+{
+    if (!c || !geom || !mthbc) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    if (check_bc3(mthbc) != 0)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides), or 0 on z-hi (piston)");
+    int rc = enqueue_bc3(c, c->dq[c->cur], geom, t, mthbc);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_step3(clawgpu3_ctx* c, double dx, double dy, double dz, double dt,
+                              const int* method, const int* mthlim, int use_fwave, double* cfl)
+{
+    if (!c || !method || !mthlim || !cfl) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    if (method[2] < 0) return set_err(CLAWGPU_ERR_ARG, "method(3) < 0: use clawgpu3_dimsp3");
+    int rc = check_args3(c, method, mthlim);
+    if (rc) return rc;
+    const int res = (c->cur + 1) % 3;
+    rc = enqueue_step3(c, res, dx, dy, dz, dt, method, mthlim, use_fwave);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cur = res;
+    *cfl = unsplit_cfl(c, dx, dy, dz, dt);
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz, double dt,
+                               const int* method, const int* mthlim, const double* cflv,
+                               int use_fwave, double* cfl)
+{
+    if (!c || !method || !mthlim || !cflv || !cfl) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    if (method[2] != -1) return set_err(CLAWGPU_ERR_ARG, "dimsp3 needs method(3) = -1");
+    int rc = check_args3(c, method, mthlim);
+    if (rc) return rc;
+    int res = c->cur;
+    rc = enqueue_dimsp3(c, dx, dy, dz, dt, method, mthlim, cflv, use_fwave, &res, cfl);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cur = res;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_claw3(clawgpu3_ctx* c, const double* geom, double tstart, double* tend,
+                              double* dtv, double* cflv, int* nv, const int* method,
+                              const int* mthlim, const int* mthbc, int use_fwave, int* info_out)
+{
+    if (!c || !geom || !tend || !dtv || !cflv || !nv || !method || !mthlim || !mthbc || !info_out)
+        return set_err(CLAWGPU_ERR_ARG, "null argument");
+    if (check_bc3(mthbc) != 0)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides), or 0 on z-hi (piston)");
+    const double dx = geom[3], dy = geom[4], dz = geom[5];
+    int info = 0, rc;
+    double t = tstart;
+    int maxn = nv[0];
+    double dt = dtv[0];
+    double cflmax = 0.0, dtmin = dt, dtmax = dt, cfl = 0.0;
+    nv[1] = 0;
+    if (check_method3(method) == 6) { info = 6; goto done; }
+    rc = check_args3(c, method, mthlim);
+    if (rc) return rc;
+    if (method[0] == 0) {
+        if (*tend < tstart) {
+            maxn = 1;
+        } else {
+            maxn = (int)lround((*tend - tstart) / dt);
+            if (fabs(maxn * dt - (*tend - tstart)) > 1e-5 * (*tend - tstart)) { info = 2; goto done; }
+        }
+    }
+    if (method[0] == 1 && cflv[1] > cflv[0]) { info = 3; goto done; }
+    if (method[5] > method[6]) { info = 5; goto done; }
+    if (!std::isfinite(c->cmax)) {
+        *info_out = -5;
+        return set_err(CLAWGPU_ERR_NONFINITE, "non-finite sound speed (Courant number)");
+    }
+    if (maxn == 0) goto done;
+    for (int n = 1; n <= maxn; n++) {
+        double told = t;
+        if (told + dt > *tend && tstart < *tend) dt = *tend - told;
+        for (;;) {
+            rc = enqueue_bc3(c, c->dq[c->cur], geom, told, mthbc);
+            if (rc) return rc;
+            // The Courant number of this solver is dt/d * max|c| and known
+            // before the step: a step that would be rejected is not run
+            // (its result would be discarded and q restored).
+            if (method[2] >= 0) cfl = unsplit_cfl(c, dx, dy, dz, dt);
+            else {
+                const double cx = dt / dx * c->cmax, cy = dt / dy * c->cmax, cz = dt / dz * c->cmax;
+                cfl = cx > cflv[0] ? cx : cy > cflv[0] ? cy : fmax(cx, fmax(cy, cz));
+            }
+            t = told + dt;
+            if (cfl <= cflv[0]) {
+                int res = (c->cur + 1) % 3;
+                if (method[2] >= 0) rc = enqueue_step3(c, res, dx, dy, dz, dt, method, mthlim, use_fwave);
+                else {
+                    double cf;
+                    rc = enqueue_dimsp3(c, dx, dy, dz, dt, method, mthlim, cflv, use_fwave, &res, &cf);
+                }
+                if (rc) return rc;
+                cflmax = fmax(cfl, cflmax);
+                c->cur = res;
+                break;
+            }
+            t = told;
+            if (method[0] == 1) {
+                dt = fmin(dtv[1], dt * cflv[1] / cfl);
+                continue;
+            }
+            cflmax = fmax(cfl, cflmax);
+            goto done;
+        }
+        if (method[0] == 1) {
+            if (cfl == 0.0) dt = dtv[1];
+            else dt = fmin(dtv[1], dt * cflv[1] / cfl);
+            dtmin = fmin(dt, dtmin);
+            dtmax = fmax(dt, dtmax);
+        }
+        nv[1] = nv[1] + 1;
+        if (t >= *tend) break;
+    }
+done:
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (method[0] == 1 && t < *tend && nv[1] == maxn) info = 11;
+    if (method[0] == 0 && cflmax > cflv[0]) info = 12;
+    *tend = t;
+    cflv[2] = cflmax;
+    cflv[3] = cfl;
+    dtv[2] = dtmin;
+    dtv[3] = dtmax;
+    dtv[4] = dt;
+    *info_out = info;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_set_segment(clawgpu3_ctx* c, int cells)
+{
+    if (!c || cells < 1) return set_err(CLAWGPU_ERR_ARG, "cells must be >= 1");
+    c->seg = cells;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_time_steps(clawgpu3_ctx* c, const double* geom, double dt,
+                                   const int* method, const int* mthlim, const int* mthbc,
+                                   int use_fwave, int nsteps, float* total_ms,
+                                   float* step_kernel_ms, double* cfl_out)
+{
+    // Fixed-dt unsplit stepping timed with HIP events on the library's
+    // stream, no host synchronisation between steps: total_ms spans all
+    // launches (ghost fill + step kernel per step), step_kernel_ms sums the
+    // event pairs around each step-kernel launch.  Used by the benchmark.
+    if (!c || !geom || !method || !mthlim || !mthbc || !total_ms || nsteps < 1)
+        return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    if (method[2] < 0) return set_err(CLAWGPU_ERR_ARG, "time_steps runs the unsplit step");
+    int rc = check_args3(c, method, mthlim);
+    if (rc) return rc;
+    if (check_bc3(mthbc) != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "bad mthbc");
+    const double dx = geom[3], dy = geom[4], dz = geom[5];
+    hipEvent_t* ev = new (std::nothrow) hipEvent_t[2 * nsteps + 2];
+    if (!ev) return set_err(CLAWGPU_ERR_ARG, "out of host memory");
+    for (int k = 0; k < 2 * nsteps + 2; k++) HIPCHK(c, hipEventCreate(&ev[k]));
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    for (int n = 0; n < nsteps; n++) {
+        rc = enqueue_bc3(c, c->dq[c->cur], geom, n * dt, mthbc);
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(ev[2 + 2 * n], c->stream));
+        const int res = (c->cur + 1) % 3;
+        rc = enqueue_step3(c, res, dx, dy, dz, dt, method, mthlim, use_fwave);
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(ev[3 + 2 * n], c->stream));
+        c->cur = res;
+    }
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    HIPCHK(c, hipEventSynchronize(ev[1]));
+    HIPCHK(c, hipEventElapsedTime(total_ms, ev[0], ev[1]));
+    float ksum = 0.0f;
+    for (int n = 0; n < nsteps; n++) {
+        float k = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&k, ev[2 + 2 * n], ev[3 + 2 * n]));
+        ksum += k;
+    }
+    if (step_kernel_ms) *step_kernel_ms = ksum;
+    for (int k = 0; k < 2 * nsteps + 2; k++) (void)hipEventDestroy(ev[k]);
+    delete[] ev;
+    if (cfl_out) *cfl_out = unsplit_cfl(c, dx, dy, dz, dt);
+    return CLAWGPU_OK;
+}

@@ -29,52 +29,13 @@
 
 #include "common.hpp"
 #include "riemann2d.hpp"
+#include "lanes.hpp"
 
-#ifndef CLAWGPU_DPP
-#define CLAWGPU_DPP 1
-#endif
 #ifndef CLAWGPU_MIN_WAVES
 #define CLAWGPU_MIN_WAVES 1
 #endif
 
 namespace clawgpu {
-
-// Whole-wave shift by one lane: up(v) gives lane l the value of lane l-1,
-// dn(v) the value of lane l+1 (lanes with no source get 0 or their own).
-__device__ inline double lane_up(double v)
-{
-#if CLAWGPU_DPP
-    int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);  // wave_shr:1
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-#else
-    return __shfl_up(v, 1);
-#endif
-}
-
-__device__ inline double lane_dn(double v)
-{
-#if CLAWGPU_DPP
-    int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);  // wave_shl:1
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-#else
-    return __shfl_down(v, 1);
-#endif
-}
-
-template <class T>
-__device__ inline T lane_up_t(const T& v)
-{
-    T r;
-    const double* s = reinterpret_cast<const double*>(&v);
-    double* d = reinterpret_cast<double*>(&r);
-#pragma unroll
-    for (int k = 0; k < (int)(sizeof(T) / sizeof(double)); k++) d[k] = lane_up(s[k]);
-    return r;
-}
 
 // Max of a non-negative double over the wave, then one atomic per wave.
 __device__ inline void publish_cfl(double cfl, unsigned long long* bits)

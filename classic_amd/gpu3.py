@@ -1,0 +1,148 @@
+"""Python front end of the 3D device path: a context per grid.
+
+``Grid3D`` mirrors the arguments of the reference's claw3/step3/dimsp3/bc3
+(sizes, method(1:7), mthlim, mthbc(1:6)) and keeps q and aux resident on the
+GPU.  Host arrays use the reference layout q(meqn, 1-mbc:mx+mbc,
+1-mbc:my+mbc, 1-mbc:mz+mbc), i.e. C-contiguous numpy arrays of shape
+(mz+2*mbc, my+2*mbc, mx+2*mbc, meqn); aux likewise with maux last.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SOLVER_VC_ACOUSTICS, check
+from .gpu import _dp, _ip, _ivec
+
+
+class Claw3Result:
+    def __init__(self, info, tend, dtv, cflv, nv):
+        self.info, self.tend, self.dtv, self.cflv, self.nv = info, tend, dtv, cflv, nv
+
+
+class Grid3D:
+    """Device-resident 3D grid (variable-coefficient acoustics, q = (p, u, v, w))."""
+
+    def __init__(self, mx, my, mz, maux=2, mbc=2, lower=(-1.0, -1.0, -1.0),
+                 upper=(1.0, 1.0, 1.0)):
+        self.lib = _lib.load()
+        self.mx, self.my, self.mz, self.mbc, self.maux = mx, my, mz, mbc, maux
+        self.meqn, self.mwaves = 4, 2
+        d = [(upper[a] - lower[a]) / n for a, n in enumerate((mx, my, mz))]
+        self.geom = np.array([lower[0], lower[1], lower[2], d[0], d[1], d[2]], dtype=np.float64)
+        h = C.c_void_p()
+        check(self.lib.clawgpu3_create(C.byref(h), 4, 2, maux, mbc, mx, my, mz,
+                                       SOLVER_VC_ACOUSTICS))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clawgpu3_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def dx(self):
+        return tuple(self.geom[3:])
+
+    def shape(self, ncomp=None):
+        return (self.mz + 2 * self.mbc, self.my + 2 * self.mbc, self.mx + 2 * self.mbc,
+                self.meqn if ncomp is None else ncomp)
+
+    def set_q(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.shape != self.shape():
+            raise ValueError(f"q has shape {q.shape}, expected {self.shape()}")
+        check(self.lib.clawgpu3_set_q(self.h, _dp(q)))
+
+    def get_q(self):
+        q = np.empty(self.shape())
+        check(self.lib.clawgpu3_get_q(self.h, _dp(q)))
+        return q
+
+    def set_aux(self, aux):
+        aux = np.ascontiguousarray(aux, dtype=np.float64)
+        if aux.shape != self.shape(self.maux):
+            raise ValueError(f"aux has shape {aux.shape}, expected {self.shape(self.maux)}")
+        check(self.lib.clawgpu3_set_aux(self.h, _dp(aux)))
+
+    def bc3(self, mthbc, t=0.0):
+        check(self.lib.clawgpu3_bc3(self.h, _dp(self.geom), t, _ip(_ivec(mthbc, 6))))
+
+    def step3(self, dt, method=(1, 2, 22, 0, 0, 0, 2), mthlim=(3, 3), use_fwave=False):
+        cfl = C.c_double(0.0)
+        dx, dy, dz = self.dx
+        check(self.lib.clawgpu3_step3(self.h, dx, dy, dz, dt, _ip(_ivec(method, 7)),
+                                      _ip(_ivec(mthlim, 2)), int(use_fwave), C.byref(cfl)))
+        return cfl.value
+
+    def dimsp3(self, dt, method=(1, 2, -1, 0, 0, 0, 2), mthlim=(3, 3), cflv=(1.0, 0.9),
+               use_fwave=False):
+        cfl = C.c_double(0.0)
+        cv = np.zeros(4)
+        cv[: len(cflv)] = cflv
+        dx, dy, dz = self.dx
+        check(self.lib.clawgpu3_dimsp3(self.h, dx, dy, dz, dt, _ip(_ivec(method, 7)),
+                                       _ip(_ivec(mthlim, 2)), _dp(cv), int(use_fwave),
+                                       C.byref(cfl)))
+        return cfl.value
+
+    def claw3(self, tstart, tend, dtv, cflv, nmax, method=(1, 2, 22, 0, 0, 0, 2), mthlim=(3, 3),
+              mthbc=(1, 1, 1, 1, 1, 1), use_fwave=False):
+        """The claw3 time loop on the device; returns Claw3Result."""
+        te = C.c_double(tend)
+        dtv_a = np.zeros(5)
+        dtv_a[: len(dtv)] = dtv
+        cflv_a = np.zeros(4)
+        cflv_a[: len(cflv)] = cflv
+        nv = np.array([nmax, 0], dtype=np.int32)
+        info = C.c_int(0)
+        check(self.lib.clawgpu3_claw3(self.h, _dp(self.geom), tstart, C.byref(te), _dp(dtv_a),
+                                      _dp(cflv_a), _ip(nv), _ip(_ivec(method, 7)),
+                                      _ip(_ivec(mthlim, 2)), _ip(_ivec(mthbc, 6)), int(use_fwave),
+                                      C.byref(info)))
+        return Claw3Result(info.value, te.value, dtv_a, cflv_a, nv)
+
+    def frames(self, t0, tfinal, nout, dt_initial, cflv=(1.0, 0.9), dt_max=1e9, nmax=50000,
+               **kw):
+        """claw3ez.f's output_style 1 frame loop; returns [(t, q)]."""
+        out = []
+        dtv = [dt_initial, dt_max]
+        tend = t0
+        dtout = (tfinal - t0) / float(nout)
+        for _ in range(nout):
+            tstart = tend
+            r = self.claw3(tstart, tstart + dtout, dtv, cflv, nmax, **kw)
+            if r.info != 0:
+                raise RuntimeError(f"claw3 info={r.info}")
+            dtv = [r.dtv[4], dt_max]
+            tend = r.tend
+            out.append((tend, self.get_q()))
+        return out
+
+    def set_segment(self, cells):
+        check(self.lib.clawgpu3_set_segment(self.h, cells))
+
+    def time_steps(self, dt, nsteps, method=(1, 2, 22, 0, 0, 0, 2), mthlim=(3, 3),
+                   mthbc=(3, 3, 3, 3, 3, 3), use_fwave=False):
+        tot = C.c_float(0.0)
+        ker = C.c_float(0.0)
+        cfl = C.c_double(0.0)
+        check(self.lib.clawgpu3_time_steps(self.h, _dp(self.geom), dt, _ip(_ivec(method, 7)),
+                                           _ip(_ivec(mthlim, 2)), _ip(_ivec(mthbc, 6)),
+                                           int(use_fwave), nsteps, C.byref(tot), C.byref(ker),
+                                           C.byref(cfl)))
+        return tot.value, ker.value, cfl.value

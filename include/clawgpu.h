@@ -16,6 +16,9 @@
  * procedure pointers of the reference contract, src/2d/claw2.f:312-402):
  *   CLAWGPU_SOLVER_ACOUSTICS  meqn 3, mwaves 2, params (rho, bulk)
  *   CLAWGPU_SOLVER_EULER      meqn 4, mwaves 3, params (gamma, efix 0/1)
+ * and, for the 3D path (clawgpu3_*, src/3d/claw3.f:292-516):
+ *   CLAWGPU_SOLVER_VC_ACOUSTICS  meqn 4, mwaves 2, maux >= 2 with
+ *                                aux(1) = impedance Z, aux(2) = sound speed c
  */
 #ifndef CLAWGPU_H
 #define CLAWGPU_H
@@ -34,7 +37,7 @@ enum {
     CLAWGPU_ERR_NONFINITE = -4    /* a wave speed was NaN: the step cannot be judged */
 };
 
-enum { CLAWGPU_SOLVER_ACOUSTICS = 1, CLAWGPU_SOLVER_EULER = 2 };
+enum { CLAWGPU_SOLVER_ACOUSTICS = 1, CLAWGPU_SOLVER_EULER = 2, CLAWGPU_SOLVER_VC_ACOUSTICS = 3 };
 
 /* Arithmetic of the step kernel.  EXACT (the default) performs the
  * reference's floating-point operations in the reference's order with no
@@ -144,6 +147,68 @@ int clawgpu_learned_write(clawgpu_ctx* ctx, int what, const void* in, long nbyte
 int clawgpu_learned_set_target(clawgpu_ctx* ctx, const double* q);
 int clawgpu_learned_grad(clawgpu_ctx* ctx, double dx, double dy, double dt, const int* method,
                          const int* mthbc, int mode, float* grad, double* loss);
+
+
+/* ------------------------------------------------------------------ 3D */
+
+typedef struct clawgpu3_ctx clawgpu3_ctx;
+
+/* Create / destroy a device context for one 3D grid: q(4, 1-mbc:mx+mbc,
+ * 1-mbc:my+mbc, 1-mbc:mz+mbc) and aux(maux, ...) in the reference layout on
+ * the host.  Replaces the allocations of src/3d/claw3ez.f:189-255 (three
+ * device copies of q replace copyq3's work copy, src/3d/copyq3.f).  The
+ * solver must be CLAWGPU_SOLVER_VC_ACOUSTICS; mbc must be 2. */
+int clawgpu3_create(clawgpu3_ctx** ctx, int meqn, int mwaves, int maux, int mbc, int mx, int my,
+                    int mz, int solver_id);
+int clawgpu3_destroy(clawgpu3_ctx* ctx);
+
+/* Copy q / aux (reference layout, ghost cells included) to and from the
+ * device; set_aux is what src/3d/claw3ez.f:231-234 (setaux) fills. */
+int clawgpu3_set_q(clawgpu3_ctx* ctx, const double* q);
+int clawgpu3_get_q(clawgpu3_ctx* ctx, double* q);
+int clawgpu3_set_aux(clawgpu3_ctx* ctx, const double* aux);
+
+/* Fill the ghost cells of the device q at time t: src/3d/bc3.f with
+ * mthbc(1:6) = (x-lo, x-hi, y-lo, y-hi, z-lo, z-hi) in {1 extrap, 2 periodic,
+ * 3 wall}; mthbc(6) = 0 selects the moving-piston wall of the
+ * heterogeneous-acoustics example (examples/acoustics_3d_heterogeneous/bc3.f),
+ * whose velocity needs geom = (xlower, ylower, zlower, dx, dy, dz). */
+int clawgpu3_bc3(clawgpu3_ctx* ctx, const double* geom, double t, const int* mthbc);
+
+/* One unsplit step, the contract of step3 (src/3d/step3.f90:3-873): the
+ * device q with ghost cells set is qold and the interior of the result
+ * replaces it (ghost cells of the result are unspecified; the reference
+ * leaves partial updates there); *cfl is the Courant number.  method(2) in
+ * {1, 2}, method(3) in {0, 10, 11, 20, 21, 22} (src/3d/chkmth.f:3-72),
+ * method(6) = 0, mthlim(1:2) in 0..5.  Summation order differs from the
+ * reference's slice order: results agree to round-off (tested to 1e-12). */
+int clawgpu3_step3(clawgpu3_ctx* ctx, double dx, double dy, double dz, double dt,
+                   const int* method, const int* mthlim, int use_fwave, double* cfl);
+
+/* One Godunov-split step, the contract of dimsp3 (src/3d/dimsp3.f:5-85,
+ * step3ds src/3d/step3ds.f90:3-408) with method(3) = -1, including its early
+ * return after a sweep whose Courant number exceeds cflv(1).  Bit for bit
+ * with the reference, ghost cells included. */
+int clawgpu3_dimsp3(clawgpu3_ctx* ctx, double dx, double dy, double dz, double dt,
+                    const int* method, const int* mthlim, const double* cflv, int use_fwave,
+                    double* cfl);
+
+/* The claw3 time loop (src/3d/claw3.f:596-904) on the device: bc3, step3 or
+ * dimsp3, CFL check with retake, new dt, with the reference's dtv(1:5),
+ * cflv(1:4), nv(1:2) and info codes (0, 2, 3, 5, 6, 11, 12; -5 for a NaN
+ * sound speed).  geom = (xlower, ylower, zlower, dx, dy, dz). */
+int clawgpu3_claw3(clawgpu3_ctx* ctx, const double* geom, double tstart, double* tend,
+                   double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
+                   const int* mthbc, int use_fwave, int* info);
+
+/* Cells per segment of the y/z split sweeps (tuning; default 64). */
+int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
+
+/* Benchmark helper (no reference counterpart): nsteps fixed-dt unsplit
+ * steps (bc3 + step) timed with HIP events, as clawgpu_time_steps. */
+int clawgpu3_time_steps(clawgpu3_ctx* ctx, const double* geom, double dt, const int* method,
+                        const int* mthlim, const int* mthbc, int use_fwave, int nsteps,
+                        float* total_ms, float* step_kernel_ms, double* cfl);
 
 #ifdef __cplusplus
 }

@@ -43,3 +43,33 @@ def smooth_acoustics(mx, my, seed=0, mbc=2):
 
 def euler_problem(mx, my, seed):
     return P.euler_quadrants(mx, my, seed=seed)
+
+
+def interior3(q, mbc=2):
+    return q[mbc:-mbc, mbc:-mbc, mbc:-mbc]
+
+
+def max_rel_err3(a, b, mbc=2):
+    """max_rel_err for 3D arrays (z, y, x, m)."""
+    a = interior3(a, mbc)
+    b = interior3(b, mbc)
+    top = max(np.abs(b[..., m]).max() for m in range(b.shape[-1]))
+    worst = 0.0
+    for m in range(a.shape[-1]):
+        scale = max(np.abs(b[..., m]).max(), FIELD_FLOOR * top)
+        d = np.abs(a[..., m] - b[..., m]).max()
+        worst = max(worst, d / scale if scale > 0 else d)
+    return worst
+
+
+def random_layered(p, rng, maux=2):
+    """Random q (ghost cells filled by the oracle's bc3) and random materials
+    for an oracle3.Problem3."""
+    from oracle import oracle3 as O3
+    q = O3.bc3(p, rng.standard_normal(p.shape()))
+    aux = np.empty(p.shape(maux))
+    aux[..., 0] = rng.uniform(0.5, 3.0, aux.shape[:3])
+    aux[..., 1] = rng.uniform(0.5, 2.0, aux.shape[:3])
+    if maux > 2:
+        aux[..., 2:] = 1.0
+    return q, aux

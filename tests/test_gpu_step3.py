@@ -1,0 +1,143 @@
+"""3D path on the GPU against the oracle (oracle/claw3d_oracle.c): bc3 and
+the split step (dimsp3) bit for bit, the unsplit step (step3, every
+method(3) family) to 1e-12 relative per field (the tile kernel sums each
+cell's increments in its own order), the claw3 loop (dt, Courant numbers,
+step counts exactly; q to 1e-12), and the heterogeneous-acoustics example's
+regression frame sums."""
+import os
+
+import numpy as np
+import pytest
+
+from classic_amd import problems as P
+from classic_amd.gpu3 import Grid3D
+from oracle import oracle3 as O3
+from tests.helpers import max_rel_err3, random_layered
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# sizes with ragged tiles in every direction (x strips of 60, 4x4 y-z tiles)
+MX, MY, MZ = 70, 9, 6
+
+
+def _grid(p, aux, q):
+    g = Grid3D(p.mx, p.my, p.mz, maux=p.maux)
+    g.set_aux(aux)
+    g.set_q(q)
+    return g
+
+
+@pytest.mark.parametrize("mthbc", [(1, 1, 1, 1, 1, 1), (2, 2, 2, 2, 2, 2), (3, 3, 3, 3, 3, 3),
+                                   (1, 3, 2, 2, 3, 0)])
+def test_bc3_bitwise(mthbc):
+    rng = np.random.default_rng(1)
+    p = O3.Problem3(MX, MY, MZ, mthbc=mthbc, lower=(-0.3, -0.2, -1.0))
+    q = rng.standard_normal(p.shape())
+    aux = np.ones(p.shape(2))
+    for t in (0.0, 0.02):
+        ref = O3.bc3(p, q, t)
+        with _grid(p, aux, q) as g:
+            g.geom[:] = [p.geom.xlower, p.geom.ylower, p.geom.zlower, p.geom.dx, p.geom.dy, p.geom.dz]
+            g.bc3(mthbc, t)
+            np.testing.assert_array_equal(g.get_q(), ref)
+
+
+@pytest.mark.parametrize("order,lim,fw", [(2, (3, 3), False), (2, (4, 1), False),
+                                          (2, (2, 3), True), (1, (0, 0), False)])
+def test_dimsp3_bitwise(order, lim, fw):
+    rng = np.random.default_rng(2)
+    p = O3.Problem3(MX, MY, MZ, mthlim=lim, method=(1, order, -1, 0, 0, 0, 0), use_fwave=fw)
+    q, aux = random_layered(p, rng)
+    dt = 0.3 * p.geom.dx / 2.0
+    ref, cfl_ref = O3.dimsp3(p, q, aux, dt)
+    with _grid(p, aux, q) as g:
+        cfl = g.dimsp3(dt, method=p.method, mthlim=lim, use_fwave=fw)
+        out = g.get_q()
+    assert cfl == cfl_ref
+    np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.parametrize("m3,lim,fw", [(22, (3, 3), False), (22, (4, 2), False), (21, (1, 4), False),
+                                       (20, (4, 4), False), (11, (0, 0), False), (10, (0, 0), False),
+                                       (0, (3, 3), False), (22, (3, 3), True)])
+def test_step3_matches_oracle(m3, lim, fw):
+    rng = np.random.default_rng(3 + m3)
+    order = 1 if lim == (0, 0) else 2
+    p = O3.Problem3(MX, MY, MZ, mthlim=lim, method=(1, order, m3, 0, 0, 0, 0), use_fwave=fw)
+    q, aux = random_layered(p, rng)
+    dt = 0.25 * p.geom.dx / 2.0
+    ref, cfl_ref = O3.step3(p, q, aux, dt)
+    with _grid(p, aux, q) as g:
+        cfl = g.step3(dt, method=p.method, mthlim=lim, use_fwave=fw)
+        out = g.get_q()
+    assert cfl == cfl_ref
+    err = max_rel_err3(out, ref)
+    print(f"step3 method {order},{m3} lim {lim} fwave {fw}: max rel err {err:.2e}")
+    assert err <= TOL
+
+
+def test_step3_larger_grid_many_tiles():
+    rng = np.random.default_rng(9)
+    p = O3.Problem3(125, 34, 23, mthlim=(3, 4))
+    q, aux = random_layered(p, rng)
+    dt = 0.3 * p.geom.dx / 2.0
+    ref, _ = O3.step3(p, q, aux, dt)
+    with _grid(p, aux, q) as g:
+        g.step3(dt, method=p.method, mthlim=(3, 4))
+        out = g.get_q()
+    assert max_rel_err3(out, ref) <= TOL
+
+
+def test_heterogeneous_example_frames_and_regression_sums():
+    gold = np.load(os.path.join(GOLD, "hetero3d_frames_20.npz"))
+    p = O3.Problem3(20, 20, 20, mthbc=(1, 1, 1, 1, 1, 0), mthlim=(3, 3))
+    q, aux = P.acoustics3d_heterogeneous(20)
+    ref = O3.frames(p, q, aux, 0.0, 1.0, 2, 0.01)
+    with _grid(p, aux, q) as g:
+        out = g.frames(0.0, 1.0, 2, 0.01, mthlim=(3, 3), mthbc=(1, 1, 1, 1, 1, 0))
+    for k, ((t_r, q_r), (t_g, q_g)) in enumerate(zip(ref, out)):
+        assert t_r == t_g
+        err = max_rel_err3(q_g, q_r)
+        sums = np.array([q_g[2:-2, 2:-2, 2:-2, m].sum() for m in range(3)])
+        reg = gold["regression_sums"][k]
+        print(f"frame {k + 1}: t={t_g} max rel err {err:.2e} sums {sums} regression {reg}")
+        assert err <= TOL
+        assert abs(sums[0] - reg[0]) <= 1e-13 * abs(reg[0])
+        assert np.all(np.abs(sums[1:] - reg[1:]) <= 1e-14)
+
+
+@pytest.mark.parametrize("m3", [22, -1])
+def test_claw3_run_with_retakes_matches_oracle(m3):
+    p = O3.Problem3(36, 28, 20, mthbc=(3, 3, 3, 3, 3, 3), mthlim=(2, 2),
+                    method=(1, 2, m3, 0, 0, 0, 0))
+    qc, _, _ = P.acoustics3d_pulse(36, seed=4, width=0.3)
+    q = qc[: p.shape()[0], : p.shape()[1]].copy()
+    aux = P.layered_medium_aux(36, 28, 20, below=(1.0, 1.0), above=(6.0, 2.0))
+    r = O3.claw3(p, q, aux, 0.0, 0.4, [0.2, 1e9], [1.0, 0.9], 500)
+    with _grid(p, aux, q) as g:
+        rg = g.claw3(0.0, 0.4, [0.2, 1e9], [1.0, 0.9], 500, method=p.method, mthlim=(2, 2),
+                     mthbc=(3, 3, 3, 3, 3, 3))
+        out = g.get_q()
+    assert rg.info == r.info == 0
+    assert rg.tend == r.tend
+    np.testing.assert_array_equal(rg.nv, r.nv)
+    np.testing.assert_array_equal(rg.dtv, r.dtv)
+    np.testing.assert_array_equal(rg.cflv, r.cflv)
+    err = max_rel_err3(out, r.q)
+    print(f"claw3 method(3)={m3}: {r.nv[1]} steps, max rel err {err:.2e}")
+    if m3 < 0:
+        np.testing.assert_array_equal(out[2:-2, 2:-2, 2:-2], r.q[2:-2, 2:-2, 2:-2])
+    assert err <= TOL
+
+
+def test_unsupported_options_fail_loudly():
+    from classic_amd._lib import ClawGpuError
+    with Grid3D(8, 8, 8) as g:
+        with pytest.raises(ClawGpuError):
+            g.step3(0.01, method=(1, 2, 22, 0, 0, 1, 2))   # capacity function
+        with pytest.raises(ClawGpuError):
+            g.step3(0.01, method=(1, 2, 12, 0, 0, 0, 2))   # not a chkmth method
+        with pytest.raises(ClawGpuError):
+            g.bc3((0, 1, 1, 1, 1, 1))                       # user BC off z-hi
