@@ -108,3 +108,30 @@ def acoustics3d_pulse(n=512, seed=0, mbc=2, width=0.01):
     q[..., 0] = ez[:, None, None] * ey[None, :, None] * ex[None, None, :]
     aux = layered_medium_aux(n, n, n, mbc=mbc, below=(1.0, 1.0), above=(6.0, 2.0))
     return q, aux, x0
+
+
+def acoustics1d_gaussian(mx=400, mbc=2, lower=-1.0, upper=1.0, beta=200.0):
+    """examples/acoustics_1d_example1: q = (p, u) with p = exp(-beta (x-0.3)^2),
+    u = 0 (qinit.f90).  Returns (q, dx), q of shape (mx+2mbc, 2)."""
+    xc, dx = cell_centers(lower, upper, mx, mbc)
+    q = np.zeros((mx + 2 * mbc, 2))
+    q[mbc:-mbc, 0] = np.exp(-beta * (xc[mbc:-mbc] - 0.3) ** 2)
+    return q, dx
+
+
+def acoustics1d_heterogeneous(mx=500, mbc=2, lower=-5.0, upper=5.0, rhol=1.0, cl=1.0, rhor=4.0,
+                              cr=0.5):
+    """examples/acoustics_1d_heterogeneous with ic = 1: p = u = sqrt(1-(x+3)^2)
+    on -4 < x < -2 (qinit.f); aux = (Z, c) = (rho*c, c) left / right of x = 0
+    (setaux.f, setprob.f).  Returns (q, aux, dx)."""
+    xc, dx = cell_centers(lower, upper, mx, mbc)
+    q = np.zeros((mx + 2 * mbc, 2))
+    xi = xc[mbc:-mbc]
+    hump = np.where((xi > -4.0) & (xi < -2.0), np.sqrt(np.maximum(1.0 - (xi + 3.0) ** 2, 0.0)), 0.0)
+    q[mbc:-mbc, 0] = hump
+    q[mbc:-mbc, 1] = hump
+    aux = np.empty((mx + 2 * mbc, 2))
+    left = xc < 0.0
+    aux[left] = (rhol * cl, cl)
+    aux[~left] = (rhor * cr, cr)
+    return q, aux, dx
