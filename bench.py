@@ -69,10 +69,97 @@ def cpu_baseline(mx, my, seed, nsteps=1):
     return (mx * my * steps) / el / 1e9, el, f"{mx}x{my} Euler, {steps} step(s), 1 thread"
 
 
+WORKLOAD3 = "acoustics3d_512cubed_vc_layered_superbee_method22_walls_100steps"
+BYTES_PER_CELL_UPDATE3 = 80  # (2*meqn + maux) * 8 with meqn = 4, maux = 2
+
+
+def cpu_baseline3(n, seed):
+    """Time the CPU oracle (single thread, -O2) on one unsplit step of an
+    n^3 sample of the 3D workload; returns (Gcell-updates/s, s, sample)."""
+    try:
+        from oracle import oracle3 as O3
+        from classic_amd import problems as P
+        O3.lib()
+    except Exception as e:  # pragma: no cover
+        return None, 0.0, f"oracle unavailable: {e}"
+    q, aux, _ = P.acoustics3d_pulse(n, seed=seed)
+    p = O3.Problem3(n, n, n, mthbc=(3,) * 6, mthlim=(2, 2))
+    q = O3.bc3(p, q)
+    dt = 0.9 * p.geom.dx / 2.0
+    t0 = time.perf_counter()
+    O3.step3(p, q, aux, dt)
+    el = time.perf_counter() - t0
+    return n ** 3 / el / 1e9, el, f"{n}^3 acoustics3d, 1 unsplit step, 1 thread"
+
+
+def bench3(args):
+    """3D variable-coefficient acoustics, n^3 cells on [-1,1]^3, two-layer
+    medium (Z=1, c=1 below z=0; Z=6, c=2 above), Gaussian pressure pulse at a
+    seeded position, walls on all faces, superbee on both waves, unsplit
+    method (2, 22), CFL 0.9: --steps accepted claw3 steps, q and aux
+    resident in HBM."""
+    from classic_amd import problems as P
+    from classic_amd.gpu3 import Grid3D
+
+    n = args.n3
+    q0, aux, _ = P.acoustics3d_pulse(n, seed=args.seed)
+    method, mthlim, mthbc = (1, 2, 22, 0, 0, 0, 2), (2, 2), (3, 3, 3, 3, 3, 3)
+    g = Grid3D(n, n, n)
+    g.set_aux(aux)
+    del aux
+    g.set_q(q0)
+    dx = g.dx[0]
+    if args.warmup > 0:
+        g.time_steps(0.25 * dx / 2.0, args.warmup, method, mthlim, mthbc)
+    g.set_q(q0)
+    del q0
+    t0 = time.perf_counter()
+    r = g.claw3(0.0, 1e30, [0.9 * dx / 2.0, 1e99], [1.0, 0.9], args.steps, method, mthlim, mthbc)
+    wall = time.perf_counter() - t0
+    steps = int(r.nv[1])
+    if steps != args.steps:
+        raise SystemExit(f"claw3 took {steps} steps, expected {args.steps} (info={r.info})")
+    cells = n ** 3
+    value = cells * steps / wall / 1e9
+    ksteps = max(5, min(args.steps, 20))
+    total_ms, kern_ms, _ = g.time_steps(float(r.dtv[4]), ksteps, method, mthlim, mthbc)
+    g.close()
+    kavg_s = kern_ms / ksteps / 1e3
+    achieved = cells * BYTES_PER_CELL_UPDATE3 / kavg_s / 1e9
+    cpu = None
+    if not args.no_cpu:
+        v, el, sample = cpu_baseline3(args.cpu_n3, args.seed)
+        if v is not None:
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+                   "sample": f"{sample} ({el:.1f} s), oracle C restatement compiled -O2"}
+    return {
+        "metric": METRIC, "value": value, "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Gaussian pulse in a two-layer medium)",
+        "config": {"workload": WORKLOAD3 if n == 512 else f"acoustics3d_{n}cubed", "mx": n,
+                   "my": n, "mz": n, "seed": args.seed, "method": list(method),
+                   "mthlim": list(mthlim), "mthbc": list(mthbc), "cfl_desired": 0.9,
+                   "cfl_max": 1.0},
+        "hbm_frac_of_peak": value * BYTES_PER_CELL_UPDATE3 / HBM_PEAK_GBS,
+        "kernel_ms_avg": kavg_s * 1e3,
+        "step_overhead_ms_avg": (total_ms - kern_ms) / ksteps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["euler2d", "acoustics3d"], default="euler2d",
+                    help="euler2d: the headline 4096^2 Euler run; acoustics3d: the 512^3 "
+                         "variable-coefficient acoustics run")
+    ap.add_argument("--n3", type=int, default=512)
+    ap.add_argument("--cpu-n3", type=int, default=128)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="accepted steps to time (default 200 for euler2d, 100 for acoustics3d)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
@@ -90,6 +177,12 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the single-grid step is not sharded across GPUs.", file=sys.stderr)
         sys.exit(2)
+
+    if args.steps is None:
+        args.steps = 100 if args.workload == "acoustics3d" else 200
+    if args.workload == "acoustics3d":
+        print(json.dumps(bench3(args)), flush=True)
+        return
 
     from classic_amd import SOLVER_EULER, gpu
     from classic_amd import problems as P
@@ -131,9 +224,9 @@ def main():
     if args.copy_calibration:
         import torch
 
-        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.mul(src, 2.0, out=dst)  # one elementwise kernel: reads 1 GiB, writes 1 GiB
         torch.cuda.synchronize()
 
     cpu = None

@@ -22,6 +22,7 @@ ERR_NONFINITE = -4
 SOLVER_ACOUSTICS = 1
 SOLVER_EULER = 2
 SOLVER_VC_ACOUSTICS = 3
+SOLVER_ADVECTION = 4
 
 ARITH_EXACT = 0
 ARITH_FAST = 1
@@ -56,6 +57,14 @@ _SIGS = {
     "clawgpu_learned_set_target": ([_V, _D], _i),
     "clawgpu_learned_grad": ([_V, _d, _d, _d, _I, _I, _i, C.POINTER(C.c_float), _D], _i),
     "clawgpu_time_steps": ([_V, _d, _d, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
+    "clawgpu1_create": ([C.POINTER(_V), _i, _i, _i, _i, _i, _i, _D, _i], _i),
+    "clawgpu1_destroy": ([_V], _i),
+    "clawgpu1_set_q": ([_V, _D], _i),
+    "clawgpu1_get_q": ([_V, _D], _i),
+    "clawgpu1_set_aux": ([_V, _D], _i),
+    "clawgpu1_bc1": ([_V, _I], _i),
+    "clawgpu1_step1": ([_V, _d, _d, _I, _I, _i, _D], _i),
+    "clawgpu1_claw1": ([_V, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
     "clawgpu3_create": ([C.POINTER(_V), _i, _i, _i, _i, _i, _i, _i, _i], _i),
     "clawgpu3_destroy": ([_V], _i),
     "clawgpu3_set_q": ([_V, _D], _i),
@@ -66,6 +75,7 @@ _SIGS = {
     "clawgpu3_dimsp3": ([_V, _d, _d, _d, _d, _I, _I, _D, _i, _D], _i),
     "clawgpu3_claw3": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
     "clawgpu3_set_segment": ([_V, _i], _i),
+    "clawgpu3_set_tile": ([_V, _i, _i], _i),
     "clawgpu3_time_steps": ([_V, _D, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
 }
 
@@ -81,7 +91,7 @@ class ClawGpuError(RuntimeError):
 def declared_symbols():
     """Names of the functions declared in include/clawgpu.h."""
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(clawgpu3?_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(clawgpu[13]?_\w+)\s*\(", text, re.M)))
 
 
 def load():

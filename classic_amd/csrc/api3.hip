@@ -24,6 +24,8 @@ struct clawgpu3_ctx {
     double exy_geom[4];
     hipStream_t stream;
     int seg;          // cells per segment of the y/z split sweeps
+    int tile;         // tile shape of the unsplit kernel (launch_step3_tile)
+    int xcd;          // XCD-contiguous tile order
     int last_hip_error;
 };
 
@@ -229,8 +231,8 @@ Args3 step_args(clawgpu3_ctx* c, const double* q, double* qn, double dx, double 
     a.lim0 = mthlim[0];
     a.lim1 = mthlim[1];
     a.ntx = (c->mx + kOut - 1) / kOut;
-    a.nty = (c->my + kTileY - 1) / kTileY;
-    a.ntz = (c->mz + kTileZ - 1) / kTileZ;
+    a.nty = a.ntz = 0;  // set by the launcher for its tile shape
+    a.xcd = c->xcd;
     return a;
 }
 
@@ -238,7 +240,7 @@ int enqueue_step3(clawgpu3_ctx* c, int res, double dx, double dy, double dz, dou
                   const int* method, const int* mthlim, int use_fwave)
 {
     Args3 a = step_args(c, c->dq[c->cur], c->dq[res], dx, dy, dz, dt, method, mthlim, use_fwave);
-    const int e = launch_step3_tile(a, c->stream);
+    const int e = launch_step3_tile(a, c->tile, c->stream);
     if (e) return set_err(CLAWGPU_ERR_HIP, "step3 kernel launch failed");
     return CLAWGPU_OK;
 }
@@ -315,6 +317,8 @@ extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int mau
     c->L.nx = mx + 2 * mbc; c->L.ny = my + 2 * mbc; c->L.nz = mz + 2 * mbc;
     c->L.fs = (long)c->L.nx * c->L.ny * c->L.nz;
     c->seg = 64;
+    c->tile = 1;
+    c->xcd = 0;
     c->cmax = 0.0;
     const size_t planes = (size_t)c->L.fs * sizeof(double);
     int rc = CLAWGPU_OK;
@@ -543,6 +547,14 @@ extern "C" int clawgpu3_set_segment(clawgpu3_ctx* c, int cells)
 {
     if (!c || cells < 1) return set_err(CLAWGPU_ERR_ARG, "cells must be >= 1");
     c->seg = cells;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_set_tile(clawgpu3_ctx* c, int tile, int xcd_order)
+{
+    if (!c || tile < 0 || tile > 4) return set_err(CLAWGPU_ERR_ARG, "tile must be 0..4");
+    c->tile = tile;
+    c->xcd = xcd_order ? 1 : 0;
     return CLAWGPU_OK;
 }
 

@@ -37,23 +37,6 @@
 
 namespace clawgpu {
 
-// Max of a non-negative double over the wave, then one atomic per wave.
-__device__ inline void publish_cfl(double cfl, unsigned long long* bits)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) cfl = fmax(cfl, __shfl_xor(cfl, d));
-    if ((threadIdx.x & 63) == 0) atomicMax(bits, (unsigned long long)__double_as_longlong(cfl));
-}
-
-// Courant contribution max(dtdx*s, -dtdx*s) (flux2.f90:129-130); a NaN
-// speed becomes +inf so that the host refuses the step instead of
-// silently dropping it from the max.
-__device__ inline double cfl_term(double dtdx, double s)
-{
-    double t = fmax(dtdx * s, -(dtdx * s));
-    return (t == t) ? t : __builtin_huge_val();
-}
-
 // Method flags fixed at compile time (CfgFix) or read from the arguments
 // (CfgAny).  The fixed variant serves the common case method(2)=2,
 // method(3)=2, one limiter for every wave, no f-waves.

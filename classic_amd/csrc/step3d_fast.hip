@@ -5,13 +5,44 @@
 namespace clawgpu {
 namespace s3 {
 
-int launch_step3_tile(const Args3& a, hipStream_t s)
+template <int TY, int TZ, class Cfg>
+static int launch_cfg(Args3 a, hipStream_t s)
 {
-    const long nblocks = (long)a.ntx * a.nty * a.ntz;
+    a.nty = (a.L.my + TY - 1) / TY;
+    a.ntz = (a.L.mz + TZ - 1) / TZ;
+    long nblocks = (long)a.ntx * a.nty * a.ntz;
     if (nblocks <= 0) return 0;
-    hipLaunchKernelGGL((step3_tile_kernel<kTileY, kTileZ>), dim3((unsigned)nblocks), dim3(64), 0,
-                       s, a);
+    if (a.xcd) nblocks = 8 * ((nblocks + 7) / 8);
+    hipLaunchKernelGGL((step3_tile_kernel<TY, TZ, Cfg>), dim3((unsigned)nblocks), dim3(64), 0, s, a);
     return (int)hipGetLastError();
+}
+
+// the (2, 22) method with the same limiter on both waves is compiled per
+// limiter; everything else takes the flags from the arguments
+template <int TY, int TZ>
+static int launch_tile(const Args3& a, hipStream_t s)
+{
+    if (a.m2 == 2 && a.m3 == 2 && a.m4 == 2 && a.lim0 == a.lim1) {
+        switch (a.lim0) {
+        case 1: return launch_cfg<TY, TZ, Cfg3Fix<2, 2, 2, 1, 1>>(a, s);
+        case 2: return launch_cfg<TY, TZ, Cfg3Fix<2, 2, 2, 2, 2>>(a, s);
+        case 3: return launch_cfg<TY, TZ, Cfg3Fix<2, 2, 2, 3, 3>>(a, s);
+        case 4: return launch_cfg<TY, TZ, Cfg3Fix<2, 2, 2, 4, 4>>(a, s);
+        default: break;
+        }
+    }
+    return launch_cfg<TY, TZ, Cfg3Any>(a, s);
+}
+
+int launch_step3_tile(const Args3& a, int tile, hipStream_t s)
+{
+    switch (tile) {
+    case 0: return launch_tile<4, 4>(a, s);
+    case 2: return launch_tile<2, 2>(a, s);
+    case 3: return launch_tile<8, 2>(a, s);
+    case 4: return launch_tile<2, 4>(a, s);
+    default: return launch_tile<4, 2>(a, s);
+    }
 }
 
 }  // namespace s3

@@ -78,6 +78,7 @@ struct Args3 {
     double dtdx, dtdy, dtdz;
     int m2, m3, m4, fwave, lim0, lim1;
     int ntx, nty, ntz;
+    int xcd;  // 1: give each of the 8 XCDs a contiguous range of tiles
 };
 
 struct Mat {
@@ -96,6 +97,30 @@ struct Kc {
 struct Rec {
     double gp[2][3], gt[2][3], hp[2][3], ht[2][3];  // [side][slice -1, 0, +1]
 };
+
+// Method flags and limiters: read from the arguments (Cfg3Any) or fixed at
+// compile time (Cfg3Fix) so that the common configurations lose their
+// uniform branches and the slice code becomes one basic block.
+struct Cfg3Any {
+    __device__ static int m2(const struct Args3& a);
+    __device__ static int m3(const struct Args3& a);
+    __device__ static int m4(const struct Args3& a);
+    __device__ static int lim0(const struct Args3& a);
+    __device__ static int lim1(const struct Args3& a);
+};
+template <int M2, int M3, int M4, int L0, int L1>
+struct Cfg3Fix {
+    __device__ static constexpr int m2(const struct Args3&) { return M2; }
+    __device__ static constexpr int m3(const struct Args3&) { return M3; }
+    __device__ static constexpr int m4(const struct Args3&) { return M4; }
+    __device__ static constexpr int lim0(const struct Args3&) { return L0; }
+    __device__ static constexpr int lim1(const struct Args3&) { return L1; }
+};
+__device__ inline int Cfg3Any::m2(const Args3& a) { return a.m2; }
+__device__ inline int Cfg3Any::m3(const Args3& a) { return a.m3; }
+__device__ inline int Cfg3Any::m4(const Args3& a) { return a.m4; }
+__device__ inline int Cfg3Any::lim0(const Args3& a) { return a.lim0; }
+__device__ inline int Cfg3Any::lim1(const Args3& a) { return a.lim1; }
 
 __device__ inline Mat mat_up(Mat m) { return {lane_up(m.Z), lane_up(m.c)}; }
 __device__ inline Mat mat_dn(Mat m) { return {lane_dn(m.Z), lane_dn(m.c)}; }
@@ -235,64 +260,150 @@ __device__ inline void make_record(Rec& R, double ap, double cqp, double am, dou
 }
 
 // The change of the nine cells a slice position touches (step3.f90:264-335
-// written as increments): add(a, b, dp, dn, dt1, dt2) for the cell at t1
-// offset a and t2 offset b; dn is non-zero only for the slice's own cell.
-template <class F>
+// written as increments).  Target t = 3*(b+1) + (a+1) is the cell at t1
+// offset a and t2 offset b; v[t] = (p, u_n, v_t1, v_t2), u_n non-zero only
+// for the slice's own cell (t = 4).
+struct T9 {
+    double p[9], t1[9], t2[9], n;
+};
+
 __device__ inline void apply_targets(const Rec& R, double qp, double qn, double dFp, double dFn,
-                                     double d, double d1, double d2, bool trans, F&& add)
+                                     double d, double d1, double d2, bool trans, T9& T)
+{
+#pragma unroll
+    for (int t = 0; t < 9; t++) T.p[t] = T.t1[t] = T.t2[t] = 0.0;
+    T.n = qn - d * dFn;
+    T.p[4] = qp - d * dFp;
+    if (!trans) return;
+    // G slice index = t2 offset b, H slice index = t1 offset a; side 0 = 1, 1 = 2
+    T.p[0] = -d1 * R.gp[0][0] - d2 * R.hp[0][0];
+    T.t1[0] = -d1 * R.gt[0][0];
+    T.t2[0] = -d2 * R.ht[0][0];
+    T.p[1] = -d1 * (R.gp[1][0] - R.gp[0][0]) - d2 * R.hp[0][1];
+    T.t1[1] = -d1 * (R.gt[1][0] - R.gt[0][0]);
+    T.t2[1] = -d2 * R.ht[0][1];
+    T.p[2] = d1 * R.gp[1][0] - d2 * R.hp[0][2];
+    T.t1[2] = d1 * R.gt[1][0];
+    T.t2[2] = -d2 * R.ht[0][2];
+    T.p[3] = -d1 * R.gp[0][1] - d2 * (R.hp[1][0] - R.hp[0][0]);
+    T.t1[3] = -d1 * R.gt[0][1];
+    T.t2[3] = -d2 * (R.ht[1][0] - R.ht[0][0]);
+    T.p[4] = T.p[4] - d1 * (R.gp[1][1] - R.gp[0][1]) - d2 * (R.hp[1][1] - R.hp[0][1]);
+    T.t1[4] = -d1 * (R.gt[1][1] - R.gt[0][1]);
+    T.t2[4] = -d2 * (R.ht[1][1] - R.ht[0][1]);
+    T.p[5] = d1 * R.gp[1][1] - d2 * (R.hp[1][2] - R.hp[0][2]);
+    T.t1[5] = d1 * R.gt[1][1];
+    T.t2[5] = -d2 * (R.ht[1][2] - R.ht[0][2]);
+    T.p[6] = -d1 * R.gp[0][2] + d2 * R.hp[1][0];
+    T.t1[6] = -d1 * R.gt[0][2];
+    T.t2[6] = d2 * R.ht[1][0];
+    T.p[7] = -d1 * (R.gp[1][2] - R.gp[0][2]) + d2 * R.hp[1][1];
+    T.t1[7] = -d1 * (R.gt[1][2] - R.gt[0][2]);
+    T.t2[7] = d2 * R.ht[1][1];
+    T.p[8] = d1 * R.gp[1][2] + d2 * R.hp[1][2];
+    T.t1[8] = d1 * R.gt[1][2];
+    T.t2[8] = d2 * R.ht[1][2];
+}
+
+// Add T to the accumulator: idx(t, m) gives the LDS index of component m of
+// target t (an index in the trash row for targets outside the tile).  LANE
+// names the target offset that moves to another lane (0: none, 1: the t2
+// offset b = t/3 - 1, 2: the t1 offset a = t%3 - 1).  Targets are processed
+// in rounds of one lane offset: within a round every lane touches distinct
+// words (its own cells, or all its neighbours on the same side), so the
+// round's reads can all be issued before its writes; across rounds the
+// read-modify-writes are ordered, which keeps the increments that two lanes
+// add to one cell.
+template <int LANE, class Idx>
+__device__ inline void scatter(double* acc, const T9& T, bool trans, int cn, int c1, int c2,
+                               Idx&& idx)
 {
     if (!trans) {
-        add(0, 0, qp - d * dFp, qn - d * dFn, 0.0, 0.0);
+        const int ip = idx(4, 0), in = idx(4, cn);
+        const double op = acc[ip], on = acc[in];
+        acc[ip] = op + T.p[4];
+        acc[in] = on + T.n;
         return;
     }
-    // G slice index = t2 offset b, H slice index = t1 offset a; side 0 = 1, 1 = 2
-    add(-1, -1, -d1 * R.gp[0][0] - d2 * R.hp[0][0], 0.0, -d1 * R.gt[0][0], -d2 * R.ht[0][0]);
-    add(0, -1, -d1 * (R.gp[1][0] - R.gp[0][0]) - d2 * R.hp[0][1], 0.0,
-        -d1 * (R.gt[1][0] - R.gt[0][0]), -d2 * R.ht[0][1]);
-    add(1, -1, d1 * R.gp[1][0] - d2 * R.hp[0][2], 0.0, d1 * R.gt[1][0], -d2 * R.ht[0][2]);
-    add(-1, 0, -d1 * R.gp[0][1] - d2 * (R.hp[1][0] - R.hp[0][0]), 0.0, -d1 * R.gt[0][1],
-        -d2 * (R.ht[1][0] - R.ht[0][0]));
-    add(0, 0, qp - d * dFp - d1 * (R.gp[1][1] - R.gp[0][1]) - d2 * (R.hp[1][1] - R.hp[0][1]),
-        qn - d * dFn, -d1 * (R.gt[1][1] - R.gt[0][1]), -d2 * (R.ht[1][1] - R.ht[0][1]));
-    add(1, 0, d1 * R.gp[1][1] - d2 * (R.hp[1][2] - R.hp[0][2]), 0.0, d1 * R.gt[1][1],
-        -d2 * (R.ht[1][2] - R.ht[0][2]));
-    add(-1, 1, -d1 * R.gp[0][2] + d2 * R.hp[1][0], 0.0, -d1 * R.gt[0][2], d2 * R.ht[1][0]);
-    add(0, 1, -d1 * (R.gp[1][2] - R.gp[0][2]) + d2 * R.hp[1][1], 0.0,
-        -d1 * (R.gt[1][2] - R.gt[0][2]), d2 * R.ht[1][1]);
-    add(1, 1, d1 * R.gp[1][2] + d2 * R.hp[1][2], 0.0, d1 * R.gt[1][2], d2 * R.ht[1][2]);
+    constexpr int rounds = LANE == 0 ? 1 : 3;
+#pragma unroll
+    for (int g = 0; g < rounds; g++) {
+        int ts[9];
+        int nt = 0;
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            const int off = LANE == 1 ? t / 3 : t % 3;
+            if (LANE == 0 || off == g) ts[nt++] = t;
+        }
+        int ip[9], i1[9], i2[9];
+        double op[9], o1[9], o2[9];
+#pragma unroll
+        for (int k = 0; k < nt; k++) {
+            ip[k] = idx(ts[k], 0);
+            i1[k] = idx(ts[k], c1);
+            i2[k] = idx(ts[k], c2);
+        }
+        const bool centre = LANE == 0 || g == 1;
+        const int in = centre ? idx(4, cn) : 0;
+#pragma unroll
+        for (int k = 0; k < nt; k++) {
+            op[k] = acc[ip[k]];
+            o1[k] = acc[i1[k]];
+            o2[k] = acc[i2[k]];
+        }
+        const double on = centre ? acc[in] : 0.0;
+#pragma unroll
+        for (int k = 0; k < nt; k++) {
+            acc[ip[k]] = op[k] + T.p[ts[k]];
+            acc[i1[k]] = o1[k] + T.t1[ts[k]];
+            acc[i2[k]] = o2[k] + T.t2[ts[k]];
+        }
+        if (centre) acc[in] = on + T.n;
+    }
 }
 
 // One slice position from its two limited interfaces: low interface lo (A^+
 // side, cqxx cqlo) and high interface hi (A^- side, cqxx cqhi).
-template <class M, class F>
+template <class M, class Cfg>
 __device__ inline void slice_position(const Args3& a, const Iface& lo, Wv cqlo, const Iface& hi,
                                       Wv cqhi, const Mat (&N)[3][3], double d, double d1,
-                                      double d2, F&& add)
+                                      double d2, T9& T)
 {
     Rec R;
-    make_record<M>(R, lo.ap.p, cqlo.p, hi.am.p, cqhi.p, N, d, d1, d2, a.m3, a.m4);
+    make_record<M>(R, lo.ap.p, cqlo.p, hi.am.p, cqhi.p, N, d, d1, d2, Cfg::m3(a), Cfg::m4(a));
     double qp = (-d * lo.ap.p) - d * hi.am.p;
     double qn = (-d * lo.ap.n) - d * hi.am.n;
-    apply_targets(R, qp, qn, cqhi.p - cqlo.p, cqhi.n - cqlo.n, d, d1, d2, a.m3 > 0, add);
+    apply_targets(R, qp, qn, cqhi.p - cqlo.p, cqhi.n - cqlo.n, d, d1, d2, Cfg::m3(a) > 0, T);
 }
 
-template <class M>
+template <class M, class Cfg>
 __device__ inline Wv limited_cq(const Args3& a, const Iface& f, const Iface& fl, const Iface& fr,
                                 double d)
 {
-    if (a.m2 != 2) return {0.0, 0.0};
-    Wv w1 = limit<M>(f.w1, fl.w1, fr.w1, f.s1, a.lim0);
-    Wv w2 = limit<M>(f.w2, fl.w2, fr.w2, f.s2, a.lim1);
+    if (Cfg::m2(a) != 2) return {0.0, 0.0};
+    Wv w1 = limit<M>(f.w1, fl.w1, fr.w1, f.s1, Cfg::lim0(a));
+    Wv w2 = limit<M>(f.w2, fl.w2, fr.w2, f.s2, Cfg::lim1(a));
     return cqxx(w1, f.s1, w2, f.s2, d, a.fwave);
 }
 
-template <int TY, int TZ>
+template <int TY, int TZ, class Cfg>
 __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
 {
     using M = FastMath;
-    __shared__ double acc[TZ * TY * 4 * 64];
+    // accumulator of the tile, plus one row that receives the increments of
+    // cells outside the tile (so that the slice code has no branches)
+    constexpr int kTrash = TZ * TY * 4 * 64;
+    __shared__ double acc[kTrash + 64];
     const int l = threadIdx.x;
     int t = blockIdx.x;
+    if (a.xcd) {
+        // blocks are dealt to the XCDs round-robin; renumber so that XCD x
+        // works on tiles [x*per, (x+1)*per), neighbours sharing its L2
+        const int ntiles = a.ntx * a.nty * a.ntz;
+        const int per = (ntiles + 7) / 8;
+        t = (t & 7) * per + (t >> 3);
+        if (t >= ntiles) return;
+    }
     const int tx = t % a.ntx;
     t /= a.ntx;
     const int ty = t % a.nty;
@@ -303,6 +414,8 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
 #pragma unroll
     for (int e = 0; e < TZ * TY * 4; e++) acc[e * 64 + l] = 0.0;
 #define ACC(kz, jy, m, lane) acc[(((kz) * TY + (jy)) * 4 + (m)) * 64 + (lane)]
+#define SLOTI(ok, kz, jy, m, lane) ((ok) ? (((kz) * TY + (jy)) * 4 + (m)) * 64 + (lane) : kTrash + l)
+    const bool trans = Cfg::m3(a) > 0;
 
     // ---- x sweeps: slice (j', k'), one interface per lane (its low face)
     {
@@ -331,19 +444,18 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                 Iface fl, fr;
                 fl.w1 = wv_up(f.w1); fl.w2 = wv_up(f.w2);
                 fr.w1 = wv_dn(f.w1); fr.w2 = wv_dn(f.w2);
-                Wv cq = limited_cq<M>(a, f, fl, fr, d);
+                Wv cq = limited_cq<M, Cfg>(a, f, fl, fr, d);
                 Iface hi;
                 hi.am = wv_dn(f.am);
                 Wv cqhi = wv_dn(cq);
-                slice_position<M>(a, f, cq, hi, cqhi, W, d, d1, d2,
-                                  [&](int ta, int tb, double vp, double vn, double v1, double v2) {
-                                      const int jy = jp + ta - j0, kz = kp + tb - k0;
-                                      if (jy < 0 || jy >= TY || kz < 0 || kz >= TZ) return;
-                                      ACC(kz, jy, 0, l) += vp;
-                                      if (ta == 0 && tb == 0) ACC(kz, jy, 1, l) += vn;
-                                      ACC(kz, jy, 2, l) += v1;
-                                      ACC(kz, jy, 3, l) += v2;
-                                  });
+                T9 T;
+                slice_position<M, Cfg>(a, f, cq, hi, cqhi, W, d, d1, d2, T);
+                // x: n = u, t1 = v (rows, offset a), t2 = w (planes, offset b)
+                scatter<0>(acc, T, trans, 1, 2, 3, [&](int t, int m) {
+                    const int jy = jp + (t % 3 - 1) - j0, kz = kp + (t / 3 - 1) - k0;
+                    const bool ok = jy >= 0 && jy < TY && kz >= 0 && kz < TZ;
+                    return SLOTI(ok, kz, jy, m, l);
+                });
             }
         }
     }
@@ -366,7 +478,7 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
 #pragma unroll
                 for (int s = 0; s < 3; s++) C3[cc][s] = load_mat(a, L.offc(i, j, kp + s - 1));
                 if (cc >= 1) F[cc] = solve<M>(P[cc - 1], V[cc - 1], C3[cc - 1][1], P[cc], V[cc], C3[cc][1]);
-                if (cc >= 3) CQ[cc - 1] = limited_cq<M>(a, F[cc - 1], F[cc - 2], F[cc], d);
+                if (cc >= 3) CQ[cc - 1] = limited_cq<M, Cfg>(a, F[cc - 1], F[cc - 2], F[cc], d);
                 if (cc >= 4) {
                     const int r = cc - 2, jy = cc - 4;
                     Mat N[3][3];
@@ -376,15 +488,14 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                         N[s][0] = mat_up(C3[r][s]);
                         N[s][2] = mat_dn(C3[r][s]);
                     }
-                    slice_position<M>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2,
-                                      [&](int ta, int tb, double vp, double vn, double v1, double v2) {
-                                          const int kz = kp + ta - k0, ln = l + tb;
-                                          if (kz < 0 || kz >= TZ || ln < 0 || ln > 63) return;
-                                          ACC(kz, jy, 0, ln) += vp;
-                                          if (ta == 0 && tb == 0) ACC(kz, jy, 2, ln) += vn;
-                                          ACC(kz, jy, 3, ln) += v1;
-                                          ACC(kz, jy, 1, ln) += v2;
-                                      });
+                    T9 T;
+                    slice_position<M, Cfg>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2, T);
+                    // y: n = v, t1 = w (planes, offset a), t2 = u (lanes, offset b)
+                    scatter<1>(acc, T, trans, 2, 3, 1, [&](int t, int m) {
+                        const int kz = kp + (t % 3 - 1) - k0, ln = l + (t / 3 - 1);
+                        const bool ok = kz >= 0 && kz < TZ && ln >= 0 && ln <= 63;
+                        return SLOTI(ok, kz, jy, m, ln);
+                    });
                 }
             }
         }
@@ -408,7 +519,7 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
 #pragma unroll
                 for (int s = 0; s < 3; s++) C3[cc][s] = load_mat(a, L.offc(i, jp + s - 1, k));
                 if (cc >= 1) F[cc] = solve<M>(P[cc - 1], W_[cc - 1], C3[cc - 1][1], P[cc], W_[cc], C3[cc][1]);
-                if (cc >= 3) CQ[cc - 1] = limited_cq<M>(a, F[cc - 1], F[cc - 2], F[cc], d);
+                if (cc >= 3) CQ[cc - 1] = limited_cq<M, Cfg>(a, F[cc - 1], F[cc - 2], F[cc], d);
                 if (cc >= 4) {
                     const int r = cc - 2, kz = cc - 4;
                     Mat N[3][3];
@@ -418,15 +529,14 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                         N[0][s] = mat_up(C3[r][s]);
                         N[2][s] = mat_dn(C3[r][s]);
                     }
-                    slice_position<M>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2,
-                                      [&](int ta, int tb, double vp, double vn, double v1, double v2) {
-                                          const int jy = jp + tb - j0, ln = l + ta;
-                                          if (jy < 0 || jy >= TY || ln < 0 || ln > 63) return;
-                                          ACC(kz, jy, 0, ln) += vp;
-                                          if (ta == 0 && tb == 0) ACC(kz, jy, 3, ln) += vn;
-                                          ACC(kz, jy, 1, ln) += v1;
-                                          ACC(kz, jy, 2, ln) += v2;
-                                      });
+                    T9 T;
+                    slice_position<M, Cfg>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2, T);
+                    // z: n = w, t1 = u (lanes, offset a), t2 = v (rows, offset b)
+                    scatter<2>(acc, T, trans, 3, 1, 2, [&](int t, int m) {
+                        const int jy = jp + (t / 3 - 1) - j0, ln = l + (t % 3 - 1);
+                        const bool ok = jy >= 0 && jy < TY && ln >= 0 && ln <= 63;
+                        return SLOTI(ok, kz, jy, m, ln);
+                    });
                 }
             }
         }
@@ -446,6 +556,7 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
             }
     }
 #undef ACC
+#undef SLOTI
 }
 
 // ------------------------------------------------------------ split sweeps

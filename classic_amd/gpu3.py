@@ -136,6 +136,9 @@ class Grid3D:
     def set_segment(self, cells):
         check(self.lib.clawgpu3_set_segment(self.h, cells))
 
+    def set_tile(self, tile, xcd_order=True):
+        check(self.lib.clawgpu3_set_tile(self.h, tile, int(xcd_order)))
+
     def time_steps(self, dt, nsteps, method=(1, 2, 22, 0, 0, 0, 2), mthlim=(3, 3),
                    mthbc=(3, 3, 3, 3, 3, 3), use_fwave=False):
         tot = C.c_float(0.0)

@@ -16,9 +16,13 @@
  * procedure pointers of the reference contract, src/2d/claw2.f:312-402):
  *   CLAWGPU_SOLVER_ACOUSTICS  meqn 3, mwaves 2, params (rho, bulk)
  *   CLAWGPU_SOLVER_EULER      meqn 4, mwaves 3, params (gamma, efix 0/1)
- * and, for the 3D path (clawgpu3_*, src/3d/claw3.f:292-516):
+ * for the 3D path (clawgpu3_*, src/3d/claw3.f:292-516):
  *   CLAWGPU_SOLVER_VC_ACOUSTICS  meqn 4, mwaves 2, maux >= 2 with
  *                                aux(1) = impedance Z, aux(2) = sound speed c
+ * and for the 1D path (clawgpu1_*, rp1 of src/1d/claw1.f):
+ *   CLAWGPU_SOLVER_ACOUSTICS     meqn 2, mwaves 2, params (rho, bulk)
+ *   CLAWGPU_SOLVER_VC_ACOUSTICS  meqn 2, mwaves 2, aux(1) = Z, aux(2) = c
+ *   CLAWGPU_SOLVER_ADVECTION     meqn 1, mwaves 1, params (u)
  */
 #ifndef CLAWGPU_H
 #define CLAWGPU_H
@@ -37,7 +41,12 @@ enum {
     CLAWGPU_ERR_NONFINITE = -4    /* a wave speed was NaN: the step cannot be judged */
 };
 
-enum { CLAWGPU_SOLVER_ACOUSTICS = 1, CLAWGPU_SOLVER_EULER = 2, CLAWGPU_SOLVER_VC_ACOUSTICS = 3 };
+enum {
+    CLAWGPU_SOLVER_ACOUSTICS = 1,
+    CLAWGPU_SOLVER_EULER = 2,
+    CLAWGPU_SOLVER_VC_ACOUSTICS = 3,
+    CLAWGPU_SOLVER_ADVECTION = 4
+};
 
 /* Arithmetic of the step kernel.  EXACT (the default) performs the
  * reference's floating-point operations in the reference's order with no
@@ -204,11 +213,50 @@ int clawgpu3_claw3(clawgpu3_ctx* ctx, const double* geom, double tstart, double*
 /* Cells per segment of the y/z split sweeps (tuning; default 64). */
 int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
 
+/* Tile shape of the unsplit kernel (tuning): 0 = 60x4x4, 1 = 60x4x2
+ * (default), 2 = 60x2x2, 3 = 60x8x2, 4 = 60x2x4 cells per wavefront;
+ * xcd_order = 1 hands each XCD a contiguous range of tiles (default 0). */
+int clawgpu3_set_tile(clawgpu3_ctx* ctx, int tile, int xcd_order);
+
 /* Benchmark helper (no reference counterpart): nsteps fixed-dt unsplit
  * steps (bc3 + step) timed with HIP events, as clawgpu_time_steps. */
 int clawgpu3_time_steps(clawgpu3_ctx* ctx, const double* geom, double dt, const int* method,
                         const int* mthlim, const int* mthbc, int use_fwave, int nsteps,
                         float* total_ms, float* step_kernel_ms, double* cfl);
+
+
+/* ------------------------------------------------------------------ 1D */
+
+typedef struct clawgpu1_ctx clawgpu1_ctx;
+
+/* Create / destroy a device context for one 1D grid, q(meqn, 1-mbc:mx+mbc)
+ * and aux(maux, 1-mbc:mx+mbc) in the reference layout; replaces the
+ * allocations of src/1d/claw1ez.f:150-220.  mbc must be 2. */
+int clawgpu1_create(clawgpu1_ctx** ctx, int meqn, int mwaves, int maux, int mbc, int mx,
+                    int solver_id, const double* params, int nparams);
+int clawgpu1_destroy(clawgpu1_ctx* ctx);
+int clawgpu1_set_q(clawgpu1_ctx* ctx, const double* q);
+int clawgpu1_get_q(clawgpu1_ctx* ctx, double* q);
+int clawgpu1_set_aux(clawgpu1_ctx* ctx, const double* aux);
+
+/* Ghost cells of the device q: src/1d/bc1.f with mthbc(1:2) in {1, 2, 3}. */
+int clawgpu1_bc1(clawgpu1_ctx* ctx, const int* mthbc);
+
+/* One step, the contract of step1 (src/1d/step1.f90): the device q with
+ * ghost cells set is updated (interior cells; the reference also changes
+ * ghost cells 0 and mx+1, which the next bc1 overwrites), *cfl is the Courant
+ * number.  method(2) in {1, 2}, method(6) = capacity index (0 = none),
+ * mthlim(1:mwaves) in 0..5.  Bit for bit with the reference. */
+int clawgpu1_step1(clawgpu1_ctx* ctx, double dx, double dt, const int* method, const int* mthlim,
+                   int use_fwave, double* cfl);
+
+/* The claw1 time loop (src/1d/claw1.f) on the device, with its order of
+ * operations (the new dt is chosen before the Courant check; a retake does
+ * not clamp dt to tend again) and info codes (0, 2, 3, 11, 12; -5 for a NaN
+ * wave speed). */
+int clawgpu1_claw1(clawgpu1_ctx* ctx, double dx, double tstart, double* tend, double* dtv,
+                   double* cflv, int* nv, const int* method, const int* mthlim, const int* mthbc,
+                   int use_fwave, int* info);
 
 #ifdef __cplusplus
 }
