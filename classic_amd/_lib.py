@@ -23,6 +23,7 @@ SOLVER_ACOUSTICS = 1
 SOLVER_EULER = 2
 SOLVER_VC_ACOUSTICS = 3
 SOLVER_ADVECTION = 4
+SOLVER_VC_ADVECTION = 5
 
 ARITH_EXACT = 0
 ARITH_FAST = 1
@@ -41,6 +42,7 @@ _SIGS = {
     "clawgpu_destroy": ([_V], _i),
     "clawgpu_set_q": ([_V, _D], _i),
     "clawgpu_get_q": ([_V, _D], _i),
+    "clawgpu_set_aux": ([_V, _D], _i),
     "clawgpu_bc2": ([_V, _I], _i),
     "clawgpu_step2": ([_V, _d, _d, _d, _I, _I, _i, _D], _i),
     "clawgpu_dimsp2": ([_V, _d, _d, _d, _I, _I, _D, _i, _D], _i),

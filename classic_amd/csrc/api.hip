@@ -115,6 +115,7 @@ __global__ void planes_to_aos(const double* __restrict__ src, double* __restrict
 using namespace clawgpu;
 
 #include "ctx.hpp"
+#include "step2g.hpp"
 
 thread_local char g_errbuf[256];
 
@@ -138,6 +139,10 @@ static int solver_shape(int solver, int* meqn, int* mwaves)
         *meqn = 4;
         *mwaves = 3;
         return 0;
+    case CLAWGPU_SOLVER_VC_ADVECTION:
+        *meqn = 1;
+        *mwaves = 1;
+        return 0;
     default: return -1;
     }
 }
@@ -151,7 +156,11 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     if (solver_shape(solver_id, &sm, &sw) != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "unknown solver id");
     if (meqn != sm || mwaves != sw)
         return set_err(CLAWGPU_ERR_ARG, "meqn/mwaves do not match the solver");
-    if (maux != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "aux arrays are not supported by this solver");
+    const bool general = solver_id == CLAWGPU_SOLVER_VC_ADVECTION;
+    if (!general && maux != 0)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "aux arrays are not supported by this solver");
+    if (general && maux < 2)
+        return set_err(CLAWGPU_ERR_ARG, "vc advection needs maux >= 2 (edge velocities u, v)");
     if (mbc < 2) return set_err(CLAWGPU_ERR_ARG, "mbc must be >= 2 (claw2 info=1)");
     if (mx < 1 || my < 1) return set_err(CLAWGPU_ERR_ARG, "grid must have at least one cell");
     SolverParams P{};
@@ -162,7 +171,7 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
         if (!(P.rho > 0.0) || !(P.bulk > 0.0)) return set_err(CLAWGPU_ERR_ARG, "rho and bulk must be positive");
         P.cc = sqrt(P.bulk / P.rho);
         P.zz = P.rho * P.cc;
-    } else {
+    } else if (solver_id == CLAWGPU_SOLVER_EULER) {
         if (nparams < 2 || !params) return set_err(CLAWGPU_ERR_ARG, "euler needs (gamma, efix)");
         P.gamma = params[0];
         P.gamma1 = params[0] - 1.0;
@@ -188,7 +197,7 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     c->seglen = 32;
     c->arith = ARITH_EXACT;
     size_t qbytes = (size_t)c->L.fs * meqn * sizeof(double);
-    size_t abytes = (size_t)(mx + 2 * mbc) * (my + 2 * mbc) * meqn * sizeof(double);
+    size_t abytes = (size_t)(mx + 2 * mbc) * (my + 2 * mbc) * (meqn > maux ? meqn : maux) * sizeof(double);
     hipError_t e = hipSuccess;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc((void**)&c->dq[0], qbytes)) != hipSuccess ||
@@ -196,7 +205,9 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
         (e = hipMalloc((void**)&c->dq[2], qbytes)) != hipSuccess ||
         (e = hipMalloc((void**)&c->staging, abytes)) != hipSuccess ||
         (e = hipMalloc((void**)&c->dcfl, 4 * sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipHostMalloc((void**)&c->hcfl, 4 * sizeof(double), hipHostMallocDefault)) != hipSuccess) {
+        (e = hipHostMalloc((void**)&c->hcfl, 4 * sizeof(double), hipHostMallocDefault)) != hipSuccess ||
+        (general && (e = hipMalloc((void**)&c->daux, (size_t)c->L.fs * maux * sizeof(double))) != hipSuccess) ||
+        (general && (e = hipMalloc((void**)&c->grec, (size_t)c->L.fs * 10 * sizeof(double))) != hipSuccess)) {
         snprintf(g_errbuf, sizeof g_errbuf, "allocation failed: %s", hipGetErrorString(e));
         clawgpu_destroy(c);
         return CLAWGPU_ERR_HIP;
@@ -224,6 +235,8 @@ extern "C" int clawgpu_destroy(clawgpu_ctx* c)
     if (c->staging) (void)hipFree(c->staging);
     if (c->dcfl) (void)hipFree(c->dcfl);
     if (c->hcfl) (void)hipHostFree(c->hcfl);
+    if (c->daux) (void)hipFree(c->daux);
+    if (c->grec) (void)hipFree(c->grec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CLAWGPU_OK;
@@ -243,6 +256,21 @@ extern "C" int clawgpu_set_q(clawgpu_ctx* c, const double* q)
     int nb = (int)((n + 255) / 256);
     hipLaunchKernelGGL(aos_to_planes, dim3(nb), dim3(256), 0, c->stream, c->staging, c->dq[c->cur],
                        c->L, c->meqn);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_set_aux(clawgpu_ctx* c, const double* aux)
+{
+    if (!c || !aux) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    if (!c->daux) return set_err(CLAWGPU_ERR_ARG, "context has no aux arrays (maux = 0)");
+    long n = ncells_ghost(c);
+    HIPCHK(c, hipMemcpyAsync(c->staging, aux, (size_t)n * c->maux * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    int nb = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(aos_to_planes, dim3(nb), dim3(256), 0, c->stream, c->staging, c->daux, c->L,
+                       c->maux);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CLAWGPU_OK;
@@ -296,12 +324,19 @@ extern "C" int clawgpu_bc2(clawgpu_ctx* c, const int* mthbc)
     return CLAWGPU_OK;
 }
 
+static bool general_path(const clawgpu_ctx* c) { return c->solver == CLAWGPU_SOLVER_VC_ADVECTION; }
+
 static int check_method(const clawgpu_ctx* c, const int* method, const int* mthlim)
 {
     if (method[1] != 1 && method[1] != 2) return set_err(CLAWGPU_ERR_ARG, "method(2) must be 1 or 2");
     if (method[2] < 0 || method[2] > 2)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "method(3) must be 0, 1 or 2 for the unsplit step");
-    if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported by this solver");
+    if (method[5] != 0 && !general_path(c))
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported by this solver");
+    if (method[5] < 0 || method[5] > c->maux)
+        return set_err(CLAWGPU_ERR_ARG, "method(6) must be 0..maux");
+    if (general_path(c) && mthlim[0] == kLimTable)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "the learned limiter runs with the built-in solvers only");
     if (c->mwaves > kMaxWaves) return set_err(CLAWGPU_ERR_UNSUPPORTED, "too many waves");
     for (int mw = 0; mw < c->mwaves; mw++) {
         if (mthlim[mw] == kLimTable && !learned_ready(const_cast<clawgpu_ctx*>(c)))
@@ -324,6 +359,25 @@ static bool uses_table(const clawgpu_ctx* c, const int* mthlim)
 static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const int* method,
                         const int* mthlim, int use_fwave)
 {
+    if (general_path(c)) {
+        g2::GenArgs g;
+        g.q = c->dq[c->cur];
+        g.qn = c->dq[(c->cur + 1) % 3];
+        g.aux = c->daux;
+        g.rec = c->grec;
+        g.L = c->L;
+        g.afs = c->L.fs;
+        g.dtdx = dt / dx;
+        g.dtdy = dt / dy;
+        g.capa = method[5];
+        g.order = method[1];
+        g.trans = method[2];
+        g.fwave = use_fwave ? 1 : 0;
+        g.lim0 = mthlim[0];
+        g.cfl_bits = c->dcfl;
+        HIPCHK(c, (hipError_t)launch_step2_general(g, c->stream));
+        return CLAWGPU_OK;
+    }
     StepArgs a;
     a.qold = c->dq[c->cur];
     a.qnew = c->dq[(c->cur + 1) % 3];
@@ -451,6 +505,8 @@ extern "C" int clawgpu_dimsp2(clawgpu_ctx* c, double dx, double dy, double dt, c
     if (!c || !method || !mthlim || !cflv || !cfl) return set_err(CLAWGPU_ERR_ARG, "null argument");
     if (method[2] != -1 && method[2] != -2) return set_err(CLAWGPU_ERR_ARG, "method(3) must be -1 or -2");
     if (method[1] != 1 && method[1] != 2) return set_err(CLAWGPU_ERR_ARG, "method(2) must be 1 or 2");
+    if (general_path(c))
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "split steps are not supported for solvers with aux arrays");
     if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported");
     for (int mw = 0; mw < c->mwaves; mw++)
         if (mthlim[mw] < 0 || mthlim[mw] > 5) return set_err(CLAWGPU_ERR_ARG, "invalid limiter method");
@@ -483,7 +539,10 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
         return set_err(CLAWGPU_ERR_ARG, "method(3) must be -2, -1, 0, 1 or 2");
     if (split && uses_table(c, mthlim))
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "the learned limiter runs with the unsplit step only");
-    if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported");
+    if (split && general_path(c))
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "split steps are not supported for solvers with aux arrays");
+    if (method[5] != 0 && !general_path(c))
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported");
     if (check_bc(mthbc) != 0)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides)");
     int info = 0;

@@ -24,6 +24,8 @@ struct clawgpu_ctx {
     hipStream_t stream;
     int seglen;
     int arith;
+    double* daux = nullptr;   // aux planes (solvers with aux arrays)
+    double* grec = nullptr;   // records of the general step (step2g.hip)
     // learned limiter state (allocated on first use)
     double *phix = nullptr, *phiy = nullptr;  // phi tables read by the step kernel
     void* learned = nullptr;                   // LearnedState (learned.hip)

@@ -12,9 +12,9 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import SOLVER_ACOUSTICS, SOLVER_EULER, check
+from ._lib import SOLVER_ACOUSTICS, SOLVER_EULER, SOLVER_VC_ADVECTION, check
 
-SOLVER_SHAPE = {SOLVER_ACOUSTICS: (3, 2), SOLVER_EULER: (4, 3)}
+SOLVER_SHAPE = {SOLVER_ACOUSTICS: (3, 2), SOLVER_EULER: (4, 3), SOLVER_VC_ADVECTION: (1, 1)}
 
 
 def _ip(a):
@@ -82,6 +82,13 @@ class Grid2D:
         if q.shape != self.shape:
             raise ValueError(f"q has shape {q.shape}, expected {self.shape}")
         check(self.lib.clawgpu_set_q(self.h, _dp(q)))
+
+    def set_aux(self, aux):
+        aux = np.ascontiguousarray(aux, dtype=np.float64)
+        want = self.shape[:2] + (self.maux,)
+        if aux.shape != want:
+            raise ValueError(f"aux has shape {aux.shape}, expected {want}")
+        check(self.lib.clawgpu_set_aux(self.h, _dp(aux)))
 
     def get_q(self):
         q = np.empty(self.shape)

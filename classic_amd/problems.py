@@ -135,3 +135,20 @@ def acoustics1d_heterogeneous(mx=500, mbc=2, lower=-5.0, upper=5.0, rhol=1.0, cl
     aux[left] = (rhol * cl, cl)
     aux[~left] = (rhor * cr, cr)
     return q, aux, dx
+
+
+def annulus_hump(mx=40, my=120, mbc=2, xlower=0.2, xupper=1.0, ylower=0.0,
+                 yupper=2.0 * np.pi, A=(1.0, -1.0), beta=(40.0, 40.0), centre=((-0.5, 0.0),
+                                                                              (0.5, 0.0))):
+    """examples/advection_2d_annulus qinit.f: two Gaussian humps
+    A_k exp(-beta_k |p - c_k|^2) at the mapped cell centres p = (r cos th,
+    r sin th) of the (r, th) grid.  Returns q of shape (my+2mbc, mx+2mbc, 1)."""
+    xc, _ = cell_centers(xlower, xupper, mx, mbc)
+    yc, _ = cell_centers(ylower, yupper, my, mbc)
+    R, T = np.meshgrid(xc, yc)
+    xp, yp = R * np.cos(T), R * np.sin(T)
+    q = np.zeros((my + 2 * mbc, mx + 2 * mbc, 1))
+    v = sum(a * np.exp(-b * ((xp - c[0]) ** 2 + (yp - c[1]) ** 2))
+            for a, b, c in zip(A, beta, centre))
+    q[mbc:-mbc, mbc:-mbc, 0] = v[mbc:-mbc, mbc:-mbc]
+    return q

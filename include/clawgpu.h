@@ -14,8 +14,13 @@
  *
  * Riemann solvers are selected by id (the GPU cannot call the host rpn2/rpt2
  * procedure pointers of the reference contract, src/2d/claw2.f:312-402):
- *   CLAWGPU_SOLVER_ACOUSTICS  meqn 3, mwaves 2, params (rho, bulk)
- *   CLAWGPU_SOLVER_EULER      meqn 4, mwaves 3, params (gamma, efix 0/1)
+ *   CLAWGPU_SOLVER_ACOUSTICS     meqn 3, mwaves 2, params (rho, bulk)
+ *   CLAWGPU_SOLVER_EULER         meqn 4, mwaves 3, params (gamma, efix 0/1)
+ *   CLAWGPU_SOLVER_VC_ADVECTION  meqn 1, mwaves 1, maux >= 2 with aux(1) = u
+ *                                at the left edge, aux(2) = v at the bottom
+ *                                edge of each cell (the annulus example);
+ *                                the capacity function (method(6) > 0) is
+ *                                supported with this solver
  * for the 3D path (clawgpu3_*, src/3d/claw3.f:292-516):
  *   CLAWGPU_SOLVER_VC_ACOUSTICS  meqn 4, mwaves 2, maux >= 2 with
  *                                aux(1) = impedance Z, aux(2) = sound speed c
@@ -45,7 +50,8 @@ enum {
     CLAWGPU_SOLVER_ACOUSTICS = 1,
     CLAWGPU_SOLVER_EULER = 2,
     CLAWGPU_SOLVER_VC_ACOUSTICS = 3,
-    CLAWGPU_SOLVER_ADVECTION = 4
+    CLAWGPU_SOLVER_ADVECTION = 4,
+    CLAWGPU_SOLVER_VC_ADVECTION = 5
 };
 
 /* Arithmetic of the step kernel.  EXACT (the default) performs the
@@ -65,7 +71,7 @@ const char* clawgpu_last_error(void);
  * of q, aux and the work array in src/2d/claw2ez.f:173-263 (the library owns
  * two device copies of q used as qold/qnew in turn, so copyq2,
  * src/2d/copyq2.f:4-20, becomes a buffer swap).  maux must be 0 for the
- * built-in solvers. */
+ * acoustics and Euler solvers and >= 2 for variable-coefficient advection. */
 int clawgpu_create(clawgpu_ctx** ctx, int meqn, int mwaves, int maux, int mbc, int mx, int my,
                    int solver_id, const double* params, int nparams);
 int clawgpu_destroy(clawgpu_ctx* ctx);
@@ -73,6 +79,10 @@ int clawgpu_destroy(clawgpu_ctx* ctx);
 /* Copy q (reference layout, ghost cells included) to / from the device. */
 int clawgpu_set_q(clawgpu_ctx* ctx, const double* q);
 int clawgpu_get_q(clawgpu_ctx* ctx, double* q);
+
+/* Copy aux(maux, 1-mbc:mx+mbc, 1-mbc:my+mbc) to the device: what setaux
+ * fills in src/2d/claw2ez.f:229-232 (contexts created with maux > 0). */
+int clawgpu_set_aux(clawgpu_ctx* ctx, const double* aux);
 
 /* Fill the ghost cells of the device q: src/2d/bc2.f:4-215 with
  * mthbc(1:4) = (left, right, bottom, top) in {1 extrap, 2 periodic, 3 wall}.

@@ -17,6 +17,7 @@
  *   orc_dimsp2       src/2d/dimsp2.f:4-98
  *   orc_bc2          src/2d/bc2.f:4-215
  *   orc_claw2        src/2d/claw2.f:494-784 (time loop, CFL retake, info codes)
+ *   orc_annulus_aux  examples/advection_2d_annulus/setaux.f (test data)
  * Riemann solvers (the reference only fixes their contract,
  * src/2d/claw2.f:312-402; the arithmetic lives in the absent clawpack/riemann
  * repository, so these are written from LeVeque, "Finite Volume Methods for
@@ -307,11 +308,51 @@ void orc_rpt2_euler(int ixy, int imp, int maxm, int meqn, int mwaves, int maux, 
     }
 }
 
+/* Variable-coefficient advection q_t + (u q)_x + (v q)_y = 0 with edge
+ * velocities in aux: aux(1,i,j) = u at the left edge of cell (i,j),
+ * aux(2,i,j) = v at its bottom edge (rpn2/rpt2_vc_advection of the absent
+ * clawpack/riemann, named by examples/advection_2d_annulus/Makefile; FVMHP
+ * 9.5 and 20.5).  The interface between cells i-1 and i moves with the edge
+ * velocity of cell i; a transverse fluctuation in cell i1 (i-1 for imp = 1,
+ * i for imp = 2) goes down with the bottom-edge velocity of that cell (aux2)
+ * and up with the bottom-edge velocity of the cell above it (aux3). */
+void orc_rpn2_vc_advection(int ixy, int maxm, int meqn, int mwaves, int maux, int mbc, int mx,
+                           const double* ql, const double* qr, const double* auxl,
+                           const double* auxr, double* wave, double* s, double* amdq,
+                           double* apdq)
+{
+    (void)maxm; (void)meqn; (void)mwaves; (void)auxr;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double w = ql[(size_t)(i + mbc - 1)] - qr[(size_t)(i - 1 + mbc - 1)];
+        double sp = auxl[(size_t)(i + mbc - 1) * maux + (ixy - 1)];
+        wave[(size_t)(i + mbc - 1)] = w;
+        s[(size_t)(i + mbc - 1)] = sp;
+        amdq[(size_t)(i + mbc - 1)] = fmin(sp, 0.0) * w;
+        apdq[(size_t)(i + mbc - 1)] = fmax(sp, 0.0) * w;
+    }
+}
+
+void orc_rpt2_vc_advection(int ixy, int imp, int maxm, int meqn, int mwaves, int maux, int mbc,
+                           int mx, const double* ql, const double* qr, const double* aux1,
+                           const double* aux2, const double* aux3, const double* asdq,
+                           double* bmasdq, double* bpasdq)
+{
+    (void)maxm; (void)meqn; (void)mwaves; (void)ql; (void)qr; (void)aux1;
+    const int kv = 3 - ixy;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        int i1 = i - 2 + imp;
+        double a = asdq[(size_t)(i + mbc - 1)];
+        bmasdq[(size_t)(i + mbc - 1)] = fmin(aux2[(size_t)(i1 + mbc - 1) * maux + (kv - 1)], 0.0) * a;
+        bpasdq[(size_t)(i + mbc - 1)] = fmax(aux3[(size_t)(i1 + mbc - 1) * maux + (kv - 1)], 0.0) * a;
+    }
+}
+
 orc_rpn2_t orc_get_rpn2(int solver_id)
 {
     switch (solver_id) {
     case ORC_SOLVER_ACOUSTICS: return orc_rpn2_acoustics;
     case ORC_SOLVER_EULER: return orc_rpn2_euler;
+    case ORC_SOLVER_VC_ADVECTION: return orc_rpn2_vc_advection;
     default: return NULL;
     }
 }
@@ -321,8 +362,47 @@ orc_rpt2_t orc_get_rpt2(int solver_id)
     switch (solver_id) {
     case ORC_SOLVER_ACOUSTICS: return orc_rpt2_acoustics;
     case ORC_SOLVER_EULER: return orc_rpt2_euler;
+    case ORC_SOLVER_VC_ADVECTION: return orc_rpt2_vc_advection;
     default: return NULL;
     }
+}
+
+/* examples/advection_2d_annulus/setaux.f with mapc2p.f and stream.f:
+ * polar map (r, theta) -> (r cos theta, r sin theta), stream function
+ * psi = pi (x^2 + y^2); aux(1) = (psi(top-left) - psi(bottom-left)) / dyc,
+ * aux(2) = -(psi(bottom-right) - psi(bottom-left)) / dxc, aux(3) = physical
+ * area / (dxc dyc) by the shoelace formula.  Used by the tests to build the
+ * example's aux array exactly as the reference's setaux does. */
+static double annulus_stream(double x, double y) { return 3.1415926535897931 * (x * x + y * y); }
+
+void orc_annulus_aux(int mbc, int mx, int my, double xlower, double ylower, double dxc,
+                     double dyc, double* aux)
+{
+    for (int j = 1 - mbc; j <= my + mbc; j++)
+        for (int i = 1 - mbc; i <= mx + mbc; i++) {
+            double xc[5], yc[5], xp[5], yp[5];
+            xc[0] = xlower + (i - 1) * dxc;
+            yc[0] = ylower + (j - 1) * dyc;
+            xc[1] = xc[0];
+            yc[1] = yc[0] + dyc;
+            xc[2] = xc[0] + dxc;
+            yc[2] = yc[0] + dyc;
+            xc[3] = xc[0] + dxc;
+            yc[3] = yc[0];
+            for (int k = 0; k < 4; k++) {
+                xp[k] = xc[k] * cos(yc[k]);
+                yp[k] = xc[k] * sin(yc[k]);
+            }
+            double* a = aux + ((size_t)(j + mbc - 1) * (mx + 2 * mbc) + (i + mbc - 1)) * 3;
+            a[0] = (annulus_stream(xp[1], yp[1]) - annulus_stream(xp[0], yp[0])) / dyc;
+            a[1] = -(annulus_stream(xp[3], yp[3]) - annulus_stream(xp[0], yp[0])) / dxc;
+            xp[4] = xp[0];
+            yp[4] = yp[0];
+            double area = 0.0;
+            for (int ic = 0; ic < 4; ic++)
+                area = area + 0.5 * (yp[ic] + yp[ic + 1]) * (xp[ic + 1] - xp[ic]);
+            a[2] = area / (dxc * dyc);
+        }
 }
 
 /* ---------------------------------------------------------------- limiter */
@@ -781,7 +861,10 @@ int orc_dimsp2(int meqn, int mwaves, int maux, int mbc, int mx, int my, const do
 /* -------------------------------------------------------------------- bc2 */
 
 /* bc2.f:4-215: left, right, bottom, top, in that order (corners come from
- * the bottom/top pass over the already-filled ghost columns). */
+ * the bottom/top pass over the already-filled ghost columns).  A wall
+ * negates component 2 (x) or 3 (y); the reference does so whatever meqn is,
+ * writing outside q when meqn is smaller -- here the negation is skipped
+ * then. */
 int orc_bc2(int meqn, int mbc, int mx, int my, double* q, const int* mthbc)
 {
     for (int side = 0; side < 4; side++)
@@ -795,7 +878,7 @@ int orc_bc2(int meqn, int mbc, int mx, int my, double* q, const int* mthbc)
                 else Q2(q, m, 1 - ibc, j) = Q2(q, m, ibc, j);
             }
         }
-    if (mthbc[0] == 3)
+    if (mthbc[0] == 3 && meqn >= 2)
         for (int j = 1 - mbc; j <= my + mbc; j++)
             for (int ibc = 1; ibc <= mbc; ibc++) Q2(q, 2, 1 - ibc, j) = -Q2(q, 2, ibc, j);
     /* right */
@@ -807,7 +890,7 @@ int orc_bc2(int meqn, int mbc, int mx, int my, double* q, const int* mthbc)
                 else Q2(q, m, mx + ibc, j) = Q2(q, m, mx + 1 - ibc, j);
             }
         }
-    if (mthbc[1] == 3)
+    if (mthbc[1] == 3 && meqn >= 2)
         for (int j = 1 - mbc; j <= my + mbc; j++)
             for (int ibc = 1; ibc <= mbc; ibc++) Q2(q, 2, mx + ibc, j) = -Q2(q, 2, mx + 1 - ibc, j);
     /* bottom */
@@ -818,7 +901,7 @@ int orc_bc2(int meqn, int mbc, int mx, int my, double* q, const int* mthbc)
                 else if (mthbc[2] == 2) Q2(q, m, i, 1 - jbc) = Q2(q, m, i, my + 1 - jbc);
                 else Q2(q, m, i, 1 - jbc) = Q2(q, m, i, jbc);
             }
-    if (mthbc[2] == 3)
+    if (mthbc[2] == 3 && meqn >= 3)
         for (int jbc = 1; jbc <= mbc; jbc++)
             for (int i = 1 - mbc; i <= mx + mbc; i++) Q2(q, 3, i, 1 - jbc) = -Q2(q, 3, i, jbc);
     /* top */
@@ -829,7 +912,7 @@ int orc_bc2(int meqn, int mbc, int mx, int my, double* q, const int* mthbc)
                 else if (mthbc[3] == 2) Q2(q, m, i, my + jbc) = Q2(q, m, i, jbc);
                 else Q2(q, m, i, my + jbc) = Q2(q, m, i, my + 1 - jbc);
             }
-    if (mthbc[3] == 3)
+    if (mthbc[3] == 3 && meqn >= 3)
         for (int jbc = 1; jbc <= mbc; jbc++)
             for (int i = 1 - mbc; i <= mx + mbc; i++) Q2(q, 3, i, my + jbc) = -Q2(q, 3, i, my + 1 - jbc);
     return 0;

@@ -10,7 +10,7 @@
 extern "C" {
 #endif
 
-enum { ORC_SOLVER_ACOUSTICS = 1, ORC_SOLVER_EULER = 2 };
+enum { ORC_SOLVER_ACOUSTICS = 1, ORC_SOLVER_EULER = 2, ORC_SOLVER_VC_ADVECTION = 5 };
 
 typedef struct {
     double rho, bulk, cc, zz; /* acoustics: density, bulk modulus, sound speed, impedance */
@@ -45,6 +45,17 @@ void orc_rpn2_euler(int ixy, int maxm, int meqn, int mwaves, int maux, int mbc, 
 void orc_rpt2_euler(int ixy, int imp, int maxm, int meqn, int mwaves, int maux, int mbc, int mx,
                     const double* ql, const double* qr, const double* aux1, const double* aux2,
                     const double* aux3, const double* asdq, double* bmasdq, double* bpasdq);
+
+void orc_rpn2_vc_advection(int ixy, int maxm, int meqn, int mwaves, int maux, int mbc, int mx,
+                           const double* ql, const double* qr, const double* auxl,
+                           const double* auxr, double* wave, double* s, double* amdq,
+                           double* apdq);
+void orc_rpt2_vc_advection(int ixy, int imp, int maxm, int meqn, int mwaves, int maux, int mbc,
+                           int mx, const double* ql, const double* qr, const double* aux1,
+                           const double* aux2, const double* aux3, const double* asdq,
+                           double* bmasdq, double* bpasdq);
+void orc_annulus_aux(int mbc, int mx, int my, double xlower, double ylower, double dxc,
+                     double dyc, double* aux);
 
 int orc_limiter(int maxm, int meqn, int mwaves, int mbc, int mx, double* wave, const double* s,
                 const int* mthlim);

@@ -24,6 +24,7 @@ REF_SO = os.path.join(HERE, "_ref", "libclawref2d.so")
 
 SOLVER_ACOUSTICS = 1
 SOLVER_EULER = 2
+SOLVER_VC_ADVECTION = 5
 
 
 class SolverParams(C.Structure):
@@ -252,3 +253,15 @@ def limiter(wave, s, mthlim, mbc=2):
     if rc != 0:
         raise ValueError("invalid limiter")
     return w
+
+
+def annulus_aux(mx, my, mbc=2, xlower=0.2, xupper=1.0, ylower=0.0, yupper=2.0 * np.pi):
+    """aux(3, 1-mbc:mx+mbc, 1-mbc:my+mbc) of examples/advection_2d_annulus
+    (edge velocities from the stream function, capacity = cell area ratio),
+    computed by the oracle's C restatement of its setaux.f."""
+    dxc = (xupper - xlower) / mx
+    dyc = (yupper - ylower) / my
+    aux = np.zeros((my + 2 * mbc, mx + 2 * mbc, 3))
+    lib().orc_annulus_aux(mbc, mx, my, C.c_double(xlower), C.c_double(ylower), C.c_double(dxc),
+                          C.c_double(dyc), _dp(aux))
+    return aux, dxc, dyc

@@ -24,6 +24,7 @@ from classic_amd import problems as P  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 GAMMA = 1.4
+REF_EX = "/root/reference/examples"
 
 
 def acoustics_problem(mx, my, method=(1, 2, 2, 0, 0, 0, 0), lim=(4, 4), mthbc=(3, 3, 3, 3),
@@ -102,6 +103,23 @@ def main():
         p = acoustics_problem(11, 13, mthbc=mthbc)
         bc["q_" + "".join(map(str, mthbc))] = O.bc2(p, q, backend="ref")
     out["bc2_11x13"] = bc
+
+    # 7) the annulus example (variable-coefficient advection with capacity):
+    #    frames at t = 0.25, 0.5 as its regression test runs it, and the
+    #    regression sums its directory holds
+    aux, dxc, dyc = O.annulus_aux(40, 120)
+    q = P.annulus_hump(40, 120)
+    p = O.Problem(1, 1, 40, 120, O.SOLVER_VC_ADVECTION, O.solver_params(), maux=3,
+                  method=(1, 2, 2, 0, 0, 3, 3), mthlim=[3], mthbc=(1, 1, 2, 2))
+    frames, dtv, t = [], [0.1, 1e9], 0.0
+    for _ in range(2):
+        r = O.claw2(p, q, dxc, dyc, t, t + 0.25, dtv, [1.0, 0.9], 50000, aux=aux, backend="ref")
+        q, dtv, t = r.q, [r.dtv[4], 1e9], r.tend
+        frames.append(q[2:-2, 2:-2].copy())
+    reg = np.array([np.loadtxt(os.path.join(REF_EX, "advection_2d_annulus", "regression_data",
+                                            f"frame000{k}.txt")) for k in (1, 2)])
+    out["annulus_40x120"] = dict(q=np.array(frames), t=np.array([0.25, 0.5]),
+                                 regression_sums=reg)
 
     for name, rec in out.items():
         np.savez_compressed(os.path.join(HERE, name + ".npz"),

@@ -415,3 +415,56 @@ def test_nan_state_stops_the_loop():
     p = euler_problem(45, 38, mthbc=(1, 3, 2, 2))
     r = O.claw2(p, q, dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 15)
     assert r.info == -5
+
+
+# --------------------------------------------- aux, capacity: annulus example
+
+def _annulus():
+    aux, dxc, dyc = O.annulus_aux(40, 120)
+    q = P.annulus_hump(40, 120)
+    p = O.Problem(1, 1, 40, 120, O.SOLVER_VC_ADVECTION, O.solver_params(), maux=3,
+                  method=(1, 2, 2, 0, 0, 3, 3), mthlim=[3], mthbc=(1, 1, 2, 2))
+    return p, q, aux, dxc, dyc
+
+
+def test_golden_annulus_capacity_and_aux():
+    """examples/advection_2d_annulus: the oracle's capa/aux path against the
+    frames the reference Fortran wrote and the example's regression sums
+    (which are round-off: the two humps have opposite signs)."""
+    g = np.load(os.path.join(GOLD, "annulus_40x120.npz"))
+    p, q, aux, dxc, dyc = _annulus()
+    dtv, t = [0.1, 1e9], 0.0
+    for k in range(2):
+        r = O.claw2(p, q, dxc, dyc, t, t + 0.25, dtv, [1.0, 0.9], 50000, aux=aux)
+        assert r.info == 0
+        q, dtv, t = r.q, [r.dtv[4], 1e9], r.tend
+        assert t == g["t"][k]
+        np.testing.assert_array_equal(q[2:-2, 2:-2], g["q"][k])
+        assert abs(q[2:-2, 2:-2, 0].sum() - g["regression_sums"][k]) < 1e-10
+
+
+def test_capacity_weighted_mass_is_conserved_on_the_annulus():
+    """With periodic theta and no radial flow (the stream function is
+    radial), sum(capa * q) changes only by round-off."""
+    p, q, aux, dxc, dyc = _annulus()
+    m0 = (q[2:-2, 2:-2, 0] * aux[2:-2, 2:-2, 2]).sum()
+    r = O.claw2(p, q, dxc, dyc, 0.0, 0.3, [0.01, 1e9], [1.0, 0.9], 5000, aux=aux)
+    m1 = (r.q[2:-2, 2:-2, 0] * aux[2:-2, 2:-2, 2]).sum()
+    assert abs(m1 - m0) <= 1e-12 * np.abs(q).sum()
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref absent")
+@pytest.mark.parametrize("method", [(1, 2, 2, 0, 0, 3, 3), (1, 2, 1, 0, 0, 3, 3),
+                                    (1, 1, 2, 0, 0, 0, 3), (1, 2, 0, 0, 0, 3, 3)])
+def test_live_reference_vc_advection_random(method):
+    rng = np.random.default_rng(sum(method))
+    mx, my = 23, 17
+    aux = np.stack([rng.uniform(-1, 1, (my + 4, mx + 4)), rng.uniform(-1, 1, (my + 4, mx + 4)),
+                    rng.uniform(0.5, 1.5, (my + 4, mx + 4))], -1)
+    p = O.Problem(1, 1, mx, my, O.SOLVER_VC_ADVECTION, O.solver_params(), maux=3, method=method,
+                  mthlim=[4], mthbc=(1, 1, 2, 2))
+    q = O.bc2(p, rng.standard_normal((my + 4, mx + 4, 1)))
+    a, ca = O.step2(p, q, 0.05, 0.07, 0.01, aux=aux)
+    b, cb = O.step2(p, q, 0.05, 0.07, 0.01, aux=aux, backend="ref")
+    np.testing.assert_array_equal(a[2:-2, 2:-2], b[2:-2, 2:-2])
+    assert ca == cb
