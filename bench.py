@@ -18,6 +18,7 @@ arithmetic) on a bounded sample.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -34,11 +35,12 @@ WORKLOAD = "euler2d_4096x4096_roe_efix_mc_unsplit_transverse_200steps"
 BYTES_PER_CELL_UPDATE = 64  # (2*meqn + maux) * 8 with meqn = 4, maux = 0
 
 
-def profile_traffic():
+def profile_traffic(name):
     """HBM bytes per step-kernel launch measured with rocprofv3 counters
-    (FETCH_SIZE + WRITE_SIZE, corrected by the copy-kernel calibration) as
-    recorded in profiles/; None until such a measurement exists."""
-    path = os.path.join(ROOT, "profiles", "traffic_euler4096.json")
+    (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a
+    plain copy kernel of known size) as recorded in
+    profiles/traffic_<name>.json; None until such a measurement exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(path):
         try:
             return json.load(open(path)).get("bytes_per_launch")
@@ -145,6 +147,114 @@ def bench3(args):
         "kernel_ms_avg": kavg_s * 1e3,
         "step_overhead_ms_avg": (total_ms - kern_ms) / ksteps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profile_traffic("acoustics3d512") if n == 512 else None},
+        "cpu_baseline": cpu,
+    }
+
+
+WORKLOAD_B = "batch2d_256problems_sides_loguniform_128_1024_euler_acoustics_mc_unsplit_transverse"
+
+
+def cpu_baseline_batch(specs, min_cells=16_000_000):
+    """Time the CPU oracle (single thread, -O2) on one claw2 step of each
+    batch problem in turn until min_cells cell updates are done; returns
+    (Gcell-updates/s, s, sample)."""
+    try:
+        from oracle import oracle as O
+        from classic_amd import problems as P
+        O.lib()
+    except Exception as e:  # pragma: no cover
+        return None, 0.0, f"oracle unavailable: {e}"
+    done, el, used = 0, 0.0, 0
+    for s in specs:
+        q, dx, dy = P.batch_problem_q(s)
+        if s["solver"] == "euler":
+            p = O.Problem(4, 3, s["mx"], s["my"], O.SOLVER_EULER, O.solver_params(gamma=1.4, efix=True),
+                          method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4, 4], mthbc=s["mthbc"])
+        else:
+            p = O.Problem(3, 2, s["mx"], s["my"], O.SOLVER_ACOUSTICS, O.solver_params(*s["params"]),
+                          method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4], mthbc=s["mthbc"])
+        t0 = time.perf_counter()
+        r = O.claw2(p, q, dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 1)
+        el += time.perf_counter() - t0
+        done += s["mx"] * s["my"] * int(r.nv[1])
+        used += 1
+        if done >= min_cells:
+            break
+    return done / el / 1e9, el, (f"first {used} problems of the batch, 1 claw2 step each "
+                                 f"({done} cell updates), 1 thread")
+
+
+def bench_batch(args):
+    """256 independent 2D problems (sides log-uniform in [128, 1024], Euler
+    four-quadrant or random-coefficient acoustics), each on its own claw2
+    loop with its own dt, all advanced by one persistent kernel per batched
+    iteration, replayed from a captured HIP graph; the schedule comes from
+    the autotuner.  value = accepted cell updates per second over --steps
+    batched iterations."""
+    from classic_amd import problems as P
+    from classic_amd.batch import from_specs
+
+    specs = P.batch_problem_specs(args.nprob, seed=args.seed)
+    arith = 1 if args.arith == "fast" else 0
+    b, qs = from_specs(specs, arith=arith)
+    del qs
+    method, mthlim = (1, 2, 2, 0, 0, 0, 0), (4, 4, 4)
+    b.start(0.0, 1e30, [1e-4, 1e99], [1.0, 0.9], 10 ** 9, method, mthlim)
+    cands = b.tune(reps=2)
+    sched = b.schedule()
+    if args.warmup > 0:
+        b.advance(args.warmup)
+    cells = np.array([s["mx"] * s["my"] for s in specs], dtype=np.int64)
+    meqn = np.array([4 if s["solver"] == "euler" else 3 for s in specs], dtype=np.int64)
+    nv0 = np.array([b.result(k)["nv"][1] for k in range(len(specs))], dtype=np.int64)
+    t0 = time.perf_counter()
+    left = b.advance(args.steps)
+    wall = time.perf_counter() - t0
+    if left:
+        raise SystemExit(f"{left} problems finished early")
+    nv1 = np.array([b.result(k)["nv"][1] for k in range(len(specs))], dtype=np.int64)
+    updates = int(np.sum((nv1 - nv0) * cells))
+    value = updates / wall / 1e9
+    ksteps = max(5, min(args.steps, 20))
+    total_ms, kern_ms = b.time_iterations(ksteps)
+    b.close()
+    kavg_s = kern_ms / ksteps / 1e3
+    bytes_iter = int(np.sum(2 * meqn * 8 * cells))
+    achieved = bytes_iter / kavg_s / 1e9
+    cpu = None
+    if not args.no_cpu:
+        v, el, sample = cpu_baseline_batch(specs)
+        if v is not None:
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+                   "sample": f"{sample} ({el:.1f} s), oracle C restatement compiled -O2"}
+    best = min(cands, key=lambda c: c[4])
+    worst = max(cands, key=lambda c: c[4])
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded batch: four-quadrant Euler and acoustics pulses)",
+        "config": {"workload": WORKLOAD_B, "nprob": len(specs), "seed": args.seed,
+                   "cells_total": int(cells.sum()),
+                   "euler_problems": int(sum(s["solver"] == "euler" for s in specs)),
+                   "method": list(method), "mthlim": list(mthlim), "cfl_desired": 0.9,
+                   "cfl_max": 1.0, "schedule": sched},
+        "arith": args.arith,
+        "accepted_updates": updates,
+        "kernel_ms_avg": kavg_s * 1e3,
+        "iteration_ms_avg_eager": total_ms / ksteps,
+        "autotuner": {"candidates": len(cands), "best_ms": best[4], "worst_ms": worst[4]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
     }
@@ -152,9 +262,11 @@ def bench3(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["euler2d", "acoustics3d"], default="euler2d",
+    ap.add_argument("--workload", choices=["euler2d", "acoustics3d", "batch2d"], default="euler2d",
                     help="euler2d: the headline 4096^2 Euler run; acoustics3d: the 512^3 "
-                         "variable-coefficient acoustics run")
+                         "variable-coefficient acoustics run; batch2d: 256 uneven 2D problems "
+                         "in one persistent kernel")
+    ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--n3", type=int, default=512)
     ap.add_argument("--cpu-n3", type=int, default=128)
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +294,9 @@ def main():
         args.steps = 100 if args.workload == "acoustics3d" else 200
     if args.workload == "acoustics3d":
         print(json.dumps(bench3(args)), flush=True)
+        return
+    if args.workload == "batch2d":
+        print(json.dumps(bench_batch(args)), flush=True)
         return
 
     from classic_amd import SOLVER_EULER, gpu
@@ -221,13 +336,15 @@ def main():
     g.set_arith(1 - arith)
     _, kern_other, _ = g.time_steps(dx, dy, dt_last, method, mthlim, mthbc, ksteps)
     g.close()
+    copy_cal = None
     if args.copy_calibration:
-        import torch
+        from classic_amd import _lib
 
-        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
-        dst = torch.empty_like(src)
-        torch.mul(src, 2.0, out=dst)  # one elementwise kernel: reads 1 GiB, writes 1 GiB
-        torch.cuda.synchronize()
+        ms = C.c_float(0.0)
+        ncal = 1 << 27  # 1 GiB per buffer
+        _lib.check(_lib.load().clawgpu_copy_calibration(ncal, 3, C.byref(ms)))
+        copy_cal = {"bytes_each_way": ncal * 8, "ms": ms.value,
+                    "GB_s": 2 * ncal * 8 / (ms.value * 1e-3) / 1e9}
 
     cpu = None
     if not args.no_cpu:
@@ -258,9 +375,13 @@ def main():
         "kernel_ms_avg_other_arith": kern_other / ksteps,
         "step_overhead_ms_avg": (total_ms - kern_ms) / ksteps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profile_traffic()},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (profile_traffic("euler4096")
+                                 if n == 4096 and args.arith == "fast" else None)},
         "cpu_baseline": cpu,
     }
+    if copy_cal:
+        rec["copy_calibration"] = copy_cal
     print(json.dumps(rec), flush=True)
 
 

@@ -19,26 +19,46 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CLAWGPU_ARCH", "gfx950")
 
 FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     "-I", os.path.join(ROOT, "include"),
 ]
+OBJ_DIR = os.path.join(HERE, "_build")
 
 
 def sources():
     return sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile each source to an object (in parallel) and link the library."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(OBJ_DIR, os.path.basename(s) + ".o") for s in srcs]
+    cmds = [[HIPCC, *FLAGS, "-c", "-o", o, s] for s, o in zip(srcs, objs)]
+    jobs = jobs or min(len(cmds), int(os.environ.get("MAX_JOBS", "8")), os.cpu_count() or 1)
+    running = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        running.append(subprocess.Popen(cmd))
+        if len(running) >= jobs:
+            _wait(running.pop(0), running)
+    while running:
+        _wait(running.pop(0), running)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *sources()]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], check=True)
     os.replace(tmp, LIB)
     return LIB
+
+
+def _wait(proc, others):
+    if proc.wait() != 0:
+        for p in others:
+            p.wait()
+        raise subprocess.CalledProcessError(proc.returncode, proc.args)
 
 
 if __name__ == "__main__":
