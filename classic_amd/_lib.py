@@ -59,6 +59,19 @@ _SIGS = {
     "clawgpu_learned_set_target": ([_V, _D], _i),
     "clawgpu_learned_grad": ([_V, _d, _d, _d, _I, _I, _i, C.POINTER(C.c_float), _D], _i),
     "clawgpu_time_steps": ([_V, _d, _d, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
+    "clawgpu_copy_calibration": ([C.c_long, _i, _F], _i),
+    "clawgpu_batch_create": ([C.POINTER(_V), _i, _i, _I, _I, _I, _D, _I, _D, _D], _i),
+    "clawgpu_batch_destroy": ([_V], _i),
+    "clawgpu_batch_set_q": ([_V, _i, _D], _i),
+    "clawgpu_batch_get_q": ([_V, _i, _D], _i),
+    "clawgpu_batch_set_arith": ([_V, _i], _i),
+    "clawgpu_batch_start": ([_V, _d, _D, _D, _D, _i, _I, _I, _i], _i),
+    "clawgpu_batch_advance": ([_V, _i, _i, _I], _i),
+    "clawgpu_batch_result": ([_V, _i, _D, _D, _D, _I, _I], _i),
+    "clawgpu_batch_set_schedule": ([_V, _I], _i),
+    "clawgpu_batch_get_schedule": ([_V, _I], _i),
+    "clawgpu_batch_tune": ([_V, _i, _I, _I, _F, _i], _i),
+    "clawgpu_batch_time": ([_V, _i, _F, _F], _i),
     "clawgpu1_create": ([C.POINTER(_V), _i, _i, _i, _i, _i, _i, _D, _i], _i),
     "clawgpu1_destroy": ([_V], _i),
     "clawgpu1_set_q": ([_V, _D], _i),

@@ -12,6 +12,8 @@
 #include <new>
 
 #include "common.hpp"
+#include "bc2d.hpp"
+#include "layout2d.hpp"
 #include "split2d_kernel.hpp"
 #include "../../include/clawgpu.h"
 
@@ -24,87 +26,12 @@ namespace clawgpu {
 // (left, right, bottom, top; corners from the bottom/top pass over the
 // already-filled ghost columns) because each side's rule is an index map
 // plus a sign.  Thread 0 also clears the Courant accumulator for the step.
-struct BcArgs {
-    double* q;
-    Layout2 L;
-    int meqn;
-    int mthbc[4];
-    unsigned long long* cfl_bits;
-};
-
-__device__ inline int bc_map(int k, int n, int mbc, int lo, int hi, bool& flip)
-{
-    (void)mbc;
-    flip = false;
-    if (k < 1) {
-        int g = 1 - k;  // ghost layer 1..mbc
-        if (lo == 1) return 1;
-        if (lo == 2) return n + 1 - g;
-        flip = true;
-        return g;
-    }
-    if (k > n) {
-        int g = k - n;
-        if (hi == 1) return n;
-        if (hi == 2) return g;
-        flip = true;
-        return n + 1 - g;
-    }
-    return k;
-}
-
 __global__ void bc2_kernel(BcArgs a)
 {
-    const Layout2 L = a.L;
-    const int nx = L.mx + 2 * L.mbc, ny = L.my + 2 * L.mbc;
-    const int ng = 2 * L.mbc * nx + 2 * L.mbc * L.my;  // ghost rows + ghost columns of interior rows
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0 && a.cfl_bits)
         for (int k = 0; k < 4; k++) a.cfl_bits[k] = 0ull;
-    if (t >= ng) return;
-    int i, j;
-    if (t < 2 * L.mbc * nx) {
-        int r = t / nx;  // 0..2mbc-1
-        i = 1 - L.mbc + t % nx;
-        j = r < L.mbc ? 1 - L.mbc + r : L.my + 1 + (r - L.mbc);
-    } else {
-        int u = t - 2 * L.mbc * nx;
-        int c = u / L.my;  // 0..2mbc-1
-        j = 1 + u % L.my;
-        i = c < L.mbc ? 1 - L.mbc + c : L.mx + 1 + (c - L.mbc);
-    }
-    (void)ny;
-    bool fy, fx;
-    int js = bc_map(j, L.my, L.mbc, a.mthbc[2], a.mthbc[3], fy);
-    int is = bc_map(i, L.mx, L.mbc, a.mthbc[0], a.mthbc[1], fx);
-    long src = L.off(is, js), dst = L.off(i, j);
-    for (int m = 0; m < a.meqn; m++) {
-        double v = a.q[m * L.fs + src];
-        if ((m == 1 && fx) || (m == 2 && fy)) v = -v;
-        a.q[m * L.fs + dst] = v;
-    }
-}
-
-// ------------------------------------------------- layout conversion kernels
-// Reference layout q(meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc) (m fastest) <-> planes.
-__global__ void aos_to_planes(const double* __restrict__ src, double* __restrict__ dst, Layout2 L,
-                              int meqn)
-{
-    const int nx = L.mx + 2 * L.mbc, ny = L.my + 2 * L.mbc;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nx * ny) return;
-    int ii = (int)(t % nx), jj = (int)(t / nx);
-    for (int m = 0; m < meqn; m++) dst[m * L.fs + (long)jj * L.pitch + ii] = src[t * meqn + m];
-}
-
-__global__ void planes_to_aos(const double* __restrict__ src, double* __restrict__ dst, Layout2 L,
-                              int meqn)
-{
-    const int nx = L.mx + 2 * L.mbc, ny = L.my + 2 * L.mbc;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nx * ny) return;
-    int ii = (int)(t % nx), jj = (int)(t / nx);
-    for (int m = 0; m < meqn; m++) dst[t * meqn + m] = src[m * L.fs + (long)jj * L.pitch + ii];
+    bc2_cell(a.q, a.L, a.meqn, a.mthbc, t);
 }
 
 }  // namespace clawgpu
@@ -127,6 +54,26 @@ bool learned_ready(clawgpu_ctx* c);
 extern "C" const char* clawgpu_last_error(void) { return g_errbuf; }
 
 extern "C" int clawgpu_abi_version(void) { return CLAWGPU_ABI_VERSION; }
+
+int solver_params(int solver_id, const double* params, int nparams, SolverParams* P)
+{
+    *P = SolverParams{};
+    if (solver_id == CLAWGPU_SOLVER_ACOUSTICS) {
+        if (nparams < 2 || !params) return set_err(CLAWGPU_ERR_ARG, "acoustics needs (rho, bulk)");
+        P->rho = params[0];
+        P->bulk = params[1];
+        if (!(P->rho > 0.0) || !(P->bulk > 0.0)) return set_err(CLAWGPU_ERR_ARG, "rho and bulk must be positive");
+        P->cc = sqrt(P->bulk / P->rho);
+        P->zz = P->rho * P->cc;
+    } else if (solver_id == CLAWGPU_SOLVER_EULER) {
+        if (nparams < 2 || !params) return set_err(CLAWGPU_ERR_ARG, "euler needs (gamma, efix)");
+        P->gamma = params[0];
+        P->gamma1 = params[0] - 1.0;
+        P->efix = params[1] != 0.0;
+        if (!(P->gamma > 1.0)) return set_err(CLAWGPU_ERR_ARG, "gamma must exceed 1");
+    }
+    return CLAWGPU_OK;
+}
 
 static int solver_shape(int solver, int* meqn, int* mwaves)
 {
@@ -164,20 +111,8 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     if (mbc < 2) return set_err(CLAWGPU_ERR_ARG, "mbc must be >= 2 (claw2 info=1)");
     if (mx < 1 || my < 1) return set_err(CLAWGPU_ERR_ARG, "grid must have at least one cell");
     SolverParams P{};
-    if (solver_id == CLAWGPU_SOLVER_ACOUSTICS) {
-        if (nparams < 2 || !params) return set_err(CLAWGPU_ERR_ARG, "acoustics needs (rho, bulk)");
-        P.rho = params[0];
-        P.bulk = params[1];
-        if (!(P.rho > 0.0) || !(P.bulk > 0.0)) return set_err(CLAWGPU_ERR_ARG, "rho and bulk must be positive");
-        P.cc = sqrt(P.bulk / P.rho);
-        P.zz = P.rho * P.cc;
-    } else if (solver_id == CLAWGPU_SOLVER_EULER) {
-        if (nparams < 2 || !params) return set_err(CLAWGPU_ERR_ARG, "euler needs (gamma, efix)");
-        P.gamma = params[0];
-        P.gamma1 = params[0] - 1.0;
-        P.efix = params[1] != 0.0;
-        if (!(P.gamma > 1.0)) return set_err(CLAWGPU_ERR_ARG, "gamma must exceed 1");
-    }
+    const int prc = solver_params(solver_id, params, nparams, &P);
+    if (prc) return prc;
     clawgpu_ctx* c = new (std::nothrow) clawgpu_ctx();
     if (!c) return set_err(CLAWGPU_ERR_ARG, "out of host memory");
     c->meqn = meqn;
@@ -627,6 +562,54 @@ done:
     dtv[3] = dtmax;
     dtv[4] = dt;
     *info_out = info;
+    return CLAWGPU_OK;
+}
+
+namespace {
+__global__ void calib_fill(double* p, long n)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+        p[e] = (double)(e & 1023);
+}
+// a plain stream copy, 16 bytes per lane per access
+__global__ void calib_copy(const double2* __restrict__ s, double2* __restrict__ d, long n2)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (long)gridDim.x * blockDim.x)
+        d[e] = s[e];
+}
+}  // namespace
+
+extern "C" int clawgpu_copy_calibration(long ndoubles, int reps, float* ms_avg)
+{
+    // Benchmark helper (no reference counterpart): reps copies of ndoubles
+    // doubles between two fresh device buffers, timed with HIP events; the
+    // known byte count calibrates the FETCH_SIZE / WRITE_SIZE counters.
+    if (ndoubles < 2 || reps < 1 || !ms_avg) return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    ndoubles &= ~1L;
+    double *a = nullptr, *b = nullptr;
+    hipEvent_t e0, e1;
+    clawgpu_ctx* c = nullptr;
+    HIPCHK(c, hipMalloc((void**)&a, ndoubles * sizeof(double)));
+    if (hipMalloc((void**)&b, ndoubles * sizeof(double)) != hipSuccess) {
+        (void)hipFree(a);
+        return set_err(CLAWGPU_ERR_HIP, "allocation failed");
+    }
+    hipLaunchKernelGGL(calib_fill, dim3(4096), dim3(256), 0, 0, a, ndoubles);
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(calib_copy, dim3(8192), dim3(256), 0, 0, (const double2*)a, (double2*)b,
+                           ndoubles / 2);
+    HIPCHK(c, hipEventRecord(e1, 0));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    *ms_avg = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return CLAWGPU_OK;
 }
 

@@ -40,6 +40,10 @@ inline int set_err(int code, const char* msg)
     return code;
 }
 
+// SolverParams of a 2D solver from the (rho, bulk) / (gamma, efix) argument
+// pair of clawgpu_create; CLAWGPU_OK or an error code with the message set.
+int solver_params(int solver_id, const double* params, int nparams, SolverParams* P);
+
 #define HIPCHK(ctx, call)                                                              \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \

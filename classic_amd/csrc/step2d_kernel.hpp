@@ -50,18 +50,16 @@ struct CfgFix {
     static constexpr int order = ORDER, trans = TRANS, lim = LIM;
 };
 
+// The work of one wavefront: strip `strip` (64 columns) and row segment `seg`
+// of the grid described by a.
 template <class S, class Cfg>
-__global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(StepArgs a)
+__device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, const int seg,
+                                           const int lane)
 {
     using Mth = typename S::Math;
     using Cell = typename S::Cell;
     using Cache = typename S::Cache;
     constexpr int M = S::M, MW = S::MW;
-    const int lane = threadIdx.x & 63;
-    const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (wid >= a.nstrips * a.nsegs) return;  // uniform per wave
-    const int strip = wid % a.nstrips;
-    const int seg = wid / a.nstrips;
     const Layout2 L = a.L;
     const int mx = L.mx, my = L.my, mbc = L.mbc;
     const int i = kStripOut * strip + lane - 1;
@@ -333,6 +331,15 @@ __global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(S
         row(ysB, ysA, J + 1);
     }
     publish_cfl(cfl, a.cfl_bits);
+}
+
+template <class S, class Cfg>
+__global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(StepArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (wid >= a.nstrips * a.nsegs) return;  // uniform per wave
+    step2_wave<S, Cfg>(a, wid % a.nstrips, wid / a.nstrips, lane);
 }
 
 // Solver-level evaluation for tests: interface k between states ql[k]

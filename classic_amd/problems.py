@@ -152,3 +152,35 @@ def annulus_hump(mx=40, my=120, mbc=2, xlower=0.2, xupper=1.0, ylower=0.0,
             for a, b, c in zip(A, beta, centre))
     q[mbc:-mbc, mbc:-mbc, 0] = v[mbc:-mbc, mbc:-mbc]
     return q
+
+
+def batch_problem_specs(nprob=256, seed=0, side_lo=128, side_hi=1024, p_euler=0.5):
+    """Specs of a batch of independent 2D problems: each grid side drawn
+    log-uniformly in [side_lo, side_hi] (mx and my independently), each
+    problem Euler (four random quadrant states, extrapolation boundaries)
+    with probability p_euler, else acoustics with rho ~ U[0.5, 4] and
+    bulk ~ U[1, 8] (the radial pressure pulse of ``acoustics_pulse``, walls).
+    Returns a list of dicts with solver ("euler"/"acoustics"), mx, my,
+    params, mthbc and a per-problem seed."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    lo, hi = np.log(side_lo), np.log(side_hi)
+    for k in range(nprob):
+        mx = int(round(np.exp(rng.uniform(lo, hi))))
+        my = int(round(np.exp(rng.uniform(lo, hi))))
+        if rng.uniform() < p_euler:
+            specs.append(dict(solver="euler", mx=mx, my=my, params=(1.4, 1.0), mthbc=(1, 1, 1, 1),
+                              seed=int(rng.integers(0, 2**31))))
+        else:
+            rho = rng.uniform(0.5, 4.0)
+            bulk = rng.uniform(1.0, 8.0)
+            specs.append(dict(solver="acoustics", mx=mx, my=my, params=(rho, bulk),
+                              mthbc=(3, 3, 3, 3), seed=int(rng.integers(0, 2**31))))
+    return specs
+
+
+def batch_problem_q(spec, mbc=2):
+    """Initial data (q, dx, dy) of one batch problem spec."""
+    if spec["solver"] == "euler":
+        return euler_quadrants(spec["mx"], spec["my"], seed=spec["seed"], mbc=mbc)
+    return acoustics_pulse(spec["mx"], spec["my"], mbc=mbc)

@@ -168,6 +168,63 @@ int clawgpu_learned_grad(clawgpu_ctx* ctx, double dx, double dy, double dt, cons
                          const int* mthbc, int mode, float* grad, double* loss);
 
 
+/* Benchmark helper (no reference counterpart): reps plain device copies of
+ * ndoubles doubles; *ms_avg = average time per copy.  A kernel of known
+ * traffic for calibrating the FETCH_SIZE / WRITE_SIZE counters. */
+int clawgpu_copy_calibration(long ndoubles, int reps, float* ms_avg);
+
+/* --------------------------------------------------------- batched 2D */
+
+/* Many independent 2D grids advanced together (no reference counterpart as
+ * one call: each problem follows its own claw2 loop, src/2d/claw2.f:494-784,
+ * exactly as clawgpu_claw2 would run it, and results equal that call's bit
+ * for bit in EXACT arithmetic).  One batched iteration performs, for every
+ * unfinished problem, one step attempt: dt clamp to tend, bc2, the unsplit
+ * step of clawgpu_step2, the Courant check with retake and the next dt, all
+ * decided on the device; iterations are replayed from a captured HIP graph.
+ *
+ * clawgpu_batch_create: nprob problems; per problem k: solver[k] (ACOUSTICS
+ *   or EULER), mx[k], my[k], params[2k..2k+1] (as clawgpu_create),
+ *   mthbc[4k..4k+3], dx[k], dy[k]; mbc is shared.
+ * clawgpu_batch_set_q / get_q: q of problem k in the reference layout.
+ * clawgpu_batch_start: begin claw2 runs of every problem from tstart to
+ *   tend[k] with the shared dtv(1:2), cflv(1:2), nv(1) = maxn, method(1:7)
+ *   (method(3) in {0,1,2}, method(6) = 0), mthlim(1:3) (waves 1..mwaves).
+ * clawgpu_batch_advance: at most max_iters batched iterations, stopping
+ *   early when every problem has finished; *nleft = unfinished problems.
+ * clawgpu_batch_result: claw2's outputs of problem k so far: tend (*t),
+ *   dtv(3:5), cflv(3:4), nv(1:2), info (0, 2, 11, 12, or -5 for a NaN
+ *   wave speed).
+ * clawgpu_batch_set_schedule / get_schedule: sched(1:4) = rows per tile,
+ *   blocks of each persistent step kernel (0: two per CU), tile order (1:
+ *   problems by decreasing size) and the acoustics kernel's register budget
+ *   (1: three waves per SIMD); clawgpu_batch_tune measures every candidate
+ *   schedule on state-preserving iterations and keeps the fastest
+ *   (cand_out: the four values per candidate; ms_out: ms per iteration). */
+typedef struct clawgpu_batch clawgpu_batch;
+
+int clawgpu_batch_create(clawgpu_batch** batch, int nprob, int mbc, const int* solver, const int* mx,
+                         const int* my, const double* params, const int* mthbc, const double* dx,
+                         const double* dy);
+int clawgpu_batch_destroy(clawgpu_batch* batch);
+int clawgpu_batch_set_q(clawgpu_batch* batch, int k, const double* q);
+int clawgpu_batch_get_q(clawgpu_batch* batch, int k, double* q);
+int clawgpu_batch_set_arith(clawgpu_batch* batch, int mode);
+int clawgpu_batch_start(clawgpu_batch* batch, double tstart, const double* tend, const double* dtv,
+                        const double* cflv, int maxn, const int* method, const int* mthlim,
+                        int use_fwave);
+int clawgpu_batch_advance(clawgpu_batch* batch, int max_iters, int use_graph, int* nleft);
+int clawgpu_batch_result(clawgpu_batch* batch, int k, double* t, double* dtv, double* cflv, int* nv,
+                         int* info);
+int clawgpu_batch_set_schedule(clawgpu_batch* batch, const int* sched);
+int clawgpu_batch_get_schedule(clawgpu_batch* batch, int* sched);
+int clawgpu_batch_tune(clawgpu_batch* batch, int reps, int* ncand, int* cand_out, float* ms_out,
+                       int max_cand);
+/* Measurement helper: iters batched iterations launched without the graph,
+ * HIP events around each step-kernel launch; *total_ms spans every launch,
+ * *step_kernel_ms sums the step kernels. */
+int clawgpu_batch_time(clawgpu_batch* batch, int iters, float* total_ms, float* step_kernel_ms);
+
 /* ------------------------------------------------------------------ 3D */
 
 typedef struct clawgpu3_ctx clawgpu3_ctx;
