@@ -265,3 +265,80 @@ def annulus_aux(mx, my, mbc=2, xlower=0.2, xupper=1.0, ylower=0.0, yupper=2.0 * 
     lib().orc_annulus_aux(mbc, mx, my, C.c_double(xlower), C.c_double(ylower), C.c_double(dxc),
                           C.c_double(dyc), _dp(aux))
     return aux, dxc, dyc
+
+
+# ------------------------------------------------------------ driver checks
+
+SOLVER_BY_NAME = {"acoustics": SOLVER_ACOUSTICS, "euler": SOLVER_EULER}
+SHAPE_BY_ID = {SOLVER_ACOUSTICS: (3, 2), SOLVER_EULER: (4, 3)}
+
+
+class OracleGrid2D:
+    """The stepping interface of the device grid (set_q / get_q / set_aux /
+    claw2) on the C restatement, for checking driver logic on the CPU."""
+
+    def __init__(self, solver_id, mx, my, params, mbc=2, maux=0):
+        self.solver_id, self.mx, self.my, self.mbc, self.maux = solver_id, mx, my, mbc, maux
+        self.meqn, self.mwaves = SHAPE_BY_ID[solver_id]
+        self.params = params
+        self.q = None
+        self.aux = None
+
+    def set_q(self, q):
+        self.q = np.ascontiguousarray(q, dtype=np.float64).copy()
+
+    def get_q(self):
+        return self.q.copy()
+
+    def set_aux(self, aux):
+        self.aux = np.ascontiguousarray(aux, dtype=np.float64).copy()
+
+    def claw2(self, dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc,
+              use_fwave=False):
+        p = Problem(self.meqn, self.mwaves, self.mx, self.my, self.solver_id, self.params,
+                    maux=self.maux, mbc=self.mbc, method=method, mthlim=mthlim, mthbc=mthbc,
+                    use_fwave=use_fwave)
+        r = claw2(p, self.q, dx, dy, tstart, tend, dtv, cflv, nmax, aux=self.aux)
+        self.q = r.q
+        return dict(info=r.info, tend=r.tend, dtv=r.dtv, cflv=r.cflv, nv=r.nv)
+
+
+def ref_claw2ez(rundir, solver_id, params, q0, aux0=None):
+    """Run the reference driver claw2ez (src/2d/claw2ez.f) in rundir with
+    qinit/setaux returning q0/aux0 and the given solver; it reads
+    rundir/claw.data and writes fort.* there.  Needs oracle/_ref."""
+    if not os.path.exists(os.path.join(rundir, "claw.data")):
+        raise FileNotFoundError("claw.data missing (the reference driver would stop the process)")
+    R = ref()
+    R.ref_set_solver(solver_id, C.byref(params))
+    q0 = np.ascontiguousarray(q0, dtype=np.float64)
+    a0 = None if aux0 is None else np.ascontiguousarray(aux0, dtype=np.float64)
+    R.ref_ez_set_init(_dp(q0), C.c_long(q0.size), _aux_ptr(a0), C.c_long(0 if a0 is None else a0.size))
+    cwd = os.getcwd()
+    try:
+        os.chdir(rundir)
+        R.ref_claw2ez()
+    finally:
+        os.chdir(cwd)
+        R.ref_ez_set_init(None, C.c_long(0), None, C.c_long(0))
+
+
+def ref_out2(outdir, q, t, frame, mbc, lower, delta, aux=None, outaux=False):
+    """Write fort.q/t(/a)NNNN with the reference's out2 (src/2d/out2.f);
+    note that it flushes |q| < 1e-99 to zero in q, as the reference does."""
+    R = ref()
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    my2, mx2, meqn = q.shape
+    maux = 0 if aux is None else aux.shape[-1]
+    a = aux if aux is not None else np.zeros(1)
+    i = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+    d = lambda v: C.byref(C.c_double(float(v)))  # noqa: E731
+    cwd = os.getcwd()
+    try:
+        os.chdir(outdir)
+        R.out2_(i(meqn), i(mbc), i(mx2 - 2 * mbc), i(my2 - 2 * mbc), d(lower[0]), d(lower[1]),
+                d(delta[0]), d(delta[1]), _dp(q), d(t), i(frame), _dp(a), i(maux),
+                i(1 if outaux else 0))
+    finally:
+        os.chdir(cwd)
+    return q

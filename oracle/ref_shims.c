@@ -59,6 +59,23 @@ static void f_rpt2(int* ixy, int* imp, int* maxm, int* meqn, int* mwaves, int* m
                            aux2, aux3, asdq, bmasdq, bpasdq);
 }
 
+/* the same wrappers under the external names the reference drivers pass
+ * to claw2 (claw2ez.f:15, :270-273) */
+void rpn2_(int* ixy, int* maxm, int* meqn, int* mwaves, int* maux, int* mbc, int* mx, double* ql,
+           double* qr, double* auxl, double* auxr, double* wave, double* s, double* amdq,
+           double* apdq)
+{
+    f_rpn2(ixy, maxm, meqn, mwaves, maux, mbc, mx, ql, qr, auxl, auxr, wave, s, amdq, apdq);
+}
+
+void rpt2_(int* ixy, int* imp, int* maxm, int* meqn, int* mwaves, int* maux, int* mbc, int* mx,
+           double* ql, double* qr, double* aux1, double* aux2, double* aux3, double* asdq,
+           double* bmasdq, double* bpasdq)
+{
+    f_rpt2(ixy, imp, maxm, meqn, mwaves, maux, mbc, mx, ql, qr, aux1, aux2, aux3, asdq, bmasdq,
+           bpasdq);
+}
+
 void ref_set_solver(int solver_id, const orc_solver_params* p)
 {
     g_solver = solver_id;
