@@ -119,6 +119,8 @@ def run(rundir=".", outdir=None, qinit=None, setaux=None, setprob=None, solver="
     already constructed stepping object with the Grid*D interface."""
     c = read_claw_data(os.path.join(rundir, "claw.data"))
     nd = c.ndim
+    if nd == 1:
+        c["restart"] = False  # claw1ez reads no restart record
     outdir = outdir or rundir
     os.makedirs(outdir, exist_ok=True)
     _check_periodic(c.mthbc)

@@ -197,3 +197,53 @@ def rp1(solver_id, params, q_cells, aux_cells=None, mwaves=2, mbc=2):
     fn(mx, meqn, mwaves, maux, mbc, mx, _dp(q), _dp(q), _dp(a), _dp(a), _dp(wave), _dp(s),
        _dp(amdq), _dp(apdq))
     return wave, s, amdq, apdq
+
+
+# ------------------------------------------------------------ driver checks
+
+class OracleGrid1D:
+    """The stepping interface of the device 1D grid (set_q / get_q / set_aux
+    / claw1) on the C restatement, for checking driver logic on the CPU."""
+
+    def __init__(self, solver_id, mx, params, mbc=2, maux=0, meqn=2, mwaves=2):
+        self.solver_id, self.mx, self.mbc, self.maux = solver_id, mx, mbc, maux
+        self.meqn, self.mwaves, self.params = meqn, mwaves, params
+        self.q = self.aux = None
+
+    def set_q(self, q):
+        self.q = np.ascontiguousarray(q, dtype=np.float64).copy()
+
+    def get_q(self):
+        return self.q.copy()
+
+    def set_aux(self, aux):
+        self.aux = np.ascontiguousarray(aux, dtype=np.float64).copy()
+
+    def claw1(self, dx, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc, use_fwave=False):
+        p = Problem1(self.mx, self.solver_id, self.params, meqn=self.meqn, mwaves=self.mwaves,
+                     maux=self.maux, mbc=self.mbc, method=method, mthlim=mthlim, mthbc=mthbc,
+                     use_fwave=use_fwave)
+        r = claw1(p, self.q, dx, tstart, tend, dtv, cflv, nmax, aux=self.aux)
+        self.q = r.q
+        return dict(info=r.info, tend=r.tend, dtv=r.dtv, cflv=r.cflv, nv=r.nv)
+
+
+def ref_claw1ez(rundir, solver_id, params, q0, aux0=None):
+    """Run the reference driver claw1ez (src/1d/claw1ez.f) in rundir with
+    qinit/setaux returning q0/aux0; it reads rundir/claw.data and writes
+    fort.* there.  Needs oracle/_ref."""
+    if not os.path.exists(os.path.join(rundir, "claw.data")):
+        raise FileNotFoundError("claw.data missing (the reference driver would stop the process)")
+    R = ref()
+    R.ref1_set_solver(solver_id, C.byref(params))
+    q0 = np.ascontiguousarray(q0, dtype=np.float64)
+    a0 = None if aux0 is None else np.ascontiguousarray(aux0, dtype=np.float64)
+    R.ref1_ez_set_init(_dp(q0), C.c_long(q0.size), None if a0 is None else _dp(a0),
+                       C.c_long(0 if a0 is None else a0.size))
+    cwd = os.getcwd()
+    try:
+        os.chdir(rundir)
+        R.ref_claw1ez()
+    finally:
+        os.chdir(cwd)
+        R.ref1_ez_set_init(None, C.c_long(0), None, C.c_long(0))

@@ -250,3 +250,53 @@ def rpt3(ixyz, icoor, imp, aux_slices, asdq, mwaves=2, mbc=2, impt=None):
                                      None, None, _dp(a1), _dp(a2), _dp(a3), _dp(a), _dp(bm),
                                      _dp(bp))
     return bm, bp
+
+
+# ------------------------------------------------------------ driver checks
+
+class OracleGrid3D:
+    """The stepping interface of the device 3D grid (set_q / get_q / set_aux
+    / claw3) on the C restatement, for checking driver logic on the CPU."""
+
+    def __init__(self, mx, my, mz, maux=2, mbc=2, lower=(-1.0, -1.0, -1.0),
+                 upper=(1.0, 1.0, 1.0)):
+        self.mx, self.my, self.mz, self.maux, self.mbc = mx, my, mz, maux, mbc
+        self.lower, self.upper = lower, upper
+        self.q = self.aux = None
+
+    def set_q(self, q):
+        self.q = np.ascontiguousarray(q, dtype=np.float64).copy()
+
+    def get_q(self):
+        return self.q.copy()
+
+    def set_aux(self, aux):
+        self.aux = np.ascontiguousarray(aux, dtype=np.float64).copy()
+
+    def claw3(self, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc, use_fwave=False):
+        p = Problem3(self.mx, self.my, self.mz, maux=self.maux, mbc=self.mbc, method=method,
+                     mthlim=mthlim, mthbc=mthbc, lower=self.lower, upper=self.upper,
+                     use_fwave=use_fwave)
+        r = claw3(p, self.q, self.aux, tstart, tend, dtv, cflv, nmax)
+        self.q = r.q
+        return r
+
+
+def ref_claw3ez(rundir, q0, aux0):
+    """Run the reference driver claw3ez (src/3d/claw3ez.f) in rundir with
+    qinit/setaux returning q0/aux0 and the variable-coefficient acoustics
+    solvers; it reads rundir/claw.data and writes fort.* there.  Needs
+    oracle/_ref."""
+    if not os.path.exists(os.path.join(rundir, "claw.data")):
+        raise FileNotFoundError("claw.data missing (the reference driver would stop the process)")
+    R = ref()
+    q0 = np.ascontiguousarray(q0, dtype=np.float64)
+    a0 = np.ascontiguousarray(aux0, dtype=np.float64)
+    R.ref3_ez_set_init(_dp(q0), C.c_long(q0.size), _dp(a0), C.c_long(a0.size))
+    cwd = os.getcwd()
+    try:
+        os.chdir(rundir)
+        R.ref_claw3ez()
+    finally:
+        os.chdir(cwd)
+        R.ref3_ez_set_init(None, C.c_long(0), None, C.c_long(0))

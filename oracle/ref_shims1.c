@@ -35,6 +35,13 @@ static void f_rp1(int* maxm, int* meqn, int* mwaves, int* maux, int* mbc, int* m
                             s, amdq, apdq);
 }
 
+/* the same wrapper under the external name claw1ez passes to claw1 */
+void rp1_(int* maxm, int* meqn, int* mwaves, int* maux, int* mbc, int* mx, double* ql, double* qr,
+          double* auxl, double* auxr, double* wave, double* s, double* amdq, double* apdq)
+{
+    f_rp1(maxm, meqn, mwaves, maux, mbc, mx, ql, qr, auxl, auxr, wave, s, amdq, apdq);
+}
+
 void ref1_set_solver(int solver_id, const orc1_params* p)
 {
     g_solver1 = solver_id;

@@ -70,6 +70,30 @@ static void f_rptt3(int* ixyz, int* icoor, int* imp, int* impt, int* maxm, int* 
                            ql, qr, aux1, aux2, aux3, bsasdq, cmbsasdq, cpbsasdq);
 }
 
+/* the same wrappers under the external names claw3ez passes to claw3 */
+void rpn3_(int* ixyz, int* maxm, int* meqn, int* mwaves, int* maux, int* mbc, int* mx, double* ql,
+           double* qr, double* auxl, double* auxr, double* wave, double* s, double* amdq,
+           double* apdq)
+{
+    f_rpn3(ixyz, maxm, meqn, mwaves, maux, mbc, mx, ql, qr, auxl, auxr, wave, s, amdq, apdq);
+}
+
+void rpt3_(int* ixyz, int* icoor, int* imp, int* maxm, int* meqn, int* mwaves, int* maux,
+           int* mbc, int* mx, double* ql, double* qr, double* aux1, double* aux2, double* aux3,
+           double* asdq, double* bmasdq, double* bpasdq)
+{
+    f_rpt3(ixyz, icoor, imp, maxm, meqn, mwaves, maux, mbc, mx, ql, qr, aux1, aux2, aux3, asdq,
+           bmasdq, bpasdq);
+}
+
+void rptt3_(int* ixyz, int* icoor, int* imp, int* impt, int* maxm, int* meqn, int* mwaves,
+            int* maux, int* mbc, int* mx, double* ql, double* qr, double* aux1, double* aux2,
+            double* aux3, double* bsasdq, double* cmbsasdq, double* cpbsasdq)
+{
+    f_rptt3(ixyz, icoor, imp, impt, maxm, meqn, mwaves, maux, mbc, mx, ql, qr, aux1, aux2, aux3,
+            bsasdq, cmbsasdq, cpbsasdq);
+}
+
 static int maxm3(int mx, int my, int mz)
 {
     int m = mx > my ? mx : my;

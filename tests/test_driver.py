@@ -211,3 +211,68 @@ def test_driver_matches_committed_reference_outputs(tmp_path, name):
     assert sorted(fg) == sorted(fm)
     for f in fg:
         assert fg[f] == fm[f], f
+
+
+# --------------------------------------------------------------- 1D and 3D
+
+from oracle import oracle1 as O1  # noqa: E402
+from oracle import oracle3 as O3  # noqa: E402
+
+needs_ref1 = pytest.mark.skipif(not O1.ref_available(), reason="reference build (oracle/_ref) absent")
+needs_ref3 = pytest.mark.skipif(not O3.ref_available(), reason="reference build (oracle/_ref) absent")
+
+
+def _run_driver1(tmp, name):
+    rd, q0, solver, params = E.CASES1[name]()
+    rd.write(tmp)
+    grid = O1.OracleGrid1D(O1.SOLVER_ACOUSTICS, rd.clawdata.num_cells[0], O1.params1(*params))
+    return driver.claw1ez(tmp, qinit=E.qinit_from(q0), solver=solver, grid=grid)
+
+
+def _run_driver3(tmp, name):
+    rd, (q0, aux0), solver, _ = E.CASES3[name]()
+    rd.write(tmp)
+    n = rd.clawdata.num_cells
+    grid = O3.OracleGrid3D(n[0], n[1], n[2], lower=rd.clawdata.lower, upper=rd.clawdata.upper)
+    return driver.claw3ez(tmp, qinit=E.qinit_from(q0), setaux=E.setaux_from(aux0), grid=grid)
+
+
+def _same_dirs(a, b):
+    fa, fb = _files(a), _files(b)
+    assert sorted(fa) == sorted(fb)
+    for f in fa:
+        assert fa[f] == fb[f], f
+
+
+@needs_ref1
+@pytest.mark.parametrize("name", list(E.CASES1))
+def test_driver1_matches_reference_claw1ez(tmp_path, name):
+    rd, q0, solver, params = E.CASES1[name]()
+    rd.write(str(tmp_path / "ref"))
+    O1.ref_claw1ez(str(tmp_path / "ref"), O1.SOLVER_ACOUSTICS, O1.params1(*params), q0)
+    res = _run_driver1(str(tmp_path / "mine"), name)
+    _same_dirs(str(tmp_path / "ref"), str(tmp_path / "mine"))
+    assert res.info == 0
+
+
+@needs_ref3
+@pytest.mark.parametrize("name", list(E.CASES3))
+def test_driver3_matches_reference_claw3ez(tmp_path, name):
+    rd, (q0, aux0), _, _ = E.CASES3[name]()
+    rd.write(str(tmp_path / "ref"))
+    O3.ref_claw3ez(str(tmp_path / "ref"), q0, aux0)
+    res = _run_driver3(str(tmp_path / "mine"), name)
+    _same_dirs(str(tmp_path / "ref"), str(tmp_path / "mine"))
+    assert res.info == 0 and os.path.exists(tmp_path / "mine" / "fort.a0000")
+
+
+@pytest.mark.parametrize("name", E.GOLDEN1 + E.GOLDEN3)
+def test_driver13_matches_committed_reference_outputs(tmp_path, name):
+    if name in E.CASES1:
+        _run_driver1(str(tmp_path), name)
+    else:
+        _run_driver3(str(tmp_path), name)
+    fg, fm = _golden_files(name), _files(str(tmp_path))
+    assert sorted(fg) == sorted(fm)
+    for f in fg:
+        assert fg[f] == fm[f], f
