@@ -5,7 +5,10 @@
 
 #include <cmath>
 #include <cstring>
+#include <cstdint>
 #include <new>
+#include <utility>
+#include <vector>
 
 #include "ctx.hpp"
 #include "step3d.hpp"
@@ -26,6 +29,10 @@ struct clawgpu3_ctx {
     int seg;          // cells per segment of the y/z split sweeps
     int tile;         // tile shape of the unsplit kernel (launch_step3_tile)
     int xcd;          // XCD-contiguous tile order
+    unsigned char* dmat = nullptr;  // material index per cell (nmat > 0)
+    double* dlut = nullptr;         // (Z, c) of each material
+    int nmat = 0;                   // distinct (Z, c) pairs found by set_aux, 0 if too many
+    int use_table = 1;              // read the material table instead of aux when nmat > 0
     int last_hip_error;
 };
 
@@ -219,6 +226,9 @@ Args3 step_args(clawgpu3_ctx* c, const double* q, double* qn, double dx, double 
     a.q = q;
     a.qn = qn;
     a.aux = c->daux;
+    a.mat = c->dmat;
+    a.lut = c->dlut;
+    a.nmat = c->use_table ? c->nmat : 0;
     a.L = c->L;
     a.afs = c->L.fs;
     a.dtdx = dt / dx;
@@ -356,6 +366,8 @@ extern "C" int clawgpu3_destroy(clawgpu3_ctx* c)
     (void)hipFree(c->daux);
     (void)hipFree(c->staging);
     (void)hipFree(c->exy);
+    if (c->dmat) (void)hipFree(c->dmat);
+    if (c->dlut) (void)hipFree(c->dlut);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return CLAWGPU_OK;
@@ -387,6 +399,51 @@ extern "C" int clawgpu3_get_q(clawgpu3_ctx* c, double* q)
     return CLAWGPU_OK;
 }
 
+// The distinct (Z, c) pairs of aux (compared as bit patterns, ghost cells
+// included) and, when there are at most kMaxMat of them, a byte index per
+// cell in the device's cell order (the host order: no padding in 3D).
+static int build_material_table(clawgpu3_ctx* c, const double* aux, long n)
+{
+    c->nmat = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> mats;
+    std::vector<unsigned char> idx((size_t)n);
+    std::pair<uint64_t, uint64_t> last{0, 0};
+    int last_i = -1;
+    for (long e = 0; e < n; e++) {
+        std::pair<uint64_t, uint64_t> k;
+        std::memcpy(&k.first, &aux[e * c->maux], 8);
+        std::memcpy(&k.second, &aux[e * c->maux + 1], 8);
+        if (last_i < 0 || k != last) {
+            last_i = -1;
+            for (size_t s = 0; s < mats.size(); s++)
+                if (mats[s] == k) {
+                    last_i = (int)s;
+                    break;
+                }
+            if (last_i < 0) {
+                if ((int)mats.size() == kMaxMat) return CLAWGPU_OK;  // too many: read aux
+                mats.push_back(k);
+                last_i = (int)mats.size() - 1;
+            }
+            last = k;
+        }
+        idx[(size_t)e] = (unsigned char)last_i;
+    }
+    if (!c->dmat) HIPCHK(c, hipMalloc((void**)&c->dmat, (size_t)n));
+    if (!c->dlut) HIPCHK(c, hipMalloc((void**)&c->dlut, 2 * kMaxMat * sizeof(double)));
+    std::vector<double> lut(2 * mats.size());
+    for (size_t s = 0; s < mats.size(); s++) {
+        std::memcpy(&lut[2 * s], &mats[s].first, 8);
+        std::memcpy(&lut[2 * s + 1], &mats[s].second, 8);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->dmat, idx.data(), (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dlut, lut.data(), lut.size() * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->nmat = (int)mats.size();
+    return CLAWGPU_OK;
+}
+
 extern "C" int clawgpu3_set_aux(clawgpu3_ctx* c, const double* aux)
 {
     if (!c || !aux) return set_err(CLAWGPU_ERR_ARG, "null argument");
@@ -406,6 +463,14 @@ extern "C" int clawgpu3_set_aux(clawgpu3_ctx* c, const double* aux)
     double m;
     std::memcpy(&m, &h[0], sizeof m);
     c->cmax = h[1] ? NAN : m;
+    return build_material_table(c, aux, n);
+}
+
+extern "C" int clawgpu3_set_aux_table(clawgpu3_ctx* c, int enable, int* nmat)
+{
+    if (!c) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    c->use_table = enable ? 1 : 0;
+    if (nmat) *nmat = c->nmat;
     return CLAWGPU_OK;
 }
 

@@ -13,7 +13,10 @@ static int launch_cfg(Args3 a, hipStream_t s)
     long nblocks = (long)a.ntx * a.nty * a.ntz;
     if (nblocks <= 0) return 0;
     if (a.xcd) nblocks = 8 * ((nblocks + 7) / 8);
-    hipLaunchKernelGGL((step3_tile_kernel<TY, TZ, Cfg>), dim3((unsigned)nblocks), dim3(64), 0, s, a);
+    if (a.nmat > 0 && a.nmat <= kMaxMat)
+        hipLaunchKernelGGL((step3_tile_kernel<TY, TZ, Cfg, true>), dim3((unsigned)nblocks), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((step3_tile_kernel<TY, TZ, Cfg, false>), dim3((unsigned)nblocks), dim3(64), 0, s, a);
     return (int)hipGetLastError();
 }
 

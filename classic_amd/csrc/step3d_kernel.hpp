@@ -69,10 +69,18 @@ struct Layout3 {
     }
 };
 
+// Material table: when the media holds at most kMaxMat distinct (Z, c)
+// pairs, mat[c] indexes lut (Z, c interleaved) and the unsplit kernel reads
+// one byte per cell instead of two doubles; the values are the same doubles.
+constexpr int kMaxMat = 64;
+
 struct Args3 {
     const double* __restrict__ q;    // 4 planes
     double* __restrict__ qn;         // 4 planes
     const double* __restrict__ aux;  // planes Z, c
+    const unsigned char* __restrict__ mat;  // material index per cell (nmat > 0)
+    const double* __restrict__ lut;         // 2 * nmat doubles
+    int nmat;                               // 0: read aux
     Layout3 L;
     long afs;                        // aux plane stride
     double dtdx, dtdy, dtdz;
@@ -386,7 +394,7 @@ __device__ inline Wv limited_cq(const Args3& a, const Iface& f, const Iface& fl,
     return cqxx(w1, f.s1, w2, f.s2, d, a.fwave);
 }
 
-template <int TY, int TZ, class Cfg>
+template <int TY, int TZ, class Cfg, bool MI>
 __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
 {
     using M = FastMath;
@@ -394,7 +402,20 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
     // cells outside the tile (so that the slice code has no branches)
     constexpr int kTrash = TZ * TY * 4 * 64;
     __shared__ double acc[kTrash + 64];
+    __shared__ double lut[MI ? 2 * kMaxMat : 1];
     const int l = threadIdx.x;
+    if constexpr (MI) {
+        for (int e = l; e < 2 * a.nmat; e += 64) lut[e] = a.lut[e];
+        __syncthreads();
+    }
+    auto load_mat = [&](const Args3& aa, long c) -> Mat {
+        if constexpr (MI) {
+            const int m = aa.mat[c];
+            return {lut[2 * m], lut[2 * m + 1]};
+        } else {
+            return {aa.aux[c], aa.aux[aa.afs + c]};
+        }
+    };
     int t = blockIdx.x;
     if (a.xcd) {
         // blocks are dealt to the XCDs round-robin; renumber so that XCD x

@@ -136,6 +136,13 @@ class Grid3D:
     def set_segment(self, cells):
         check(self.lib.clawgpu3_set_segment(self.h, cells))
 
+    def set_aux_table(self, enable=True):
+        """Read the material table (when set_aux found at most 64 distinct
+        (Z, c) pairs) or aux itself; returns the number of materials found."""
+        n = C.c_int(0)
+        check(self.lib.clawgpu3_set_aux_table(self.h, int(bool(enable)), C.byref(n)))
+        return n.value
+
     def set_tile(self, tile, xcd_order=True):
         check(self.lib.clawgpu3_set_tile(self.h, tile, int(xcd_order)))
 

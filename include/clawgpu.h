@@ -285,6 +285,14 @@ int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
  * xcd_order = 1 hands each XCD a contiguous range of tiles (default 0). */
 int clawgpu3_set_tile(clawgpu3_ctx* ctx, int tile, int xcd_order);
 
+/* Material table (no reference counterpart; same values, fewer bytes):
+ * clawgpu3_set_aux finds the distinct (Z, c) pairs of aux; when there are at
+ * most 64, the unsplit kernel reads a one-byte material index per cell and a
+ * table of the pairs instead of the two aux doubles, with identical results.
+ * enable = 0 makes it read aux regardless; *nmat (if not null) receives the
+ * number of materials found (0: more than 64, aux is read). */
+int clawgpu3_set_aux_table(clawgpu3_ctx* ctx, int enable, int* nmat);
+
 /* Benchmark helper (no reference counterpart): nsteps fixed-dt unsplit
  * steps (bc3 + step) timed with HIP events, as clawgpu_time_steps. */
 int clawgpu3_time_steps(clawgpu3_ctx* ctx, const double* geom, double dt, const int* method,

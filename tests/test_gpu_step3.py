@@ -141,3 +141,47 @@ def test_unsupported_options_fail_loudly():
             g.step3(0.01, method=(1, 2, 12, 0, 0, 0, 2))   # not a chkmth method
         with pytest.raises(ClawGpuError):
             g.bc3((0, 1, 1, 1, 1, 1))                       # user BC off z-hi
+
+
+def _blocky_medium(p, rng, nmat):
+    """q random, materials drawn from nmat random (Z, c) pairs assigned in
+    blocks of 3x2x2 cells (ghost cells included)."""
+    q = O3.bc3(p, rng.standard_normal(p.shape()))
+    pairs = np.stack([rng.uniform(0.5, 3.0, nmat), rng.uniform(0.5, 2.0, nmat)], axis=1)
+    nz, ny, nx, _ = p.shape(2)
+    kk, jj, ii = np.meshgrid(np.arange(nz) // 2, np.arange(ny) // 2, np.arange(nx) // 3,
+                             indexing="ij")
+    which = (kk * 7 + jj * 3 + ii * 5 + rng.integers(0, nmat, (nz, ny, nx))) % nmat
+    aux = pairs[which]
+    return q, np.ascontiguousarray(aux)
+
+
+@pytest.mark.parametrize("m3,lim", [(22, (2, 2)), (21, (3, 4)), (10, (1, 1))])
+def test_material_table_bitwise_with_aux_path(m3, lim):
+    """The one-byte material index reads the same doubles as aux: the
+    unsplit step is bit for bit the same with the table on and off, and
+    within 1e-12 of the oracle."""
+    rng = np.random.default_rng(40 + m3)
+    p = O3.Problem3(MX, MY, MZ, mthlim=lim, method=(1, 2, m3, 0, 0, 0, 0))
+    q, aux = _blocky_medium(p, rng, 7)
+    dt = 0.25 * p.geom.dx / 2.0
+    ref, cfl_ref = O3.step3(p, q, aux, dt)
+    outs = []
+    for table in (True, False):
+        with _grid(p, aux, q) as g:
+            assert g.set_aux_table(table) == 7
+            cfl = g.step3(dt, method=p.method, mthlim=lim)
+            outs.append(g.get_q())
+        assert cfl == cfl_ref
+    np.testing.assert_array_equal(outs[0], outs[1])
+    assert max_rel_err3(outs[0], ref) <= TOL
+
+
+def test_material_table_limit():
+    """64 distinct (Z, c) pairs use the table; 65 fall back to aux."""
+    p = O3.Problem3(20, 6, 5)
+    for nmat, want in ((64, 64), (65, 0)):
+        q, aux = _blocky_medium(p, np.random.default_rng(nmat), nmat)
+        assert len({(a, b) for a, b in aux.reshape(-1, 2)}) == nmat
+        with _grid(p, aux, q) as g:
+            assert g.set_aux_table(True) == want
