@@ -47,6 +47,7 @@ _SIGS = {
     "clawgpu_step2": ([_V, _d, _d, _d, _I, _I, _i, _D], _i),
     "clawgpu_dimsp2": ([_V, _d, _d, _d, _I, _I, _D, _i, _D], _i),
     "clawgpu_claw2": ([_V, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
+    "clawgpu_claw2_hooks": ([_V, _d, _d, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
     "clawgpu_set_segment_rows": ([_V, _i], _i),
     "clawgpu_set_arith": ([_V, _i], _i),
     "clawgpu_solver_probe": ([_V, _i, _i, _D, _D, _D, _D, _D, _D, _D, _D, _D], _i),

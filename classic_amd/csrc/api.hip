@@ -208,6 +208,7 @@ extern "C" int clawgpu_set_aux(clawgpu_ctx* c, const double* aux)
                        c->maux);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->haux.assign(aux, aux + (size_t)n * c->maux);
     return CLAWGPU_OK;
 }
 
@@ -461,12 +462,19 @@ extern "C" int clawgpu_dimsp2(clawgpu_ctx* c, double dx, double dy, double dt, c
 // step bc2 -> fused step into the other buffer -> Courant number back to the
 // host -> accept (swap buffers) or reject (keep the old buffer, which is the
 // reference's copyq2 restore, and retake with a smaller dt).
-extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart, double* tend,
-                             double* dtv, double* cflv, int* nv, const int* method,
-                             const int* mthlim, const int* mthbc, int use_fwave, int* info_out)
+// The claw2 loop (claw2.f:596-784).  With hooks, the user's bc2, b4step2
+// and src2 are called on a host copy of q at the reference's points:
+// bc2 and b4step2 before every attempt (retakes included), src2 over dt/2
+// before the step and dt/2 after it (method(5) = 2, the pre-source state
+// kept for a retake) or over dt after it (method(5) = 1).
+static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, double ylower,
+                      double tstart, double* tend, double* dtv, double* cflv, int* nv,
+                      const int* method, const int* mthlim, const int* mthbc, int use_fwave,
+                      const clawgpu_hooks* h, int* info_out)
 {
     if (!c || !tend || !dtv || !cflv || !nv || !method || !mthlim || !mthbc || !info_out)
         return set_err(CLAWGPU_ERR_ARG, "null argument");
+    const bool user_bc = h && h->bc2;
     const bool split = method[2] < 0;
     int rc = split ? CLAWGPU_OK : check_method(c, method, mthlim);
     if (rc) return rc;
@@ -478,8 +486,15 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "split steps are not supported for solvers with aux arrays");
     if (method[5] != 0 && !general_path(c))
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported");
-    if (check_bc(mthbc) != 0)
+    if (!user_bc && check_bc(mthbc) != 0)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides)");
+    std::vector<double> hq, saved;  // host copies of q for the hooks
+    double* haux = c->maux > 0 && !c->haux.empty() ? c->haux.data() : nullptr;
+    auto pull = [&]() -> int {
+        hq.resize((size_t)ncells_ghost(c) * c->meqn);
+        return clawgpu_get_q(c, hq.data());
+    };
+    auto push = [&]() -> int { return clawgpu_set_q(c, hq.data()); };
     int info = 0;
     double t = tstart;
     int maxn = nv[0];
@@ -505,8 +520,31 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
         double told = t;
         if (told + dt > *tend && tstart < *tend) dt = *tend - told;
         for (;;) {
-            rc = enqueue_bc(c, c->dq[c->cur], mthbc, true);
-            if (rc) return rc;
+            if (user_bc) {
+                if ((rc = pull())) return rc;
+                h->bc2(h->user, c->meqn, c->mbc, c->mx, c->my, xlower, ylower, dx, dy, hq.data(),
+                       c->maux, haux, told, dt, mthbc);
+                if ((rc = push())) return rc;
+                HIPCHK(c, hipMemsetAsync(c->dcfl, 0, 4 * sizeof(unsigned long long), c->stream));
+            } else {
+                rc = enqueue_bc(c, c->dq[c->cur], mthbc, true);
+                if (rc) return rc;
+            }
+            if (h && h->b4step2) {
+                if ((rc = pull())) return rc;
+                h->b4step2(h->user, c->mbc, c->mx, c->my, c->meqn, hq.data(), xlower, ylower, dx, dy,
+                           told, dt, c->maux, haux);
+                if ((rc = push())) return rc;
+                if (haux && (rc = clawgpu_set_aux(c, c->haux.data()))) return rc;
+            }
+            const bool strang_src = method[4] == 2 && h && h->src2;
+            if (strang_src) {
+                if ((rc = pull())) return rc;
+                saved = hq;
+                h->src2(h->user, c->meqn, c->mbc, c->mx, c->my, xlower, ylower, dx, dy, hq.data(),
+                        c->maux, haux, told, dt / 2.0);
+                if ((rc = push())) return rc;
+            }
             int res = (c->cur + 1) % 3;
             if (split) {
                 rc = enqueue_split(c, dx, dy, dt, method, mthlim, use_fwave, &res);
@@ -536,12 +574,22 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
                 break;
             }
             t = told;  // reject: dq[cur] still holds the step's initial data
+            if (strang_src) {  // back to the state before the source half step
+                hq = saved;
+                if ((rc = push())) return rc;
+            }
             if (method[0] == 1) {
                 dt = fmin(dtv[1], dt * cflv[1] / cfl);
                 continue;
             }
             cflmax = fmax(cfl, cflmax);
             goto done;
+        }
+        if (h && h->src2 && (method[4] == 1 || method[4] == 2)) {
+            if ((rc = pull())) return rc;
+            h->src2(h->user, c->meqn, c->mbc, c->mx, c->my, xlower, ylower, dx, dy, hq.data(),
+                    c->maux, haux, t, method[4] == 2 ? dt / 2.0 : dt);
+            if ((rc = push())) return rc;
         }
         if (method[0] == 1) {
             if (cfl == 0.0) dt = dtv[1];
@@ -563,6 +611,27 @@ done:
     dtv[4] = dt;
     *info_out = info;
     return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart, double* tend,
+                             double* dtv, double* cflv, int* nv, const int* method,
+                             const int* mthlim, const int* mthbc, int use_fwave, int* info_out)
+{
+    return claw2_loop(c, dx, dy, 0.0, 0.0, tstart, tend, dtv, cflv, nv, method, mthlim, mthbc,
+                      use_fwave, nullptr, info_out);
+}
+
+extern "C" int clawgpu_claw2_hooks(clawgpu_ctx* c, double dx, double dy, double xlower,
+                                   double ylower, double tstart, double* tend, double* dtv,
+                                   double* cflv, int* nv, const int* method, const int* mthlim,
+                                   const int* mthbc, int use_fwave, const clawgpu_hooks* hooks,
+                                   int* info_out)
+{
+    if (!hooks) return set_err(CLAWGPU_ERR_ARG, "null hooks (use clawgpu_claw2)");
+    if (c && c->maux > 0 && c->haux.empty() && (hooks->b4step2 || hooks->src2 || hooks->bc2))
+        return set_err(CLAWGPU_ERR_ARG, "aux has not been set (clawgpu_set_aux)");
+    return claw2_loop(c, dx, dy, xlower, ylower, tstart, tend, dtv, cflv, nv, method, mthlim,
+                      mthbc, use_fwave, hooks, info_out);
 }
 
 namespace {

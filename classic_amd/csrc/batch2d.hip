@@ -599,18 +599,18 @@ extern "C" int clawgpu_batch_get_schedule(clawgpu_batch* b, int* sched)
 // time `reps` dry iterations under the current schedule (ms per iteration)
 static int time_dry(clawgpu_batch* b, int reps, float* ms)
 {
-    hipEvent_t e0, e1;
-    HIPCHK(b, hipEventCreate(&e0));
-    HIPCHK(b, hipEventCreate(&e1));
-    hipError_t e = enqueue_iteration(b, 1);  // warm-up
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = enqueue_iteration(b, 1);  // warm-up
     if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
     for (int i = 0; i < reps && e == hipSuccess; i++) e = enqueue_iteration(b, 1);
     if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float t = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     HIPCHK(b, e);
     *ms = t / reps;
     return CLAWGPU_OK;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <vector>
 
 #include "common.hpp"
 #include "../../include/clawgpu.h"
@@ -25,6 +26,7 @@ struct clawgpu_ctx {
     int seglen;
     int arith;
     double* daux = nullptr;   // aux planes (solvers with aux arrays)
+    std::vector<double> haux;  // host copy of aux (reference layout) for user hooks
     double* grec = nullptr;   // records of the general step (step2g.hip)
     // learned limiter state (allocated on first use)
     double *phix = nullptr, *phiy = nullptr;  // phi tables read by the step kernel

@@ -40,6 +40,69 @@ def solver_params(solver_id, rho=1.0, bulk=4.0, gamma=1.4, efix=True):
     raise ValueError("unknown solver")
 
 
+_DBL = C.c_double
+_BC2_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _DBL, _DBL, _DBL, _DBL,
+                      C.POINTER(_DBL), C.c_int, C.POINTER(_DBL), _DBL, _DBL, C.POINTER(C.c_int))
+_B4STEP2_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_DBL),
+                          _DBL, _DBL, _DBL, _DBL, _DBL, _DBL, C.c_int, C.POINTER(_DBL))
+_SRC2_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _DBL, _DBL, _DBL, _DBL,
+                       C.POINTER(_DBL), C.c_int, C.POINTER(_DBL), _DBL, _DBL)
+
+
+class _Hooks(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("bc2", _BC2_FN), ("b4step2", _B4STEP2_FN),
+                ("src2", _SRC2_FN)]
+
+
+def _view(ptr, my, mx, m):
+    if not ptr or m == 0:
+        return None
+    return np.ctypeslib.as_array(ptr, shape=(my, mx, m))
+
+
+def make_hooks(bc2=None, b4step2=None, src2=None):
+    """C hook table for clawgpu_claw2_hooks from Python callables, each
+    called with numpy views of the host copies of q and aux (reference
+    layout, ghost cells included; aux None when maux = 0) to modify in place:
+    bc2(q, aux, t, dt, mthbc), b4step2(q, aux, t, dt), src2(q, aux, t, dt).
+    Returns (table, keep, errors): keep holds the ctypes callbacks alive,
+    errors collects exceptions raised inside the hooks."""
+    errors = []
+    h = _Hooks()
+    keep = []
+
+    def guard(fn):
+        def run(*a):
+            try:
+                fn(*a)
+            except BaseException as e:  # a callback cannot raise through C
+                errors.append(e)
+        return run
+
+    if bc2 is not None:
+        def _bc(user, meqn, mbc, mx, my, xl, yl, dx, dy, q, maux, aux, t, dt, mthbc):
+            n = (my + 2 * mbc, mx + 2 * mbc)
+            bc2(_view(q, *n, meqn), _view(aux, *n, maux), t, dt, tuple(mthbc[k] for k in range(4)))
+        f = _BC2_FN(guard(_bc))
+        keep.append(f)
+        h.bc2 = f
+    if b4step2 is not None:
+        def _b4(user, mbc, mx, my, meqn, q, xl, yl, dx, dy, t, dt, maux, aux):
+            n = (my + 2 * mbc, mx + 2 * mbc)
+            b4step2(_view(q, *n, meqn), _view(aux, *n, maux), t, dt)
+        f = _B4STEP2_FN(guard(_b4))
+        keep.append(f)
+        h.b4step2 = f
+    if src2 is not None:
+        def _src(user, meqn, mbc, mx, my, xl, yl, dx, dy, q, maux, aux, t, dt):
+            n = (my + 2 * mbc, mx + 2 * mbc)
+            src2(_view(q, *n, meqn), _view(aux, *n, maux), t, dt)
+        f = _SRC2_FN(guard(_src))
+        keep.append(f)
+        h.src2 = f
+    return h, keep, errors
+
+
 class Grid2D:
     """Device-resident single grid (one clawgpu context)."""
 
@@ -116,8 +179,12 @@ class Grid2D:
         return cfl.value
 
     def claw2(self, dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc,
-              use_fwave=False):
-        """claw2 time loop on the device q; returns dict of the outputs."""
+              use_fwave=False, bc2=None, b4step2=None, src2=None, xlower=0.0, ylower=0.0):
+        """claw2 time loop on the device q; returns dict of the outputs.
+        bc2 / b4step2 / src2: optional user routines (see make_hooks)."""
+        if bc2 is not None or b4step2 is not None or src2 is not None:
+            return self._claw2_hooks(dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim,
+                                     mthbc, use_fwave, bc2, b4step2, src2, xlower, ylower)
         te = C.c_double(tend)
         dtv_a = np.zeros(5)
         dtv_a[: len(dtv)] = dtv
@@ -130,6 +197,28 @@ class Grid2D:
         rc = self.lib.clawgpu_claw2(self.h, dx, dy, tstart, C.byref(te), _dp(dtv_a), _dp(cflv_a),
                                     _ip(nv), _ip(m), _ip(_ivec(mthlim, self.mwaves)),
                                     _ip(_ivec(mthbc, 4)), int(bool(use_fwave)), C.byref(info))
+        check(rc)
+        return dict(info=info.value, tend=te.value, dtv=dtv_a, cflv=cflv_a, nv=nv)
+
+    def _claw2_hooks(self, dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc,
+                     use_fwave, bc2, b4step2, src2, xlower, ylower):
+        te = C.c_double(tend)
+        dtv_a = np.zeros(5)
+        dtv_a[: len(dtv)] = dtv
+        cflv_a = np.zeros(4)
+        cflv_a[: len(cflv)] = cflv
+        nv = np.array([nmax, 0], dtype=np.int32)
+        info = C.c_int(0)
+        m = _ivec(method, 7).copy()
+        m[6] = self.maux
+        hooks, keep, errors = make_hooks(bc2, b4step2, src2)
+        rc = self.lib.clawgpu_claw2_hooks(self.h, dx, dy, float(xlower), float(ylower), tstart,
+                                          C.byref(te), _dp(dtv_a), _dp(cflv_a), _ip(nv), _ip(m),
+                                          _ip(_ivec(mthlim, self.mwaves)), _ip(_ivec(mthbc, 4)),
+                                          int(bool(use_fwave)), C.byref(hooks), C.byref(info))
+        del keep
+        if errors:
+            raise errors[0]
         check(rc)
         return dict(info=info.value, tend=te.value, dtv=dtv_a, cflv=cflv_a, nv=nv)
 

@@ -117,6 +117,36 @@ int clawgpu_claw2(clawgpu_ctx* ctx, double dx, double dy, double tstart, double*
                   double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
                   const int* mthbc, int use_fwave, int* info);
 
+/* User routines of the reference's plug-in contract (src/2d/claw2.f:404-447:
+ * bc2 with user boundary conditions, b4step2, src2), as C callbacks with a
+ * user pointer first and the reference's arguments by value (arrays by
+ * pointer, reference layout, ghost cells included; aux is NULL when
+ * maux = 0).  The library copies q to the host for each call and back after
+ * it (and aux after b4step2), so hooks cost two transfers of q per call.
+ * Any member may be NULL: bc2 = NULL uses the built-in codes 1-3, a NULL
+ * b4step2 / src2 does nothing (the reference's stubs). */
+typedef struct clawgpu_hooks {
+    void* user;
+    void (*bc2)(void* user, int meqn, int mbc, int mx, int my, double xlower, double ylower,
+                double dx, double dy, double* q, int maux, double* aux, double t, double dt,
+                const int* mthbc);
+    void (*b4step2)(void* user, int mbc, int mx, int my, int meqn, double* q, double xlower,
+                    double ylower, double dx, double dy, double t, double dt, int maux,
+                    double* aux);
+    void (*src2)(void* user, int meqn, int mbc, int mx, int my, double xlower, double ylower,
+                 double dx, double dy, double* q, int maux, double* aux, double t, double dt);
+} clawgpu_hooks;
+
+/* clawgpu_claw2 with user routines called where claw2 calls them
+ * (claw2.f:620-737): bc2 then b4step2 before every step attempt (again on a
+ * retake); with method(5) = 2 src2 over dt/2 before the step (its input
+ * restored on a retake) and over dt/2 at t + dt after it, with method(5) = 1
+ * src2 over dt after it.  xlower, ylower are passed to the hooks. */
+int clawgpu_claw2_hooks(clawgpu_ctx* ctx, double dx, double dy, double xlower, double ylower,
+                        double tstart, double* tend, double* dtv, double* cflv, int* nv,
+                        const int* method, const int* mthlim, const int* mthbc, int use_fwave,
+                        const clawgpu_hooks* hooks, int* info);
+
 /* Tuning and measurement helpers (no reference counterpart): rows per
  * wavefront segment of the step kernel, the arithmetic mode, and nsteps fixed-dt steps timed with
  * HIP events on the context's stream without per-step host synchronisation:

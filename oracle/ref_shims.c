@@ -108,6 +108,77 @@ int ref_claw2(int meqn, int mwaves, int maux, int mbc, int mx, int my, double* q
     return info;
 }
 
+/* Test user routines for the plug-in contract: b4step2 and src2 append
+ * (kind, t, dt) to a log (kind 1 = b4step2, 2 = src2); src2 applies the decay
+ * q(1) <- q(1) - (k dt) q(1) on the interior cells.  ref_claw2_hooks runs the
+ * reference claw2 with them. */
+static double g_hook_k = 0.0;
+static double* g_hook_log = 0;
+static int g_hook_n = 0, g_hook_cap = 0;
+
+void ref_hooks_set(double k, double* log, int cap)
+{
+    g_hook_k = k;
+    g_hook_log = log;
+    g_hook_cap = cap;
+    g_hook_n = 0;
+}
+
+int ref_hooks_count(void) { return g_hook_n; }
+
+static void hook_log(double kind, double t, double dt)
+{
+    if (g_hook_n < g_hook_cap) {
+        g_hook_log[3 * g_hook_n] = kind;
+        g_hook_log[3 * g_hook_n + 1] = t;
+        g_hook_log[3 * g_hook_n + 2] = dt;
+    }
+    g_hook_n++;
+}
+
+static void test_b4step2(int* mbc, int* mx, int* my, int* meqn, double* q, double* xlower,
+                         double* ylower, double* dx, double* dy, double* t, double* dt, int* maux,
+                         double* aux)
+{
+    (void)mbc; (void)mx; (void)my; (void)meqn; (void)q; (void)xlower; (void)ylower; (void)dx;
+    (void)dy; (void)maux; (void)aux;
+    hook_log(1.0, *t, *dt);
+}
+
+static void test_src2(int* meqn, int* mbc, int* mx, int* my, double* xlower, double* ylower,
+                      double* dx, double* dy, double* q, int* maux, double* aux, double* t,
+                      double* dt)
+{
+    (void)xlower; (void)ylower; (void)dx; (void)dy; (void)maux; (void)aux;
+    hook_log(2.0, *t, *dt);
+    const double kd = g_hook_k * *dt;
+    const long nx = *mx + 2 * *mbc;
+    for (int j = 1; j <= *my; j++)
+        for (int i = 1; i <= *mx; i++) {
+            double* v = &q[((long)(j + *mbc - 1) * nx + (i + *mbc - 1)) * *meqn];
+            v[0] = v[0] - kd * v[0];
+        }
+}
+
+int ref_claw2_hooks(int meqn, int mwaves, int maux, int mbc, int mx, int my, double* q,
+                    double* aux, double xlower, double ylower, double dx, double dy, double tstart,
+                    double* tend, double* dtv, double* cflv, int* nv, int* method, int* mthlim,
+                    int* mthbc, int use_fwave)
+{
+    int narray = method[4] < 2 ? 1 : 2;
+    int mwork = mwork_of(meqn, mwaves, maux, mbc, mx, my, narray);
+    double* work = (double*)calloc((size_t)mwork, sizeof(double));
+    double dummy_aux[1] = {0.0};
+    int info = 0, fw = use_fwave ? 1 : 0;
+    if (!work) return -3;
+    claw2_(&meqn, &mwaves, &maux, &mbc, &mx, &my, q, maux > 0 ? aux : dummy_aux, &xlower,
+           &ylower, &dx, &dy, &tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, work, &mwork,
+           &fw, &info, (void*)bc2_, (void*)f_rpn2, (void*)f_rpt2, (void*)test_src2,
+           (void*)test_b4step2);
+    free(work);
+    return info;
+}
+
 /* one step2 or dimsp2 call with claw2's work partition (claw2.f:565-587) */
 static int ref_step_common(int split, int meqn, int mwaves, int maux, int mbc, int mx, int my,
                            double* qold, double* qnew, double* aux, double dx, double dy,

@@ -7,6 +7,8 @@ the CPU oracle's claw2 produces), including the dt sequence, retakes,
 Courant numbers and the info codes.  The schedule (tile height, grid size,
 tile order), graph replay and autotuning must not change any bit.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -220,3 +222,40 @@ def test_batch_256_problem_workload_sample():
             assert r["nv"][1] >= 1
             np.testing.assert_array_equal(interior(qb), interior(qg))
             assert r["tend"] == rg["tend"]
+
+
+def test_batch_of_one_and_zero_steps():
+    """One problem equals its single-grid run; nv(1) = 0 finishes every
+    problem at once with the initial data untouched."""
+    specs, qs = _specs(7, sizes=[(45, 38, "euler")], bcs=[(1, 1, 1, 1)])
+    qb, rb = _batch(specs, qs, [0.05], [1e-3, 1e99], [1.0, 0.9], 5000)
+    qg, rg = _single(specs[0], qs[0], 0.05, [1e-3, 1e99], [1.0, 0.9], 5000)
+    np.testing.assert_array_equal(interior(qb[0]), interior(qg))
+    _same_outputs(rb[0], rg)
+    specs, qs = _specs(8)
+    with Batch2D(specs) as b:
+        for k, q in enumerate(qs):
+            b.set_q(k, q)
+        b.start(0.0, 1.0, [1e-3, 1e99], [1.0, 0.9], 0, METHOD, MTHLIM)
+        assert b.advance(10) == 0
+        for k in range(len(specs)):
+            r = b.result(k)
+            assert r["nv"][1] == 0 and r["info"] == 0 and r["tend"] == 0.0
+            np.testing.assert_array_equal(interior(b.get_q(k)), interior(qs[k]))
+
+
+def test_batch_rejects_bad_arguments():
+    from classic_amd._lib import ClawGpuError
+    specs, _ = _specs(9, sizes=[(10, 10, "euler")], bcs=[(2, 1, 1, 1)])
+    with pytest.raises(ClawGpuError):
+        Batch2D(specs)  # periodic on one side only
+    specs, qs = _specs(9, sizes=[(10, 10, "acoustics")], bcs=[(1, 1, 1, 1)])
+    with Batch2D(specs) as b:
+        with pytest.raises(ClawGpuError):
+            b.start(0.0, 1.0, [1e-3, 1e99], [1.0, 0.9], 10, (1, 2, -1, 0, 0, 0, 0), MTHLIM)
+        with pytest.raises(ClawGpuError):
+            b.start(0.0, 1.0, [1e-3, 1e99], [0.9, 1.0], 10, METHOD, MTHLIM)  # cflv(2) > cflv(1)
+        with pytest.raises(ClawGpuError):
+            b.advance(1)  # not started
+        q = np.ascontiguousarray(qs[0])
+        assert b.lib.clawgpu_batch_set_q(b.h, 1, q.ctypes.data_as(C.POINTER(C.c_double))) != 0
