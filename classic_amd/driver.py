@@ -101,8 +101,11 @@ class _Stepper:
     def claw(self, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc, use_fwave):
         """One call of the claw time loop; returns (tend, info, dtv, cflv, nv)."""
         if self.ndim == 2:
+            kw = dict(getattr(self, "hooks", {}))
+            if kw:
+                kw.update(xlower=self.c.lower[0], ylower=self.c.lower[1])
             r = self.g.claw2(self.delta[0], self.delta[1], tstart, tend, dtv, cflv, nmax, method,
-                             mthlim, mthbc, use_fwave)
+                             mthlim, mthbc, use_fwave, **kw)
             return r["tend"], r["info"], r["dtv"], r["cflv"], r["nv"]
         if self.ndim == 1:
             r = self.g.claw1(self.delta[0], tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc,
@@ -113,11 +116,18 @@ class _Stepper:
 
 
 def run(rundir=".", outdir=None, qinit=None, setaux=None, setprob=None, solver="acoustics",
-        solver_params=None, arith=0, grid=None, verbose=False):
+        solver_params=None, arith=0, grid=None, verbose=False, bc2=None, b4step2=None, src2=None):
     """Run the claw*ez driver for the claw.data in rundir; frames go to
     outdir (default: rundir).  Returns a RunResult.  ``grid`` may supply an
-    already constructed stepping object with the Grid*D interface."""
+    already constructed stepping object with the Grid*D interface.  In 2D,
+    bc2 / b4step2 / src2 are the user routines (see gpu.make_hooks); bc2 is
+    required when a boundary code is 0 (user boundary)."""
     c = read_claw_data(os.path.join(rundir, "claw.data"))
+    hooks = {k: v for k, v in (("bc2", bc2), ("b4step2", b4step2), ("src2", src2)) if v is not None}
+    if hooks and c.ndim != 2:
+        raise ValueError("user routines are supported in 2D only")
+    if 0 in c.mthbc and "bc2" not in hooks and not (c.ndim == 3 and c.mthbc == [1, 1, 1, 1, 1, 0]):
+        raise ValueError("a boundary code 0 (user boundary) needs the bc2 routine")
     nd = c.ndim
     if nd == 1:
         c["restart"] = False  # claw1ez reads no restart record
@@ -129,6 +139,7 @@ def run(rundir=".", outdir=None, qinit=None, setaux=None, setprob=None, solver="
     if qinit is None and not c.restart:
         raise ValueError("qinit is required unless the run restarts from a frame")
     st = _Stepper(nd, c, solver, solver_params, arith, grid)
+    st.hooks = hooks
     res = RunResult()
     info_path = os.path.join(outdir, "fort.info")
     open(info_path, "w").close()  # claw2ez.f:35 opens fort.info anew
