@@ -35,6 +35,25 @@ __device__ inline double lane_dn(double v)
 #endif
 }
 
+// the same shifts for one 32-bit integer
+__device__ inline int lane_up_i(int v)
+{
+#if CLAWGPU_DPP
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+#else
+    return __shfl_up(v, 1);
+#endif
+}
+
+__device__ inline int lane_dn_i(int v)
+{
+#if CLAWGPU_DPP
+    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+#else
+    return __shfl_down(v, 1);
+#endif
+}
+
 template <class T>
 __device__ inline T lane_up_t(const T& v)
 {

@@ -13,10 +13,15 @@ static int launch_cfg(Args3 a, hipStream_t s)
     long nblocks = (long)a.ntx * a.nty * a.ntz;
     if (nblocks <= 0) return 0;
     if (a.xcd) nblocks = 8 * ((nblocks + 7) / 8);
-    if (a.nmat > 0 && a.nmat <= kMaxMat)
-        hipLaunchKernelGGL((step3_tile_kernel<TY, TZ, Cfg, true>), dim3((unsigned)nblocks), dim3(64), 0, s, a);
+    // the material-table variants are compiled for the two fastest tiles
+    constexpr bool tables = (TY == 4 && TZ == 2) || (TY == 2 && TZ == 4);
+    const dim3 g((unsigned)nblocks);
+    if (tables && a.nmat > 0 && a.nmat <= kMaxKcMat)
+        hipLaunchKernelGGL((step3_tile_kernel<TY, TZ, Cfg, tables, tables>), g, dim3(64), 0, s, a);
+    else if (tables && a.nmat > 0 && a.nmat <= kMaxMat)
+        hipLaunchKernelGGL((step3_tile_kernel<TY, TZ, Cfg, tables, false>), g, dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL((step3_tile_kernel<TY, TZ, Cfg, false>), dim3((unsigned)nblocks), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((step3_tile_kernel<TY, TZ, Cfg, false, false>), g, dim3(64), 0, s, a);
     return (int)hipGetLastError();
 }
 

@@ -73,6 +73,9 @@ struct Layout3 {
 // pairs, mat[c] indexes lut (Z, c interleaved) and the unsplit kernel reads
 // one byte per cell instead of two doubles; the values are the same doubles.
 constexpr int kMaxMat = 64;
+// up to this many materials the transverse coefficients of every material
+// triple are tabulated too (kMaxKcMat^3 entries of 32 bytes in LDS)
+constexpr int kMaxKcMat = 4;
 
 struct Args3 {
     const double* __restrict__ q;    // 4 planes
@@ -91,6 +94,7 @@ struct Args3 {
 
 struct Mat {
     double Z, c;
+    int id;  // material index (material-table mode; 0 otherwise)
 };
 struct Wv {
     double p, n;  // pressure and normal velocity components
@@ -130,12 +134,12 @@ __device__ inline int Cfg3Any::m4(const Args3& a) { return a.m4; }
 __device__ inline int Cfg3Any::lim0(const Args3& a) { return a.lim0; }
 __device__ inline int Cfg3Any::lim1(const Args3& a) { return a.lim1; }
 
-__device__ inline Mat mat_up(Mat m) { return {lane_up(m.Z), lane_up(m.c)}; }
-__device__ inline Mat mat_dn(Mat m) { return {lane_dn(m.Z), lane_dn(m.c)}; }
+__device__ inline Mat mat_up(Mat m) { return {lane_up(m.Z), lane_up(m.c), lane_up_i(m.id)}; }
+__device__ inline Mat mat_dn(Mat m) { return {lane_dn(m.Z), lane_dn(m.c), lane_dn_i(m.id)}; }
 __device__ inline Wv wv_up(Wv w) { return {lane_up(w.p), lane_up(w.n)}; }
 __device__ inline Wv wv_dn(Wv w) { return {lane_dn(w.p), lane_dn(w.n)}; }
 
-__device__ inline Mat load_mat(const Args3& a, long c) { return {a.aux[c], a.aux[a.afs + c]}; }
+__device__ inline Mat load_mat(const Args3& a, long c) { return {a.aux[c], a.aux[a.afs + c], 0}; }
 
 // Interface between cell L (i-1) and R (i): the jump in (p, u_n) splits into
 // the left-going wave (-Z_L, 1) a1 at speed -c_L and the right-going wave
@@ -243,25 +247,32 @@ __device__ inline void rec_side(Rec& R, double a, double cq, double sgn, const K
     }
 }
 
-// N[a+1][b+1] = materials of the cell at offset a along t1 and b along t2
-template <class M>
+// N[a+1][b+1] = materials of the cell at offset a along t1 and b along t2.
+// With KL the transverse coefficients of a (lo, mid, hi) material triple
+// come from the table klut (built by kcoef itself, so the values are the
+// same bits) instead of two reciprocals each.
+template <class M, bool KL>
 __device__ inline void make_record(Rec& R, double ap, double cqp, double am, double cqm,
                                    const Mat (&N)[3][3], double d, double d1, double d2, int m3,
-                                   int m4)
+                                   int m4, const Kc* klut, int nm)
 {
 #pragma unroll
     for (int s = 0; s < 2; s++)
 #pragma unroll
         for (int t = 0; t < 3; t++) R.gp[s][t] = R.gt[s][t] = R.hp[s][t] = R.ht[s][t] = 0.0;
     if (m3 <= 0) return;
-    Kc Y0 = kcoef<M>(N[0][1], N[1][1], N[2][1]);
-    Kc Z0 = kcoef<M>(N[1][0], N[1][1], N[1][2]);
+    auto kc = [&](const Mat& lo, const Mat& mid, const Mat& hi) -> Kc {
+        if constexpr (KL) return klut[(lo.id * nm + mid.id) * nm + hi.id];
+        else return kcoef<M>(lo, mid, hi);
+    };
+    Kc Y0 = kc(N[0][1], N[1][1], N[2][1]);
+    Kc Z0 = kc(N[1][0], N[1][1], N[1][2]);
     Kc Ym = {0, 0, 0, 0}, Yp = Ym, Zm = Ym, Zp = Ym;
     if (m4 > 0) {
-        Ym = kcoef<M>(N[0][0], N[1][0], N[2][0]);
-        Yp = kcoef<M>(N[0][2], N[1][2], N[2][2]);
-        Zm = kcoef<M>(N[0][0], N[0][1], N[0][2]);
-        Zp = kcoef<M>(N[2][0], N[2][1], N[2][2]);
+        Ym = kc(N[0][0], N[1][0], N[2][0]);
+        Yp = kc(N[0][2], N[1][2], N[2][2]);
+        Zm = kc(N[0][0], N[0][1], N[0][2]);
+        Zp = kc(N[2][0], N[2][1], N[2][2]);
     }
     rec_side(R, ap, cqp, 1.0, Y0, Z0, Ym, Yp, Zm, Zp, d, d1, d2, m3, m4);
     rec_side(R, am, cqm, -1.0, Y0, Z0, Ym, Yp, Zm, Zp, d, d1, d2, m3, m4);
@@ -372,13 +383,14 @@ __device__ inline void scatter(double* acc, const T9& T, bool trans, int cn, int
 
 // One slice position from its two limited interfaces: low interface lo (A^+
 // side, cqxx cqlo) and high interface hi (A^- side, cqxx cqhi).
-template <class M, class Cfg>
+template <class M, class Cfg, bool KL>
 __device__ inline void slice_position(const Args3& a, const Iface& lo, Wv cqlo, const Iface& hi,
                                       Wv cqhi, const Mat (&N)[3][3], double d, double d1,
-                                      double d2, T9& T)
+                                      double d2, T9& T, const Kc* klut, int nm)
 {
     Rec R;
-    make_record<M>(R, lo.ap.p, cqlo.p, hi.am.p, cqhi.p, N, d, d1, d2, Cfg::m3(a), Cfg::m4(a));
+    make_record<M, KL>(R, lo.ap.p, cqlo.p, hi.am.p, cqhi.p, N, d, d1, d2, Cfg::m3(a), Cfg::m4(a),
+                       klut, nm);
     double qp = (-d * lo.ap.p) - d * hi.am.p;
     double qn = (-d * lo.ap.n) - d * hi.am.n;
     apply_targets(R, qp, qn, cqhi.p - cqlo.p, cqhi.n - cqlo.n, d, d1, d2, Cfg::m3(a) > 0, T);
@@ -394,26 +406,39 @@ __device__ inline Wv limited_cq(const Args3& a, const Iface& f, const Iface& fl,
     return cqxx(w1, f.s1, w2, f.s2, d, a.fwave);
 }
 
-template <int TY, int TZ, class Cfg, bool MI>
+template <int TY, int TZ, class Cfg, bool MI, bool KL>
 __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
 {
     using M = FastMath;
+    static_assert(MI || !KL, "the coefficient table needs the material table");
     // accumulator of the tile, plus one row that receives the increments of
     // cells outside the tile (so that the slice code has no branches)
     constexpr int kTrash = TZ * TY * 4 * 64;
     __shared__ double acc[kTrash + 64];
     __shared__ double lut[MI ? 2 * kMaxMat : 1];
+    __shared__ Kc klut_s[KL ? kMaxKcMat * kMaxKcMat * kMaxKcMat : 1];
+    const Kc* klut = klut_s;
     const int l = threadIdx.x;
     if constexpr (MI) {
         for (int e = l; e < 2 * a.nmat; e += 64) lut[e] = a.lut[e];
+        if constexpr (KL) {
+            // transverse coefficients of every (lo, mid, hi) material triple
+            const int n = a.nmat;
+            for (int e = l; e < n * n * n; e += 64) {
+                const int lo = e / (n * n), mid = (e / n) % n, hi = e % n;
+                klut_s[e] = kcoef<M>(Mat{a.lut[2 * lo], a.lut[2 * lo + 1], lo},
+                                     Mat{a.lut[2 * mid], a.lut[2 * mid + 1], mid},
+                                     Mat{a.lut[2 * hi], a.lut[2 * hi + 1], hi});
+            }
+        }
         __syncthreads();
     }
     auto load_mat = [&](const Args3& aa, long c) -> Mat {
         if constexpr (MI) {
             const int m = aa.mat[c];
-            return {lut[2 * m], lut[2 * m + 1]};
+            return {lut[2 * m], lut[2 * m + 1], m};
         } else {
-            return {aa.aux[c], aa.aux[aa.afs + c]};
+            return {aa.aux[c], aa.aux[aa.afs + c], 0};
         }
     };
     int t = blockIdx.x;
@@ -470,7 +495,7 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                 hi.am = wv_dn(f.am);
                 Wv cqhi = wv_dn(cq);
                 T9 T;
-                slice_position<M, Cfg>(a, f, cq, hi, cqhi, W, d, d1, d2, T);
+                slice_position<M, Cfg, KL>(a, f, cq, hi, cqhi, W, d, d1, d2, T, klut, a.nmat);
                 // x: n = u, t1 = v (rows, offset a), t2 = w (planes, offset b)
                 scatter<0>(acc, T, trans, 1, 2, 3, [&](int t, int m) {
                     const int jy = jp + (t % 3 - 1) - j0, kz = kp + (t / 3 - 1) - k0;
@@ -510,7 +535,7 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                         N[s][2] = mat_dn(C3[r][s]);
                     }
                     T9 T;
-                    slice_position<M, Cfg>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2, T);
+                    slice_position<M, Cfg, KL>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2, T, klut, a.nmat);
                     // y: n = v, t1 = w (planes, offset a), t2 = u (lanes, offset b)
                     scatter<1>(acc, T, trans, 2, 3, 1, [&](int t, int m) {
                         const int kz = kp + (t % 3 - 1) - k0, ln = l + (t / 3 - 1);
@@ -551,7 +576,7 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                         N[2][s] = mat_dn(C3[r][s]);
                     }
                     T9 T;
-                    slice_position<M, Cfg>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2, T);
+                    slice_position<M, Cfg, KL>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2, T, klut, a.nmat);
                     // z: n = w, t1 = u (lanes, offset a), t2 = v (rows, offset b)
                     scatter<2>(acc, T, trans, 3, 1, 2, [&](int t, int m) {
                         const int jy = jp + (t / 3 - 1) - j0, ln = l + (t % 3 - 1);

@@ -156,20 +156,22 @@ def _blocky_medium(p, rng, nmat):
     return q, np.ascontiguousarray(aux)
 
 
+@pytest.mark.parametrize("nmat", [3, 7])
 @pytest.mark.parametrize("m3,lim", [(22, (2, 2)), (21, (3, 4)), (10, (1, 1))])
-def test_material_table_bitwise_with_aux_path(m3, lim):
-    """The one-byte material index reads the same doubles as aux: the
-    unsplit step is bit for bit the same with the table on and off, and
-    within 1e-12 of the oracle."""
-    rng = np.random.default_rng(40 + m3)
+def test_material_table_bitwise_with_aux_path(m3, lim, nmat):
+    """The one-byte material index reads the same doubles as aux, and with
+    at most 4 materials the tabulated transverse coefficients are the bits
+    kcoef computes: the unsplit step is bit for bit the same with the tables
+    on and off, and within 1e-12 of the oracle."""
+    rng = np.random.default_rng(40 + m3 + nmat)
     p = O3.Problem3(MX, MY, MZ, mthlim=lim, method=(1, 2, m3, 0, 0, 0, 0))
-    q, aux = _blocky_medium(p, rng, 7)
+    q, aux = _blocky_medium(p, rng, nmat)
     dt = 0.25 * p.geom.dx / 2.0
     ref, cfl_ref = O3.step3(p, q, aux, dt)
     outs = []
     for table in (True, False):
         with _grid(p, aux, q) as g:
-            assert g.set_aux_table(table) == 7
+            assert g.set_aux_table(table) == nmat
             cfl = g.step3(dt, method=p.method, mthlim=lim)
             outs.append(g.get_q())
         assert cfl == cfl_ref
