@@ -49,26 +49,66 @@ def profile_traffic(name):
     return None
 
 
+REF_NOTE = ("reference Fortran (src/2d or src/3d claw/step/flux/limiter/bc, flang -O2, built "
+            "from the reference sources into oracle/_ref) with the project's Riemann solvers")
+PORT_NOTE = "oracle C restatement compiled -O2"
+
+
+class _QuietStdout:
+    """Send file descriptor 1 to /dev/null (the reference Fortran prints its
+    own messages there) so that stdout carries only the JSON line; the
+    Fortran unit is flushed before the descriptor is restored."""
+
+    def __init__(self, mod=None):
+        self.mod = mod
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        self.null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self.null, 1)
+        return self
+
+    def __exit__(self, *exc):
+        if self.mod is not None and self.mod.ref_available():
+            self.mod.ref().ref_flush_unit(6)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        os.close(self.null)
+
+
+def _backend(mod):
+    """'ref' when the reference build is present (it is built where the
+    reference sources are and travels with the tree), else the oracle."""
+    return "ref" if mod.ref_available() else "oracle"
+
+
 def cpu_baseline(mx, my, seed, nsteps=1):
-    """Time the CPU oracle (single thread, -O2) on `nsteps` steps of the same
-    workload; returns (Gcell-updates/s, seconds, sample description)."""
+    """Time the reference's CPU path (single thread) on `nsteps` steps of the
+    same workload; returns (Gcell-updates/s, seconds, sample, kind)."""
     try:
         from oracle import oracle as O
         from classic_amd import problems as P
     except Exception as e:  # pragma: no cover
-        return None, 0.0, f"oracle unavailable: {e}"
+        return None, 0.0, f"oracle unavailable: {e}", None
     try:
         O.lib()
     except Exception as e:
-        return None, 0.0, f"oracle unavailable: {e}"
+        return None, 0.0, f"oracle unavailable: {e}", None
+    backend = _backend(O)
     q, dx, dy = P.euler_quadrants(mx, my, seed=seed)
     p = O.Problem(4, 3, mx, my, O.SOLVER_EULER, O.solver_params(gamma=1.4, efix=True),
                   method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4, 4], mthbc=(1, 1, 1, 1))
-    t0 = time.perf_counter()
-    r = O.claw2(p, q, dx, dy, 0.0, 1e9, [0.9 * dx / 3.0, 1e99], [1.0, 0.9], nsteps)
-    el = time.perf_counter() - t0
+    with _QuietStdout(O):
+        t0 = time.perf_counter()
+        r = O.claw2(p, q, dx, dy, 0.0, 1e9, [0.9 * dx / 3.0, 1e99], [1.0, 0.9], nsteps,
+                    backend=backend)
+        el = time.perf_counter() - t0
     steps = int(r.nv[1])
-    return (mx * my * steps) / el / 1e9, el, f"{mx}x{my} Euler, {steps} step(s), 1 thread"
+    note = REF_NOTE if backend == "ref" else PORT_NOTE
+    return ((mx * my * steps) / el / 1e9, el,
+            f"{mx}x{my} Euler, {steps} claw2 step(s), 1 thread; {note}",
+            "reference" if backend == "ref" else "port")
 
 
 WORKLOAD3 = "acoustics3d_512cubed_vc_layered_superbee_method22_walls_100steps"
@@ -76,22 +116,26 @@ BYTES_PER_CELL_UPDATE3 = 80  # (2*meqn + maux) * 8 with meqn = 4, maux = 2
 
 
 def cpu_baseline3(n, seed):
-    """Time the CPU oracle (single thread, -O2) on one unsplit step of an
-    n^3 sample of the 3D workload; returns (Gcell-updates/s, s, sample)."""
+    """Time the reference's CPU path (single thread) on one unsplit step of an
+    n^3 sample of the 3D workload; returns (Gcell-updates/s, s, sample, kind)."""
     try:
         from oracle import oracle3 as O3
         from classic_amd import problems as P
         O3.lib()
     except Exception as e:  # pragma: no cover
-        return None, 0.0, f"oracle unavailable: {e}"
+        return None, 0.0, f"oracle unavailable: {e}", None
+    backend = _backend(O3)
     q, aux, _ = P.acoustics3d_pulse(n, seed=seed)
     p = O3.Problem3(n, n, n, mthbc=(3,) * 6, mthlim=(2, 2))
     q = O3.bc3(p, q)
     dt = 0.9 * p.geom.dx / 2.0
-    t0 = time.perf_counter()
-    O3.step3(p, q, aux, dt)
-    el = time.perf_counter() - t0
-    return n ** 3 / el / 1e9, el, f"{n}^3 acoustics3d, 1 unsplit step, 1 thread"
+    with _QuietStdout(O3):
+        t0 = time.perf_counter()
+        O3.step3(p, q, aux, dt, backend=backend)
+        el = time.perf_counter() - t0
+    note = REF_NOTE if backend == "ref" else PORT_NOTE
+    return (n ** 3 / el / 1e9, el, f"{n}^3 acoustics3d, 1 unsplit step3, 1 thread; {note}",
+            "reference" if backend == "ref" else "port")
 
 
 def bench3(args):
@@ -130,10 +174,10 @@ def bench3(args):
     achieved = cells * BYTES_PER_CELL_UPDATE3 / kavg_s / 1e9
     cpu = None
     if not args.no_cpu:
-        v, el, sample = cpu_baseline3(args.cpu_n3, args.seed)
+        v, el, sample, kind = cpu_baseline3(args.cpu_n3, args.seed)
         if v is not None:
-            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-                   "sample": f"{sample} ({el:.1f} s), oracle C restatement compiled -O2"}
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": kind,
+                   "sample": f"{sample} ({el:.1f} s)"}
     return {
         "metric": METRIC, "value": value, "unit": "Gcell-updates/s", "n_gpus": 1,
         "steps": steps, "warmup": args.warmup, "ms_per_step": wall / steps * 1e3,
@@ -165,7 +209,8 @@ def cpu_baseline_batch(specs, min_cells=16_000_000):
         from classic_amd import problems as P
         O.lib()
     except Exception as e:  # pragma: no cover
-        return None, 0.0, f"oracle unavailable: {e}"
+        return None, 0.0, f"oracle unavailable: {e}", None
+    backend = _backend(O)
     done, el, used = 0, 0.0, 0
     for s in specs:
         q, dx, dy = P.batch_problem_q(s)
@@ -175,15 +220,18 @@ def cpu_baseline_batch(specs, min_cells=16_000_000):
         else:
             p = O.Problem(3, 2, s["mx"], s["my"], O.SOLVER_ACOUSTICS, O.solver_params(*s["params"]),
                           method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4], mthbc=s["mthbc"])
-        t0 = time.perf_counter()
-        r = O.claw2(p, q, dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 1)
-        el += time.perf_counter() - t0
+        with _QuietStdout(O):
+            t0 = time.perf_counter()
+            r = O.claw2(p, q, dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 1, backend=backend)
+            el += time.perf_counter() - t0
         done += s["mx"] * s["my"] * int(r.nv[1])
         used += 1
         if done >= min_cells:
             break
-    return done / el / 1e9, el, (f"first {used} problems of the batch, 1 claw2 step each "
-                                 f"({done} cell updates), 1 thread")
+    note = REF_NOTE if backend == "ref" else PORT_NOTE
+    return (done / el / 1e9, el, (f"first {used} problems of the batch, 1 claw2 step each "
+                                  f"({done} cell updates), 1 thread; {note}"),
+            "reference" if backend == "ref" else "port")
 
 
 def bench_batch(args):
@@ -230,10 +278,10 @@ def bench_batch(args):
     achieved = bytes_iter / kavg_s / 1e9
     cpu = None
     if not args.no_cpu:
-        v, el, sample = cpu_baseline_batch(specs)
+        v, el, sample, kind = cpu_baseline_batch(specs)
         if v is not None:
-            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-                   "sample": f"{sample} ({el:.1f} s), oracle C restatement compiled -O2"}
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": kind,
+                   "sample": f"{sample} ({el:.1f} s)"}
     best = min(cands, key=lambda c: c[4])
     worst = max(cands, key=lambda c: c[4])
     return {
@@ -354,10 +402,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        v, el, sample = cpu_baseline(n, n, args.seed, args.cpu_steps)
+        v, el, sample, kind = cpu_baseline(n, n, args.seed, args.cpu_steps)
         if v is not None:
-            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-                   "sample": f"{sample} ({el:.1f} s), oracle C restatement compiled -O2"}
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": kind,
+                   "sample": f"{sample} ({el:.1f} s)"}
 
     rec = {
         "metric": METRIC,
