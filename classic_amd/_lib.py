@@ -81,6 +81,7 @@ _SIGS = {
     "clawgpu1_bc1": ([_V, _I], _i),
     "clawgpu1_step1": ([_V, _d, _d, _I, _I, _i, _D], _i),
     "clawgpu1_claw1": ([_V, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
+    "clawgpu1_claw1_hooks": ([_V, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
     "clawgpu3_create": ([C.POINTER(_V), _i, _i, _i, _i, _i, _i, _i, _i], _i),
     "clawgpu3_destroy": ([_V], _i),
     "clawgpu3_set_q": ([_V, _D], _i),
@@ -90,6 +91,7 @@ _SIGS = {
     "clawgpu3_step3": ([_V, _d, _d, _d, _d, _I, _I, _i, _D], _i),
     "clawgpu3_dimsp3": ([_V, _d, _d, _d, _d, _I, _I, _D, _i, _D], _i),
     "clawgpu3_claw3": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
+    "clawgpu3_claw3_hooks": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
     "clawgpu3_set_segment": ([_V, _i], _i),
     "clawgpu3_set_tile": ([_V, _i, _i], _i),
     "clawgpu3_set_aux_table": ([_V, _i, _I], _i),

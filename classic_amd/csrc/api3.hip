@@ -34,6 +34,7 @@ struct clawgpu3_ctx {
     int nmat = 0;                   // distinct (Z, c) pairs found by set_aux, 0 if too many
     int use_table = 1;              // read the material table instead of aux when nmat > 0
     int last_hip_error;
+    std::vector<double> haux;       // host copy of aux for the user routines
 };
 
 namespace {
@@ -448,6 +449,7 @@ extern "C" int clawgpu3_set_aux(clawgpu3_ctx* c, const double* aux)
 {
     if (!c || !aux) return set_err(CLAWGPU_ERR_ARG, "null argument");
     const long n = c->L.fs;
+    if (aux != c->haux.data()) c->haux.assign(aux, aux + (size_t)n * c->maux);
     HIPCHK(c, hipMemcpyAsync(c->staging, aux, (size_t)n * c->maux * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(aos_to_planes3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
@@ -517,15 +519,32 @@ extern "C" int clawgpu3_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz,
     return CLAWGPU_OK;
 }
 
-extern "C" int clawgpu3_claw3(clawgpu3_ctx* c, const double* geom, double tstart, double* tend,
-                              double* dtv, double* cflv, int* nv, const int* method,
-                              const int* mthlim, const int* mthbc, int use_fwave, int* info_out)
+// The claw3 loop (src/3d/claw3.f:735-904).  The Courant number of this
+// solver is dt/d * max|c|, known before the step, so an attempt that would be
+// rejected is not run (the reference runs it, then discards the result and
+// restores q).  With hooks, the user's bc3, b4step3 and src3 are called on a
+// host copy of q at the reference's points (claw3.f:766-885): bc3 and b4step3
+// before every attempt, src3 over dt/2 before the step (its input kept for a
+// retake) and over dt/2 after it (method(5) = 2), or over dt after it
+// (method(5) = 1).
+static int claw3_loop(clawgpu3_ctx* c, const double* geom, double tstart, double* tend,
+                      double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
+                      const int* mthbc, int use_fwave, const clawgpu3_hooks* h, int* info_out)
 {
     if (!c || !geom || !tend || !dtv || !cflv || !nv || !method || !mthlim || !mthbc || !info_out)
         return set_err(CLAWGPU_ERR_ARG, "null argument");
-    if (check_bc3(mthbc) != 0)
+    const bool user_bc = h && h->bc3;
+    if (!user_bc && check_bc3(mthbc) != 0)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides), or 0 on z-hi (piston)");
+    const double xl = geom[0], yl = geom[1], zl = geom[2];
     const double dx = geom[3], dy = geom[4], dz = geom[5];
+    std::vector<double> hq, saved;  // host copies of q for the hooks
+    double* haux = c->maux > 0 && !c->haux.empty() ? c->haux.data() : nullptr;
+    auto pull = [&]() -> int {
+        hq.resize((size_t)c->L.fs * c->meqn);
+        return clawgpu3_get_q(c, hq.data());
+    };
+    auto push = [&]() -> int { return clawgpu3_set_q(c, hq.data()); };
     int info = 0, rc;
     double t = tstart;
     int maxn = nv[0];
@@ -545,20 +564,39 @@ extern "C" int clawgpu3_claw3(clawgpu3_ctx* c, const double* geom, double tstart
     }
     if (method[0] == 1 && cflv[1] > cflv[0]) { info = 3; goto done; }
     if (method[5] > method[6]) { info = 5; goto done; }
-    if (!std::isfinite(c->cmax)) {
-        *info_out = -5;
-        return set_err(CLAWGPU_ERR_NONFINITE, "non-finite sound speed (Courant number)");
-    }
     if (maxn == 0) goto done;
     for (int n = 1; n <= maxn; n++) {
         double told = t;
         if (told + dt > *tend && tstart < *tend) dt = *tend - told;
         for (;;) {
-            rc = enqueue_bc3(c, c->dq[c->cur], geom, told, mthbc);
-            if (rc) return rc;
-            // The Courant number of this solver is dt/d * max|c| and known
-            // before the step: a step that would be rejected is not run
-            // (its result would be discarded and q restored).
+            if (user_bc) {
+                if ((rc = pull())) return rc;
+                h->bc3(h->user, c->meqn, c->mbc, c->mx, c->my, c->mz, xl, yl, zl, dx, dy, dz,
+                       hq.data(), c->maux, haux, told, dt, mthbc);
+                if ((rc = push())) return rc;
+            } else {
+                rc = enqueue_bc3(c, c->dq[c->cur], geom, told, mthbc);
+                if (rc) return rc;
+            }
+            if (h && h->b4step3) {
+                if ((rc = pull())) return rc;
+                h->b4step3(h->user, c->mbc, c->mx, c->my, c->mz, c->meqn, hq.data(), xl, yl, zl,
+                           dx, dy, dz, told, dt, c->maux, haux);
+                if ((rc = push())) return rc;
+                if (haux && (rc = clawgpu3_set_aux(c, c->haux.data()))) return rc;
+            }
+            const bool strang_src = method[4] == 2 && h && h->src3;
+            if (strang_src) {
+                if ((rc = pull())) return rc;
+                saved = hq;
+                h->src3(h->user, c->meqn, c->mbc, c->mx, c->my, c->mz, xl, yl, zl, dx, dy, dz,
+                        hq.data(), c->maux, haux, told, dt / 2.0);
+                if ((rc = push())) return rc;
+            }
+            if (!std::isfinite(c->cmax)) {
+                *info_out = -5;
+                return set_err(CLAWGPU_ERR_NONFINITE, "non-finite sound speed (Courant number)");
+            }
             if (method[2] >= 0) cfl = unsplit_cfl(c, dx, dy, dz, dt);
             else {
                 const double cx = dt / dx * c->cmax, cy = dt / dy * c->cmax, cz = dt / dz * c->cmax;
@@ -578,12 +616,22 @@ extern "C" int clawgpu3_claw3(clawgpu3_ctx* c, const double* geom, double tstart
                 break;
             }
             t = told;
+            if (strang_src) {  // back to the state before the source half step
+                hq = saved;
+                if ((rc = push())) return rc;
+            }
             if (method[0] == 1) {
                 dt = fmin(dtv[1], dt * cflv[1] / cfl);
                 continue;
             }
             cflmax = fmax(cfl, cflmax);
             goto done;
+        }
+        if (h && h->src3 && (method[4] == 1 || method[4] == 2)) {
+            if ((rc = pull())) return rc;
+            h->src3(h->user, c->meqn, c->mbc, c->mx, c->my, c->mz, xl, yl, zl, dx, dy, dz,
+                    hq.data(), c->maux, haux, t, method[4] == 2 ? dt / 2.0 : dt);
+            if ((rc = push())) return rc;
         }
         if (method[0] == 1) {
             if (cfl == 0.0) dt = dtv[1];
@@ -606,6 +654,26 @@ done:
     dtv[4] = dt;
     *info_out = info;
     return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_claw3(clawgpu3_ctx* c, const double* geom, double tstart, double* tend,
+                              double* dtv, double* cflv, int* nv, const int* method,
+                              const int* mthlim, const int* mthbc, int use_fwave, int* info_out)
+{
+    return claw3_loop(c, geom, tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, use_fwave,
+                      nullptr, info_out);
+}
+
+extern "C" int clawgpu3_claw3_hooks(clawgpu3_ctx* c, const double* geom, double tstart,
+                                    double* tend, double* dtv, double* cflv, int* nv,
+                                    const int* method, const int* mthlim, const int* mthbc,
+                                    int use_fwave, const clawgpu3_hooks* hooks, int* info_out)
+{
+    if (!hooks) return set_err(CLAWGPU_ERR_ARG, "null hooks (use clawgpu3_claw3)");
+    if (c && c->maux > 0 && c->haux.empty() && (hooks->b4step3 || hooks->src3 || hooks->bc3))
+        return set_err(CLAWGPU_ERR_ARG, "aux has not been set (clawgpu3_set_aux)");
+    return claw3_loop(c, geom, tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, use_fwave,
+                      hooks, info_out);
 }
 
 extern "C" int clawgpu3_set_segment(clawgpu3_ctx* c, int cells)

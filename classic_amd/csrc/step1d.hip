@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "ctx.hpp"
 #include "lanes.hpp"
@@ -229,6 +230,7 @@ struct clawgpu1_ctx {
     unsigned long long* dcfl;
     hipStream_t stream;
     int last_hip_error;
+    std::vector<double> haux;  // host copy of aux for the user routines
 };
 
 namespace {
@@ -393,6 +395,7 @@ extern "C" int clawgpu1_set_aux(clawgpu1_ctx* c, const double* aux)
     if (!c || !aux) return set_err(CLAWGPU_ERR_ARG, "null argument");
     if (c->maux == 0) return set_err(CLAWGPU_ERR_ARG, "context has maux = 0");
     const size_t n = (size_t)(c->mx + 2 * c->mbc) * c->maux;
+    if (aux != c->haux.data()) c->haux.assign(aux, aux + n);
     HIPCHK(c, hipMemcpyAsync(c->daux, aux, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CLAWGPU_OK;
@@ -422,14 +425,32 @@ extern "C" int clawgpu1_step1(clawgpu1_ctx* c, double dx, double dt, const int* 
     return CLAWGPU_OK;
 }
 
-extern "C" int clawgpu1_claw1(clawgpu1_ctx* c, double dx, double tstart, double* tend,
-                              double* dtv, double* cflv, int* nv, const int* method,
-                              const int* mthlim, const int* mthbc, int use_fwave, int* info_out)
+// The claw1 time loop (src/1d/claw1.f:448-560).  With hooks, the user's
+// bc1, b4step1 and src1 are called on a host copy of q where claw1 calls
+// them (claw1.f:478-510): bc1 and b4step1 before every attempt, src1 over
+// dt/2 before the step and over dt/2 after it (method(5) = 2) or over dt
+// after it (method(5) = 1).  The post-step source comes before the Courant
+// check, so a rejected attempt calls it too; a rejection then restores q as
+// it was at the start of the step (claw1.f's copyq1 of qwork).
+static int claw1_loop(clawgpu1_ctx* c, double dx, double xlower, double tstart, double* tend,
+                      double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
+                      const int* mthbc, int use_fwave, const clawgpu1_hooks* h, int* info_out)
 {
     if (!c || !tend || !dtv || !cflv || !nv || !method || !mthlim || !mthbc || !info_out)
         return set_err(CLAWGPU_ERR_ARG, "null argument");
     int rc = check1(c, method, mthlim);
     if (rc) return rc;
+    const bool user_bc = h && h->bc1;
+    const bool hooked = h && (h->bc1 || h->b4step1 || h->src1);
+    const bool src = h && h->src1 && (method[4] == 1 || method[4] == 2);
+    std::vector<double> hq, qwork;  // host copies of q for the hooks
+    double* haux = c->maux > 0 && !c->haux.empty() ? c->haux.data() : nullptr;
+    const size_t nq = (size_t)(c->mx + 2 * c->mbc) * c->meqn;
+    auto pull = [&]() -> int {
+        hq.resize(nq);
+        return clawgpu1_get_q(c, hq.data());
+    };
+    auto push = [&]() -> int { return clawgpu1_set_q(c, hq.data()); };
     int info = 0;
     double t = tstart;
     int maxn = nv[0];
@@ -449,10 +470,36 @@ extern "C" int clawgpu1_claw1(clawgpu1_ctx* c, double dx, double tstart, double*
     for (int n = 1; n <= maxn; n++) {
         const double told = t;
         if (told + dt > *tend && tstart < *tend) dt = *tend - told;
+        if (hooked) {  // claw1.f's qwork, restored on a rejection
+            if ((rc = pull())) return rc;
+            qwork = hq;
+        }
         for (;;) {  // claw1.f label 40
+            const double dt2 = dt / 2.0;
+            const double thalf = told + dt2;
             t = told + dt;
-            rc = enqueue_bc1(c, mthbc);
-            if (rc) return rc;
+            if (user_bc) {
+                if ((rc = pull())) return rc;
+                h->bc1(h->user, c->meqn, c->mbc, c->mx, xlower, dx, hq.data(), c->maux, haux, told,
+                       dt, mthbc);
+                if ((rc = push())) return rc;
+            } else {
+                rc = enqueue_bc1(c, mthbc);
+                if (rc) return rc;
+            }
+            if (h && h->b4step1) {
+                if ((rc = pull())) return rc;
+                h->b4step1(h->user, c->mbc, c->mx, c->meqn, hq.data(), xlower, dx, told, dt,
+                           c->maux, haux);
+                if ((rc = push())) return rc;
+                if (haux && (rc = clawgpu1_set_aux(c, c->haux.data()))) return rc;
+            }
+            if (src && method[4] == 2) {
+                if ((rc = pull())) return rc;
+                h->src1(h->user, c->meqn, c->mbc, c->mx, xlower, dx, hq.data(), c->maux, haux, told,
+                        dt2);
+                if ((rc = push())) return rc;
+            }
             rc = enqueue_step1(c, dx, dt, method, mthlim, use_fwave);
             if (rc) return rc;
             rc = read_cfl1(c, &cfl);
@@ -460,6 +507,13 @@ extern "C" int clawgpu1_claw1(clawgpu1_ctx* c, double dx, double tstart, double*
             if (!std::isfinite(cfl)) {
                 *info_out = -5;
                 return set_err(CLAWGPU_ERR_NONFINITE, "non-finite wave speed (Courant number)");
+            }
+            c->cur = 1 - c->cur;  // the new buffer holds the step's result
+            if (src) {
+                if ((rc = pull())) return rc;
+                h->src1(h->user, c->meqn, c->mbc, c->mx, xlower, dx, hq.data(), c->maux, haux,
+                        method[4] == 2 ? thalf : t, method[4] == 2 ? dt2 : dt);
+                if ((rc = push())) return rc;
             }
             // the new dt is chosen before the Courant check (claw1.f)
             if (method[0] == 1) {
@@ -473,10 +527,16 @@ extern "C" int clawgpu1_claw1(clawgpu1_ctx* c, double dx, double tstart, double*
             }
             if (cfl <= cflv[0]) {
                 cflmax = fmax(cfl, cflmax);
-                c->cur = 1 - c->cur;  // accept the new buffer
                 break;
             }
-            t = told;  // reject: the old buffer still holds q (with its ghost cells)
+            // reject: back to the old buffer, which still holds q unless a
+            // user routine changed it
+            t = told;
+            c->cur = 1 - c->cur;
+            if (hooked) {
+                hq = qwork;
+                if ((rc = push())) return rc;
+            }
             if (method[0] == 1) continue;
             cflmax = fmax(cfl, cflmax);
             goto done;
@@ -495,4 +555,24 @@ done:
     dtv[4] = dt;
     *info_out = info;
     return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu1_claw1(clawgpu1_ctx* c, double dx, double tstart, double* tend,
+                              double* dtv, double* cflv, int* nv, const int* method,
+                              const int* mthlim, const int* mthbc, int use_fwave, int* info_out)
+{
+    return claw1_loop(c, dx, 0.0, tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, use_fwave,
+                      nullptr, info_out);
+}
+
+extern "C" int clawgpu1_claw1_hooks(clawgpu1_ctx* c, double dx, double xlower, double tstart,
+                                    double* tend, double* dtv, double* cflv, int* nv,
+                                    const int* method, const int* mthlim, const int* mthbc,
+                                    int use_fwave, const clawgpu1_hooks* hooks, int* info_out)
+{
+    if (!hooks) return set_err(CLAWGPU_ERR_ARG, "null hooks (use clawgpu1_claw1)");
+    if (c && c->maux > 0 && c->haux.empty() && (hooks->b4step1 || hooks->src1 || hooks->bc1))
+        return set_err(CLAWGPU_ERR_ARG, "aux has not been set (clawgpu1_set_aux)");
+    return claw1_loop(c, dx, xlower, tstart, tend, dtv, cflv, nv, method, mthlim, mthbc,
+                      use_fwave, hooks, info_out);
 }

@@ -100,35 +100,48 @@ class _Stepper:
 
     def claw(self, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc, use_fwave):
         """One call of the claw time loop; returns (tend, info, dtv, cflv, nv)."""
+        kw = dict(getattr(self, "hooks", {}))
         if self.ndim == 2:
-            kw = dict(getattr(self, "hooks", {}))
             if kw:
                 kw.update(xlower=self.c.lower[0], ylower=self.c.lower[1])
             r = self.g.claw2(self.delta[0], self.delta[1], tstart, tend, dtv, cflv, nmax, method,
                              mthlim, mthbc, use_fwave, **kw)
             return r["tend"], r["info"], r["dtv"], r["cflv"], r["nv"]
         if self.ndim == 1:
+            if kw:
+                kw.update(xlower=self.c.lower[0])
             r = self.g.claw1(self.delta[0], tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc,
-                             use_fwave)
+                             use_fwave, **kw)
             return r["tend"], r["info"], r["dtv"], r["cflv"], r["nv"]
-        r = self.g.claw3(tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc, use_fwave)
+        r = self.g.claw3(tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc, use_fwave, **kw)
         return r.tend, r.info, r.dtv, r.cflv, r.nv
 
 
+ROUTINES = {1: ("bc1", "b4step1", "src1"), 2: ("bc2", "b4step2", "src2"),
+            3: ("bc3", "b4step3", "src3")}
+
+
 def run(rundir=".", outdir=None, qinit=None, setaux=None, setprob=None, solver="acoustics",
-        solver_params=None, arith=0, grid=None, verbose=False, bc2=None, b4step2=None, src2=None):
+        solver_params=None, arith=0, grid=None, verbose=False, **routines):
     """Run the claw*ez driver for the claw.data in rundir; frames go to
     outdir (default: rundir).  Returns a RunResult.  ``grid`` may supply an
-    already constructed stepping object with the Grid*D interface.  In 2D,
-    bc2 / b4step2 / src2 are the user routines (see gpu.make_hooks); bc2 is
-    required when a boundary code is 0 (user boundary)."""
+    already constructed stepping object with the Grid*D interface.  The
+    keyword arguments bc<d> / b4step<d> / src<d> (d = the dimension) are the
+    user routines (see gpu.make_hooks, gpu1.make_hooks1, gpu3.make_hooks3);
+    bc<d> is required when a boundary code is 0 (user boundary), except for
+    the 3D piston wall (mthbc(6) = 0)."""
     c = read_claw_data(os.path.join(rundir, "claw.data"))
-    hooks = {k: v for k, v in (("bc2", bc2), ("b4step2", b4step2), ("src2", src2)) if v is not None}
-    if hooks and c.ndim != 2:
-        raise ValueError("user routines are supported in 2D only")
-    if 0 in c.mthbc and "bc2" not in hooks and not (c.ndim == 3 and c.mthbc == [1, 1, 1, 1, 1, 0]):
-        raise ValueError("a boundary code 0 (user boundary) needs the bc2 routine")
     nd = c.ndim
+    known = {k for names in ROUTINES.values() for k in names}
+    unknown = sorted(set(routines) - known)
+    if unknown:
+        raise TypeError(f"unexpected keyword arguments {unknown}")
+    hooks = {k: v for k, v in routines.items() if v is not None}
+    if any(k not in ROUTINES[nd] for k in hooks):
+        raise ValueError(f"the user routines of a {nd}D run are {', '.join(ROUTINES[nd])}")
+    bc = ROUTINES[nd][0]
+    if 0 in c.mthbc and bc not in hooks and not (nd == 3 and c.mthbc == [1, 1, 1, 1, 1, 0]):
+        raise ValueError(f"a boundary code 0 (user boundary) needs the {bc} routine")
     if nd == 1:
         c["restart"] = False  # claw1ez reads no restart record
     outdir = outdir or rundir

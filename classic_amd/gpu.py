@@ -60,6 +60,17 @@ def _view(ptr, my, mx, m):
     return np.ctypeslib.as_array(ptr, shape=(my, mx, m))
 
 
+def _guarded(fn, errors):
+    """fn wrapped for use as a C callback: an exception cannot propagate
+    through C, so it is appended to errors and re-raised by the caller."""
+    def run(*a):
+        try:
+            fn(*a)
+        except BaseException as e:
+            errors.append(e)
+    return run
+
+
 def make_hooks(bc2=None, b4step2=None, src2=None):
     """C hook table for clawgpu_claw2_hooks from Python callables, each
     called with numpy views of the host copies of q and aux (reference
@@ -72,12 +83,7 @@ def make_hooks(bc2=None, b4step2=None, src2=None):
     keep = []
 
     def guard(fn):
-        def run(*a):
-            try:
-                fn(*a)
-            except BaseException as e:  # a callback cannot raise through C
-                errors.append(e)
-        return run
+        return _guarded(fn, errors)
 
     if bc2 is not None:
         def _bc(user, meqn, mbc, mx, my, xl, yl, dx, dy, q, maux, aux, t, dt, mthbc):

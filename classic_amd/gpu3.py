@@ -14,7 +14,54 @@ import numpy as np
 
 from . import _lib
 from ._lib import SOLVER_VC_ACOUSTICS, check
-from .gpu import _dp, _ip, _ivec
+from .gpu import _dp, _guarded, _ip, _ivec
+
+_DBL = C.c_double
+_PD = C.POINTER(_DBL)
+_I = C.c_int
+_BC3_FN = C.CFUNCTYPE(None, C.c_void_p, _I, _I, _I, _I, _I, _DBL, _DBL, _DBL, _DBL, _DBL, _DBL, _PD,
+                      _I, _PD, _DBL, _DBL, C.POINTER(_I))
+_B4STEP3_FN = C.CFUNCTYPE(None, C.c_void_p, _I, _I, _I, _I, _I, _PD, _DBL, _DBL, _DBL, _DBL, _DBL,
+                          _DBL, _DBL, _DBL, _I, _PD)
+_SRC3_FN = C.CFUNCTYPE(None, C.c_void_p, _I, _I, _I, _I, _I, _DBL, _DBL, _DBL, _DBL, _DBL, _DBL,
+                       _PD, _I, _PD, _DBL, _DBL)
+
+
+class _Hooks3(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("bc3", _BC3_FN), ("b4step3", _B4STEP3_FN),
+                ("src3", _SRC3_FN)]
+
+
+def _view3(ptr, mbc, mx, my, mz, m):
+    if not ptr or m == 0:
+        return None
+    return np.ctypeslib.as_array(ptr, shape=(mz + 2 * mbc, my + 2 * mbc, mx + 2 * mbc, m))
+
+
+def make_hooks3(bc3=None, b4step3=None, src3=None):
+    """C hook table for clawgpu3_claw3_hooks from Python callables, called
+    with numpy views (shape (mz+2*mbc, my+2*mbc, mx+2*mbc, m)) of the host
+    copies of q and aux to modify in place: bc3(q, aux, t, dt, mthbc),
+    b4step3(q, aux, t, dt), src3(q, aux, t, dt).  Returns (table, keep,
+    errors) as gpu.make_hooks."""
+    errors, keep, h = [], [], _Hooks3()
+    if bc3 is not None:
+        def _bc(user, meqn, mbc, mx, my, mz, xl, yl, zl, dx, dy, dz, q, maux, aux, t, dt, mthbc):
+            bc3(_view3(q, mbc, mx, my, mz, meqn), _view3(aux, mbc, mx, my, mz, maux), t, dt,
+                tuple(mthbc[k] for k in range(6)))
+        keep.append(_BC3_FN(_guarded(_bc, errors)))
+        h.bc3 = keep[-1]
+    if b4step3 is not None:
+        def _b4(user, mbc, mx, my, mz, meqn, q, xl, yl, zl, dx, dy, dz, t, dt, maux, aux):
+            b4step3(_view3(q, mbc, mx, my, mz, meqn), _view3(aux, mbc, mx, my, mz, maux), t, dt)
+        keep.append(_B4STEP3_FN(_guarded(_b4, errors)))
+        h.b4step3 = keep[-1]
+    if src3 is not None:
+        def _src(user, meqn, mbc, mx, my, mz, xl, yl, zl, dx, dy, dz, q, maux, aux, t, dt):
+            src3(_view3(q, mbc, mx, my, mz, meqn), _view3(aux, mbc, mx, my, mz, maux), t, dt)
+        keep.append(_SRC3_FN(_guarded(_src, errors)))
+        h.src3 = keep[-1]
+    return h, keep, errors
 
 
 class Claw3Result:
@@ -101,8 +148,9 @@ class Grid3D:
         return cfl.value
 
     def claw3(self, tstart, tend, dtv, cflv, nmax, method=(1, 2, 22, 0, 0, 0, 2), mthlim=(3, 3),
-              mthbc=(1, 1, 1, 1, 1, 1), use_fwave=False):
-        """The claw3 time loop on the device; returns Claw3Result."""
+              mthbc=(1, 1, 1, 1, 1, 1), use_fwave=False, bc3=None, b4step3=None, src3=None):
+        """The claw3 time loop on the device; returns Claw3Result.
+        bc3 / b4step3 / src3: optional user routines (see make_hooks3)."""
         te = C.c_double(tend)
         dtv_a = np.zeros(5)
         dtv_a[: len(dtv)] = dtv
@@ -110,10 +158,17 @@ class Grid3D:
         cflv_a[: len(cflv)] = cflv
         nv = np.array([nmax, 0], dtype=np.int32)
         info = C.c_int(0)
-        check(self.lib.clawgpu3_claw3(self.h, _dp(self.geom), tstart, C.byref(te), _dp(dtv_a),
-                                      _dp(cflv_a), _ip(nv), _ip(_ivec(method, 7)),
-                                      _ip(_ivec(mthlim, 2)), _ip(_ivec(mthbc, 6)), int(use_fwave),
-                                      C.byref(info)))
+        args = (_dp(self.geom), tstart, C.byref(te), _dp(dtv_a), _dp(cflv_a), _ip(nv),
+                _ip(_ivec(method, 7)), _ip(_ivec(mthlim, 2)), _ip(_ivec(mthbc, 6)), int(use_fwave))
+        if bc3 is None and b4step3 is None and src3 is None:
+            check(self.lib.clawgpu3_claw3(self.h, *args, C.byref(info)))
+        else:
+            hooks, keep, errors = make_hooks3(bc3, b4step3, src3)
+            rc = self.lib.clawgpu3_claw3_hooks(self.h, *args, C.byref(hooks), C.byref(info))
+            del keep
+            if errors:
+                raise errors[0]
+            check(rc)
         return Claw3Result(info.value, te.value, dtv_a, cflv_a, nv)
 
     def frames(self, t0, tfinal, nout, dt_initial, cflv=(1.0, 0.9), dt_max=1e9, nmax=50000,

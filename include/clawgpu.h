@@ -307,6 +307,32 @@ int clawgpu3_claw3(clawgpu3_ctx* ctx, const double* geom, double tstart, double*
                    double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
                    const int* mthbc, int use_fwave, int* info);
 
+/* User routines of the 3D plug-in contract (src/3d/claw3.f:766-885: bc3,
+ * b4step3, src3), with the conventions of clawgpu_hooks; a NULL bc3 takes the
+ * built-in bc3 (codes 1-3, the piston with mthbc(6) = 0). */
+typedef struct clawgpu3_hooks {
+    void* user;
+    void (*bc3)(void* user, int meqn, int mbc, int mx, int my, int mz, double xlower,
+                double ylower, double zlower, double dx, double dy, double dz, double* q, int maux,
+                double* aux, double t, double dt, const int* mthbc);
+    void (*b4step3)(void* user, int mbc, int mx, int my, int mz, int meqn, double* q,
+                    double xlower, double ylower, double zlower, double dx, double dy, double dz,
+                    double t, double dt, int maux, double* aux);
+    void (*src3)(void* user, int meqn, int mbc, int mx, int my, int mz, double xlower,
+                 double ylower, double zlower, double dx, double dy, double dz, double* q,
+                 int maux, double* aux, double t, double dt);
+} clawgpu3_hooks;
+
+/* clawgpu3_claw3 with user routines called where claw3 calls them
+ * (claw3.f:766-885): bc3 then b4step3 before every attempt; with
+ * method(5) = 2 src3 over dt/2 before the step (its input restored on a
+ * rejection) and over dt/2 at t + dt after an accepted step, with
+ * method(5) = 1 src3 over dt after it.  aux changed by b4step3 is copied back
+ * (and its sound speeds and material table recomputed). */
+int clawgpu3_claw3_hooks(clawgpu3_ctx* ctx, const double* geom, double tstart, double* tend,
+                         double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
+                         const int* mthbc, int use_fwave, const clawgpu3_hooks* hooks, int* info);
+
 /* Cells per segment of the y/z split sweeps (tuning; default 64). */
 int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
 
@@ -362,6 +388,32 @@ int clawgpu1_step1(clawgpu1_ctx* ctx, double dx, double dt, const int* method, c
 int clawgpu1_claw1(clawgpu1_ctx* ctx, double dx, double tstart, double* tend, double* dtv,
                    double* cflv, int* nv, const int* method, const int* mthlim, const int* mthbc,
                    int use_fwave, int* info);
+
+/* User routines of the 1D plug-in contract (src/1d/claw1.f:386, called at
+ * claw1.f:478-510: bc1, b4step1, src1), with the conventions of
+ * clawgpu_hooks: user pointer first, the reference's arguments by value,
+ * arrays in the reference layout with ghost cells; NULL members take the
+ * built-in bc1 (codes 1-3) or do nothing. */
+typedef struct clawgpu1_hooks {
+    void* user;
+    void (*bc1)(void* user, int meqn, int mbc, int mx, double xlower, double dx, double* q,
+                int maux, double* aux, double t, double dt, const int* mthbc);
+    void (*b4step1)(void* user, int mbc, int mx, int meqn, double* q, double xlower, double dx,
+                    double t, double dt, int maux, double* aux);
+    void (*src1)(void* user, int meqn, int mbc, int mx, double xlower, double dx, double* q,
+                 int maux, double* aux, double t, double dt);
+} clawgpu1_hooks;
+
+/* clawgpu1_claw1 with user routines called where claw1 calls them
+ * (claw1.f:478-510): bc1 then b4step1 before every attempt; with
+ * method(5) = 2 src1 over dt/2 at t before the step and over dt/2 at
+ * t + dt/2 after it, with method(5) = 1 src1 over dt at t + dt after it.
+ * The post-step source precedes the Courant check, so a rejected attempt
+ * calls it too, and a rejection restores q as it was when the step began
+ * (claw1.f:459 and 539).  aux changed by b4step1 is copied back. */
+int clawgpu1_claw1_hooks(clawgpu1_ctx* ctx, double dx, double xlower, double tstart, double* tend,
+                         double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
+                         const int* mthbc, int use_fwave, const clawgpu1_hooks* hooks, int* info);
 
 #ifdef __cplusplus
 }

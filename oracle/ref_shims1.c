@@ -92,3 +92,66 @@ int ref1_bc1(int meqn, int mbc, int mx, double* q, int* mthbc)
     bc1_(&meqn, &mbc, &mx, &z, &z, q, &maux, dummy_aux, &z, &z, mthbc);
     return 0;
 }
+
+/* Test user routines for the plug-in contract: b4step1 and src1 append
+ * (kind, t, dt) to a log (kind 1 = b4step1, 2 = src1); src1 applies the decay
+ * q(1) <- q(1) - (k dt) q(1) on the interior cells.  ref1_claw1_hooks runs
+ * the reference claw1 with them. */
+static double g_hook1_k = 0.0;
+static double* g_hook1_log = 0;
+static int g_hook1_n = 0, g_hook1_cap = 0;
+
+void ref1_hooks_set(double k, double* log, int cap)
+{
+    g_hook1_k = k;
+    g_hook1_log = log;
+    g_hook1_cap = cap;
+    g_hook1_n = 0;
+}
+
+int ref1_hooks_count(void) { return g_hook1_n; }
+
+static void hook1_log(double kind, double t, double dt)
+{
+    if (g_hook1_n < g_hook1_cap) {
+        g_hook1_log[3 * g_hook1_n] = kind;
+        g_hook1_log[3 * g_hook1_n + 1] = t;
+        g_hook1_log[3 * g_hook1_n + 2] = dt;
+    }
+    g_hook1_n++;
+}
+
+static void test_b4step1(int* mbc, int* mx, int* meqn, double* q, double* xlower, double* dx,
+                         double* t, double* dt, int* maux, double* aux)
+{
+    (void)mbc; (void)mx; (void)meqn; (void)q; (void)xlower; (void)dx; (void)maux; (void)aux;
+    hook1_log(1.0, *t, *dt);
+}
+
+static void test_src1(int* meqn, int* mbc, int* mx, double* xlower, double* dx, double* q,
+                      int* maux, double* aux, double* t, double* dt)
+{
+    (void)xlower; (void)dx; (void)maux; (void)aux;
+    hook1_log(2.0, *t, *dt);
+    const double kd = g_hook1_k * *dt;
+    for (int i = 1; i <= *mx; i++) {
+        double* v = &q[(long)(i + *mbc - 1) * *meqn];
+        v[0] = v[0] - kd * v[0];
+    }
+}
+
+int ref1_claw1_hooks(int meqn, int mwaves, int maux, int mbc, int mx, double* q, double* aux,
+                     double xlower, double dx, double tstart, double* tend, double* dtv,
+                     double* cflv, int* nv, int* method, int* mthlim, int* mthbc, int use_fwave)
+{
+    int mwork = (mx + 2 * mbc) * (2 + 4 * meqn + mwaves + meqn * mwaves);
+    double* work = (double*)calloc((size_t)mwork, sizeof(double));
+    double dummy_aux[1] = {0.0};
+    int info = 0, fw = use_fwave ? 1 : 0;
+    if (!work) return -3;
+    claw1_(&meqn, &mwaves, &maux, &mbc, &mx, q, maux > 0 ? aux : dummy_aux, &xlower, &dx,
+           &tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, work, &mwork, &fw, &info,
+           (void*)bc1_, (void*)f_rp1, (void*)test_src1, (void*)test_b4step1);
+    free(work);
+    return info;
+}

@@ -191,3 +191,78 @@ int ref3_bc3(int meqn, int mbc, int mx, int my, int mz, const orc3_geom* g, doub
              &dt, mthbc);
     return 0;
 }
+
+/* Test user routines for the plug-in contract: b4step3 and src3 append
+ * (kind, t, dt) to a log (kind 1 = b4step3, 2 = src3); src3 applies the decay
+ * q(1) <- q(1) - (k dt) q(1) on the interior cells.  ref3_claw3_hooks runs
+ * the reference claw3 with them. */
+static double g_hook3_k = 0.0;
+static double* g_hook3_log = 0;
+static int g_hook3_n = 0, g_hook3_cap = 0;
+
+void ref3_hooks_set(double k, double* log, int cap)
+{
+    g_hook3_k = k;
+    g_hook3_log = log;
+    g_hook3_cap = cap;
+    g_hook3_n = 0;
+}
+
+int ref3_hooks_count(void) { return g_hook3_n; }
+
+static void hook3_log(double kind, double t, double dt)
+{
+    if (g_hook3_n < g_hook3_cap) {
+        g_hook3_log[3 * g_hook3_n] = kind;
+        g_hook3_log[3 * g_hook3_n + 1] = t;
+        g_hook3_log[3 * g_hook3_n + 2] = dt;
+    }
+    g_hook3_n++;
+}
+
+static void test_b4step3(int* mbc, int* mx, int* my, int* mz, int* meqn, double* q,
+                         double* xlower, double* ylower, double* zlower, double* dx, double* dy,
+                         double* dz, double* t, double* dt, int* maux, double* aux)
+{
+    (void)mbc; (void)mx; (void)my; (void)mz; (void)meqn; (void)q; (void)xlower; (void)ylower;
+    (void)zlower; (void)dx; (void)dy; (void)dz; (void)maux; (void)aux;
+    hook3_log(1.0, *t, *dt);
+}
+
+static void test_src3(int* meqn, int* mbc, int* mx, int* my, int* mz, double* xlower,
+                      double* ylower, double* zlower, double* dx, double* dy, double* dz,
+                      double* q, int* maux, double* aux, double* t, double* dt)
+{
+    (void)xlower; (void)ylower; (void)zlower; (void)dx; (void)dy; (void)dz; (void)maux;
+    (void)aux;
+    hook3_log(2.0, *t, *dt);
+    const double kd = g_hook3_k * *dt;
+    const long nx = *mx + 2 * *mbc, ny = *my + 2 * *mbc;
+    for (int k = 1; k <= *mz; k++)
+        for (int j = 1; j <= *my; j++)
+            for (int i = 1; i <= *mx; i++) {
+                const long cell = ((long)(k + *mbc - 1) * ny + (j + *mbc - 1)) * nx + (i + *mbc - 1);
+                double* v = &q[cell * *meqn];
+                v[0] = v[0] - kd * v[0];
+            }
+}
+
+int ref3_claw3_hooks(int meqn, int mwaves, int maux, int mbc, int mx, int my, int mz, double* q,
+                     double* aux, const orc3_geom* g, double tstart, double* tend, double* dtv,
+                     double* cflv, int* nv, int* method, int* mthlim, int* mthbc, int use_fwave)
+{
+    int narray = method[4] < 2 ? 1 : 2;
+    int maxm = maxm3(mx, my, mz);
+    int mwork = (maxm + 2 * mbc) * (46 * meqn + mwaves + meqn * mwaves + 9 * maux + 3) +
+                narray * (mx + 2 * mbc) * (my + 2 * mbc) * (mz + 2 * mbc) * meqn;
+    double* work = (double*)calloc((size_t)mwork, sizeof(double));
+    double xl = g->xlower, yl = g->ylower, zl = g->zlower, dx = g->dx, dy = g->dy, dz = g->dz;
+    int info = 0, fw = use_fwave ? 1 : 0;
+    if (!work) return -3;
+    claw3_(&meqn, &mwaves, &mbc, &mx, &my, &mz, &maux, q, aux, &xl, &yl, &zl, &dx, &dy, &dz,
+           &tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, work, &mwork, &fw, &info,
+           (void*)bc3_, (void*)f_rpn3, (void*)f_rpt3, (void*)f_rptt3, (void*)test_src3,
+           (void*)test_b4step3);
+    free(work);
+    return info;
+}

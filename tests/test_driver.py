@@ -276,3 +276,21 @@ def test_driver13_matches_committed_reference_outputs(tmp_path, name):
     assert sorted(fg) == sorted(fm)
     for f in fg:
         assert fg[f] == fm[f], f
+
+
+def test_driver_user_routine_names(tmp_path):
+    """The driver takes bc<d> / b4step<d> / src<d> for the run's dimension
+    only, and a user boundary code (0) needs bc<d>."""
+    name = E.GOLDEN1[0] if E.GOLDEN1 else list(E.CASES1)[0]
+    rd, q0, solver, params = E.CASES1[name]()
+    rd.write(str(tmp_path))
+    grid = O1.OracleGrid1D(O1.SOLVER_ACOUSTICS, rd.clawdata.num_cells[0], O1.params1(*params))
+    kw = dict(qinit=E.qinit_from(q0), solver=solver, grid=grid)
+    with pytest.raises(ValueError, match="bc1, b4step1, src1"):
+        driver.claw1ez(str(tmp_path), src2=lambda *a: None, **kw)
+    with pytest.raises(TypeError, match="src4"):
+        driver.claw1ez(str(tmp_path), src4=lambda *a: None, **kw)
+    rd.clawdata.bc_lower = [0]
+    rd.write(str(tmp_path))
+    with pytest.raises(ValueError, match="needs the bc1 routine"):
+        driver.claw1ez(str(tmp_path), **kw)

@@ -1,17 +1,21 @@
-"""User routines of the plug-in contract (bc2, b4step2, src2) on the device
-claw2 loop (clawgpu_claw2_hooks) against the reference claw2 run with the
-same routines (tests/golden/claw2_hooks_*.npz, written by the reference
+"""User routines of the plug-in contract (bc, b4step, src) on the device
+claw1 / claw2 / claw3 loops (clawgpu1_claw1_hooks, clawgpu_claw2_hooks,
+clawgpu3_claw3_hooks) against the reference loops run with the same
+routines (tests/golden/claw{1,2,3}_hooks_*.npz, written by the reference
 build through tests/golden/make_golden_hooks.py): the same calls in the same
 order with the same (t, dt) — retakes and Strang half steps included — and
-the same final state bit for bit (EXACT arithmetic)."""
+the same final state, bit for bit in 1D and 2D (EXACT arithmetic) and for the
+split 3D step, to 1e-12 for the unsplit 3D step."""
 import os
 
 import numpy as np
 import pytest
 
-from classic_amd import SOLVER_ACOUSTICS, SOLVER_EULER, gpu
+from classic_amd import SOLVER_ACOUSTICS, SOLVER_EULER, SOLVER_VC_ACOUSTICS, gpu
+from classic_amd.gpu1 import Grid1D
+from classic_amd.gpu3 import Grid3D
 from tests import hooks_cases as H
-from tests.helpers import interior
+from tests.helpers import interior, max_rel_err3
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -79,3 +83,97 @@ def test_hook_errors_surface():
         with pytest.raises(ValueError, match="boom"):
             g.claw2(dx, dy, 0.0, 0.01, [1e-3, 1e99], [1.0, 0.9], 5, (1, 2, 2, 0, 1, 0, 0), [4, 4],
                     (3, 3, 3, 3), src2=bad)
+
+
+@pytest.mark.parametrize("name", list(H.CASES1))
+def test_hooks_match_reference_claw1(name):
+    c = H.CASES1[name]
+    g0 = np.load(os.path.join(GOLD, f"claw1_hooks_{name}.npz"))
+    q0, aux, dx = H.initial1(name)
+    np.testing.assert_array_equal(q0, g0["q0"])
+    vc = c["solver"] == "vc_acoustics"
+    b4, src, log = H.python_hooks(c["k"], dim=1)
+    with Grid1D(SOLVER_VC_ACOUSTICS if vc else SOLVER_ACOUSTICS, c["mx"], [] if vc else [1.3, 2.9],
+                maux=c["maux"]) as g:
+        if aux is not None:
+            g.set_aux(aux)
+        g.set_q(q0)
+        out = g.claw1(dx, 0.0, c["tend"], list(c["dtv"]), [1.0, 0.9], 500, c["method"],
+                      c["mthlim"], c["mthbc"], b4step1=b4, src1=src)
+        q = g.get_q()
+    print(name, "steps", out["nv"][1], "calls", len(log), flush=True)
+    np.testing.assert_array_equal(np.array(log), g0["log"])
+    assert out["info"] == int(g0["info"]) and out["tend"] == float(g0["tend"])
+    assert out["nv"][1] == g0["nv"][1]
+    np.testing.assert_array_equal(out["dtv"][2:5], g0["dtv"][2:5])
+    np.testing.assert_array_equal(out["cflv"][2:4], g0["cflv"][2:4])
+    np.testing.assert_array_equal(q[2:-2], g0["q"][2:-2])
+
+
+@pytest.mark.parametrize("name", list(H.CASES3))
+def test_hooks_match_reference_claw3(name):
+    c = H.CASES3[name]
+    g0 = np.load(os.path.join(GOLD, f"claw3_hooks_{name}.npz"))
+    q0, aux = H.initial3(name)
+    np.testing.assert_array_equal(q0, g0["q0"])
+    b4, src, log = H.python_hooks(c["k"], dim=3)
+    with Grid3D(c["mx"], c["my"], c["mz"], maux=2) as g:
+        g.set_aux(aux)
+        g.set_q(q0)
+        out = g.claw3(0.0, c["tend"], list(c["dtv"]), [1.0, 0.9], 500, method=c["method"],
+                      mthlim=c["mthlim"], mthbc=c["mthbc"], b4step3=b4, src3=src)
+        q = g.get_q()
+    print(name, "steps", out.nv[1], "calls", len(log), flush=True)
+    np.testing.assert_array_equal(np.array(log), g0["log"])
+    assert out.info == int(g0["info"]) and out.tend == float(g0["tend"])
+    assert out.nv[1] == g0["nv"][1]
+    np.testing.assert_array_equal(out.dtv[2:5], g0["dtv"][2:5])
+    np.testing.assert_array_equal(out.cflv[2:4], g0["cflv"][2:4])
+    if c["method"][2] < 0:
+        np.testing.assert_array_equal(q[2:-2, 2:-2, 2:-2], g0["q"][2:-2, 2:-2, 2:-2])
+    assert max_rel_err3(q, g0["q"]) <= 1e-12
+
+
+def test_user_bc1_and_bc3_equal_builtin():
+    """User boundary routines that extrapolate (bc1.f and bc3.f with
+    mthbc = 1: x faces, then y over every column, then z) give the built-in
+    results bit for bit, retakes included."""
+    q0, aux, dx = H.initial1("vc1d_src_strang")
+
+    def bc1(q, aux, t, dt, mthbc, mbc=2):
+        q[:mbc] = q[mbc]
+        q[-mbc:] = q[-mbc - 1]
+
+    res = []
+    for user in (False, True):
+        with Grid1D(SOLVER_VC_ACOUSTICS, q0.shape[0] - 4, [], maux=2) as g:
+            g.set_aux(aux)
+            g.set_q(q0)
+            kw = dict(bc1=bc1) if user else {}
+            out = g.claw1(dx, 0.0, 0.3, [0.04, 1e99], [1.0, 0.9], 500, (1, 2, 0, 0, 0, 0, 0),
+                          (3, 3), (0, 0) if user else (1, 1), **kw)
+            res.append((g.get_q()[2:-2], out["nv"][1], out["tend"]))
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    assert res[0][1:] == res[1][1:]
+
+    def bc3(q, aux, t, dt, mthbc, mbc=2):
+        q[:, :, :mbc] = q[:, :, mbc:mbc + 1]
+        q[:, :, -mbc:] = q[:, :, -mbc - 1:-mbc]
+        q[:, :mbc] = q[:, mbc:mbc + 1]
+        q[:, -mbc:] = q[:, -mbc - 1:-mbc]
+        q[:mbc] = q[mbc:mbc + 1]
+        q[-mbc:] = q[-mbc - 1:-mbc]
+
+    q0, aux = H.initial3("acoustics3d_src_strang_split")
+    c = H.CASES3["acoustics3d_src_strang_split"]
+    res = []
+    for user in (False, True):
+        with Grid3D(c["mx"], c["my"], c["mz"], maux=2) as g:
+            g.set_aux(aux)
+            g.set_q(q0)
+            kw = dict(bc3=bc3) if user else {}
+            out = g.claw3(0.0, 0.3, [0.3, 1e9], [1.0, 0.9], 500, method=(1, 2, -1, 0, 0, 0, 0),
+                          mthlim=(3, 3), mthbc=(0,) * 6 if user else (1,) * 6, **kw)
+            res.append((g.get_q()[2:-2, 2:-2, 2:-2], out.nv[1], out.tend))
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    assert res[0][1:] == res[1][1:]
