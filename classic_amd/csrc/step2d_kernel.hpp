@@ -37,6 +37,14 @@
 
 namespace clawgpu {
 
+// Destination of the row stores of lanes that have no cell to write (halo
+// lanes, rows outside the segment).  Every row iteration issues its four
+// stores unconditionally, redirected here when there is nothing to store:
+// with stores under a branch the compiler cannot count the outstanding
+// vector-memory operations, and its wait for the prefetched next row would
+// also wait for the previous row's stores to be acknowledged.
+static __device__ double g_store_sink[kMaxWaves * 2 * 64];
+
 // Method flags fixed at compile time (CfgFix) or read from the arguments
 // (CfgAny).  The fixed variant serves the common case method(2)=2,
 // method(3)=2, one limiter for every wave, no f-waves.
@@ -229,6 +237,11 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
             for (int m = 0; m < M; m++) vC[m] = 0.0;
         }
 
+        double vout[M];  // cell (i, J-2) when it is complete and this lane owns it
+        bool store = false;
+#pragma unroll
+        for (int m = 0; m < M; m++) vout[m] = 0.0;
+
         // ------------------------------ y-interface J (flux2, ixy=2) on column i
         S::template rpn<2, 1>(P, yp.cell, cc, yc.w, yc.s, yc.am, yc.ap, yc.cache);
         if (ycfl_lane && J >= 1 && J <= my + 1) {
@@ -292,7 +305,7 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
             }
             if (J - 2 >= r0) {
                 // cell (i, J-2): its y-sweep terms are now complete
-                const bool store = out_lane && J - 2 < r1;
+                store = out_lane && J - 2 < r1;
 #pragma unroll
                 for (int m = 0; m < M; m++) {
                     double qadd = qa_part[m] - dtdy * yp.am[m];
@@ -304,7 +317,7 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
                     double v = vA[m] + dtdx * g2l;
                     v = v + (qadd - dtdy * dF - dtdx * (G2 - G1));
                     v = v - dtdx * g1r;
-                    if (store) qn[m * fs + (long)(J - 2) * pitch] = v;
+                    vout[m] = v;
                 }
             }
 #pragma unroll
@@ -314,6 +327,12 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
                 gp1[m] = 0.5 * dtdy * bmp[m];
                 gp2[m] = 0.5 * dtdy * bpp[m];
             }
+        }
+        {
+            double* dst = store ? qn + (long)(J - 2) * pitch : g_store_sink + lane;
+            const long stride = store ? fs : 64;
+#pragma unroll
+            for (int m = 0; m < M; m++) dst[m * stride] = vout[m];
         }
 #pragma unroll
         for (int m = 0; m < M; m++) {
@@ -326,6 +345,10 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
     // recomputes in-bounds data and stores nothing)
     const int nrow = r1 - r0 + 3;
     const int jend = r0 - 1 + nrow + (nrow & 1);
+    // the loop is entered, like every iteration ends, with the row loads
+    // followed by M stores outstanding
+#pragma unroll
+    for (int m = 0; m < M; m++) g_store_sink[m * 64 + lane] = 0.0;
     for (int J = r0 - 1; J < jend; J += 2) {
         row(ysA, ysB, J);
         row(ysB, ysA, J + 1);
