@@ -313,7 +313,9 @@ def bench_batch(args):
         "iteration_ms_avg_eager": total_ms / ksteps,
         "autotuner": {"candidates": len(cands), "best_ms": best[4], "worst_ms": worst[4]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (profile_traffic("batch2d256")
+                                 if len(specs) == 256 and args.seed == 0 else None)},
         "cpu_baseline": cpu,
     }
 
