@@ -340,7 +340,9 @@ def main():
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast",
                     help="fast: FMA + reciprocal reuse, held to 1e-12 of the reference by the "
                          "tests; exact: the reference's rounding bit for bit")
-    ap.add_argument("--seg-rows", type=int, default=32)
+    ap.add_argument("--seg-rows", type=int, default=0,
+                    help="rows per wavefront segment of the step kernel (0: the library's "
+                         "per-grid choice)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain device copy of a known size (for the rocprof "
                          "FETCH_SIZE/WRITE_SIZE calibration)")
@@ -392,6 +394,7 @@ def main():
     total_ms, kern_ms, _ = g.time_steps(dx, dy, dt_last, method, mthlim, mthbc, ksteps)
     kavg_s = kern_ms / ksteps / 1e3
     achieved = cells * BYTES_PER_CELL_UPDATE / kavg_s / 1e9
+    seg_rows = g.segment_rows()
     # the same kernel time with the other arithmetic, for reference
     g.set_arith(1 - arith)
     _, kern_other, _ = g.time_steps(dx, dy, dt_last, method, mthlim, mthbc, ksteps)
@@ -432,6 +435,7 @@ def main():
         "hbm_frac_of_peak": value * BYTES_PER_CELL_UPDATE / HBM_PEAK_GBS,
         "arith": args.arith,
         "kernel_ms_avg": kavg_s * 1e3,
+        "segment_rows": seg_rows,
         "kernel_ms_avg_other_arith": kern_other / ksteps,
         "step_overhead_ms_avg": (total_ms - kern_ms) / ksteps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -49,6 +49,7 @@ _SIGS = {
     "clawgpu_claw2": ([_V, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
     "clawgpu_claw2_hooks": ([_V, _d, _d, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
     "clawgpu_set_segment_rows": ([_V, _i], _i),
+    "clawgpu_get_segment_rows": ([_V, _I], _i),
     "clawgpu_set_arith": ([_V, _i], _i),
     "clawgpu_solver_probe": ([_V, _i, _i, _D, _D, _D, _D, _D, _D, _D, _D, _D], _i),
     "clawgpu_learned_set_params": ([_V, C.POINTER(C.c_float), _i], _i),

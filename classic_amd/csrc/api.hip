@@ -129,8 +129,14 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     c->L.pitch = ((mx + 2 * mbc) + 15) / 16 * 16;
     c->L.fs = ((long)c->L.pitch * (my + 2 * mbc) + 31) / 32 * 32;
     c->cur = 0;
-    c->seglen = 32;
+    c->seglen = 0;
     c->arith = ARITH_EXACT;
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            c->ncu = 0;
+    }
     size_t qbytes = (size_t)c->L.fs * meqn * sizeof(double);
     size_t abytes = (size_t)(mx + 2 * mbc) * (my + 2 * mbc) * (meqn > maux ? meqn : maux) * sizeof(double);
     hipError_t e = hipSuccess;
@@ -290,6 +296,41 @@ static bool uses_table(const clawgpu_ctx* c, const int* mthlim)
     return false;
 }
 
+// Rows per wavefront segment of the unsplit step when none was set.  Every
+// wave of the kernel does the same work, a segment of s rows costing s + 3
+// rows (halo rows, padded to an even count), and the chip runs the waves in
+// rounds of (CUs x waves one CU holds).  The choice minimises rounds x rows
+// per wave over s in [8, 72] (longer segments measured slower than this
+// count predicts); 4096^2 Euler gets 70 rows: 4 rounds of 74 rows instead
+// of 9 rounds of 36 with 32-row segments.
+static int step_rows(clawgpu_ctx* c, const StepArgs& a)
+{
+    if (c->seglen > 0) return c->seglen;
+    bool mc = a.order == 2 && a.trans == 2 && !a.use_fwave;
+    for (int k = 0; k < c->mwaves; k++) mc = mc && a.lim[k] == 4;
+    if (c->auto_key[0] == c->arith && c->auto_key[1] == (int)mc && c->auto_key[2] == c->my)
+        return c->auto_rows;
+    const int wpc = c->arith == ARITH_FAST ? step2_waves_per_cu_fast(c->solver, a)
+                                           : step2_waves_per_cu_exact(c->solver, a);
+    const long slots = (long)(c->ncu > 0 ? c->ncu : 1) * (wpc > 0 ? wpc : kWavesPerBlock);
+    int best = 32;
+    long best_cost = -1;
+    for (int rows = 8; rows <= 72; rows++) {
+        const long waves = (long)a.nstrips * ((c->my + rows - 1) / rows);
+        const long rounds = (waves + slots - 1) / slots;
+        const long cost = rounds * (rows + 3 + ((rows + 3) & 1));
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = rows;
+        }
+    }
+    c->auto_key[0] = c->arith;
+    c->auto_key[1] = (int)mc;
+    c->auto_key[2] = c->my;
+    c->auto_rows = best;
+    return best;
+}
+
 // Enqueue one unsplit step from dq[cur] into dq[1-cur]; the Courant
 // accumulator must have been cleared (by the bc pass or explicitly).
 static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const int* method,
@@ -325,7 +366,7 @@ static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const i
     a.use_fwave = use_fwave ? 1 : 0;
     for (int k = 0; k < kMaxWaves; k++) a.lim[k] = k < c->mwaves ? mthlim[k] : 0;
     a.nstrips = (c->mx + kStripOut - 1) / kStripOut;
-    a.seglen = c->seglen;
+    a.seglen = step_rows(c, a);
     a.nsegs = (c->my + a.seglen - 1) / a.seglen;
     a.cfl_bits = c->dcfl;
     a.P = c->P;
@@ -384,7 +425,7 @@ static int enqueue_sweep(clawgpu_ctx* c, int dir, int from, int to, double dtd, 
     a.order = method[1];
     a.use_fwave = use_fwave ? 1 : 0;
     for (int k = 0; k < kMaxWaves; k++) a.lim[k] = k < c->mwaves ? mthlim[k] : 0;
-    a.seglen = c->seglen;
+    a.seglen = c->seglen > 0 ? c->seglen : 32;
     if (dir == 1) {
         a.nstrips = (c->mx + kStripOut - 1) / kStripOut;
         a.nsegs = (c->my + 2 * c->mbc + a.seglen - 1) / a.seglen;
@@ -684,8 +725,15 @@ extern "C" int clawgpu_copy_calibration(long ndoubles, int reps, float* ms_avg)
 
 extern "C" int clawgpu_set_segment_rows(clawgpu_ctx* c, int rows)
 {
-    if (!c || rows < 1) return set_err(CLAWGPU_ERR_ARG, "rows must be >= 1");
+    if (!c || rows < 0) return set_err(CLAWGPU_ERR_ARG, "rows must be >= 0 (0: chosen per grid)");
     c->seglen = rows;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_get_segment_rows(clawgpu_ctx* c, int* rows)
+{
+    if (!c || !rows) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    *rows = c->seglen > 0 ? c->seglen : (c->auto_key[0] >= 0 ? c->auto_rows : 0);
     return CLAWGPU_OK;
 }
 

@@ -65,6 +65,10 @@ int launch_sweep_fast(int solver, int dir, const SweepArgs& a, int nblocks, hipS
 // correctly rounded division); fast: FMA contraction and reciprocal reuse.
 int launch_step2_exact(int solver, const StepArgs& a, int nblocks, hipStream_t s);
 int launch_step2_fast(int solver, const StepArgs& a, int nblocks, hipStream_t s);
+// Wavefronts of the step kernel launch_step2_* would pick for a that one CU
+// holds at once (0 if the occupancy query fails).
+int step2_waves_per_cu_exact(int solver, const StepArgs& a);
+int step2_waves_per_cu_fast(int solver, const StepArgs& a);
 
 int launch_probe_exact(int solver, int n, int ixy, const SolverParams& P, const double* ql,
                        const double* qr, const double* asdq, double* wave, double* s,

@@ -23,7 +23,10 @@ struct clawgpu_ctx {
     unsigned long long* dcfl;
     double* hcfl;  // pinned
     hipStream_t stream;
-    int seglen;
+    int seglen;       // rows per wavefront segment of the unsplit step; 0: chosen per launch
+    int ncu = 0;      // compute units of the context's device
+    int auto_key[3] = {-1, -1, -1};  // (arith, fixed config, rows) the cached choice was made for
+    int auto_rows = 32;
     int arith;
     double* daux = nullptr;   // aux planes (solvers with aux arrays)
     std::vector<double> haux;  // host copy of aux (reference layout) for user hooks

@@ -9,6 +9,10 @@ int launch_step2_exact(int solver, const StepArgs& a, int nblocks, hipStream_t s
 {
     return launch_step2_impl<ExactMath>(solver, a, nblocks, s);
 }
+int step2_waves_per_cu_exact(int solver, const StepArgs& a)
+{
+    return step2_waves_per_cu_impl<ExactMath>(solver, a);
+}
 int launch_probe_exact(int solver, int n, int ixy, const SolverParams& P, const double* ql,
                       const double* qr, const double* asdq, double* wave, double* s, double* amdq,
                       double* apdq, double* bm, double* bp, hipStream_t st)

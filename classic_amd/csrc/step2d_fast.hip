@@ -12,6 +12,10 @@ int launch_step2_fast(int solver, const StepArgs& a, int nblocks, hipStream_t s)
 {
     return launch_step2_impl<FastMath>(solver, a, nblocks, s);
 }
+int step2_waves_per_cu_fast(int solver, const StepArgs& a)
+{
+    return step2_waves_per_cu_impl<FastMath>(solver, a);
+}
 int launch_probe_fast(int solver, int n, int ixy, const SolverParams& P, const double* ql,
                       const double* qr, const double* asdq, double* wave, double* s, double* amdq,
                       double* apdq, double* bm, double* bp, hipStream_t st)

@@ -415,11 +415,36 @@ static int launch_probe_impl(int solver, int n, int ixy, const SolverParams& P, 
     return (int)hipGetLastError();
 }
 
+// the common configuration (second order, transverse 2, MC on every wave,
+// no f-waves) has its own instantiation with the flags fixed
 template <class S>
-static void launch_cfg(const StepArgs& a, int nblocks, hipStream_t s)
+static bool fixed_cfg(const StepArgs& a)
 {
     bool mc = a.order == 2 && a.trans == 2 && !a.use_fwave;
     for (int k = 0; k < S::MW; k++) mc = mc && a.lim[k] == 4;
+    return mc;
+}
+
+template <class S>
+static int waves_per_cu_cfg(const StepArgs& a)
+{
+    int nb = 0;
+    hipError_t e = fixed_cfg<S>(a)
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2_unsplit_kernel<S, CfgFix<2, 2, 4>>, 256, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step2_unsplit_kernel<S, CfgAny>, 256, 0);
+    return e == hipSuccess ? nb * kWavesPerBlock : 0;
+}
+
+template <class Mth>
+static int step2_waves_per_cu_impl(int solver, const StepArgs& a)
+{
+    return solver == 1 ? waves_per_cu_cfg<Acoustics2<Mth>>(a) : waves_per_cu_cfg<Euler2<Mth>>(a);
+}
+
+template <class S>
+static void launch_cfg(const StepArgs& a, int nblocks, hipStream_t s)
+{
+    const bool mc = fixed_cfg<S>(a);
     if (mc)
         hipLaunchKernelGGL((step2_unsplit_kernel<S, CfgFix<2, 2, 4>>), dim3(nblocks), dim3(256), 0, s, a);
     else

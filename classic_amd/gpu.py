@@ -244,7 +244,14 @@ class Grid2D:
         return (wave, s, *out)
 
     def set_segment_rows(self, rows):
+        """Rows per wavefront segment of the step kernel (0: chosen per grid)."""
         check(self.lib.clawgpu_set_segment_rows(self.h, int(rows)))
+
+    def segment_rows(self):
+        """Rows per segment the last unsplit step used (0 before the first)."""
+        r = C.c_int(0)
+        check(self.lib.clawgpu_get_segment_rows(self.h, C.byref(r)))
+        return r.value
 
     def set_arith(self, mode):
         """0 = exact reference arithmetic (default), 1 = fast (FMA)."""

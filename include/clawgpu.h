@@ -148,11 +148,15 @@ int clawgpu_claw2_hooks(clawgpu_ctx* ctx, double dx, double dy, double xlower, d
                         const clawgpu_hooks* hooks, int* info);
 
 /* Tuning and measurement helpers (no reference counterpart): rows per
- * wavefront segment of the step kernel, the arithmetic mode, and nsteps fixed-dt steps timed with
+ * wavefront segment of the step kernel (0, the default: chosen per grid so
+ * that the kernel's waves fill whole rounds of the chip; get_segment_rows
+ * reports the rows the last unsplit step used, 0 before the first), the
+ * arithmetic mode, and nsteps fixed-dt steps timed with
  * HIP events on the context's stream without per-step host synchronisation:
  * *total_ms spans every launch, *step_kernel_ms sums the step-kernel launches
  * alone, *cfl_max receives the largest Courant number of those steps. */
 int clawgpu_set_segment_rows(clawgpu_ctx* ctx, int rows);
+int clawgpu_get_segment_rows(clawgpu_ctx* ctx, int* rows);
 int clawgpu_set_arith(clawgpu_ctx* ctx, int mode);
 int clawgpu_time_steps(clawgpu_ctx* ctx, double dx, double dy, double dt, const int* method,
                        const int* mthlim, const int* mthbc, int use_fwave, int nsteps,

@@ -100,7 +100,7 @@ def test_config1_acoustics_200_to_t06():
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
-@pytest.mark.parametrize("seglen", [64, 5])
+@pytest.mark.parametrize("seglen", [64, 5, 0])
 def test_euler_claw2_quadrants(seed, seglen):
     mx, my = 150, 131
     q, dx, dy = euler_problem(mx, my, seed)
@@ -112,6 +112,9 @@ def test_euler_claw2_quadrants(seed, seglen):
         g.set_q(q)
         out = g.claw2(dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 40, method, mthlim, mthbc)
         q_g = g.get_q()
+        rows = g.segment_rows()
+    # 0: the library's per-grid choice, within its search range
+    assert rows == seglen if seglen else 8 <= rows <= 72
     assert out["info"] == r.info == 11  # stopped by steps_max, as requested
     assert out["nv"][1] == r.nv[1] == 40
     np.testing.assert_array_equal(out["dtv"], r.dtv)
