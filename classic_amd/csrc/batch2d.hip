@@ -628,7 +628,7 @@ extern "C" int clawgpu_batch_tune(clawgpu_batch* b, int reps, int* ncand, int* c
     if (!b || !ncand) return set_err(CLAWGPU_ERR_ARG, "null argument");
     if (!b->started) return set_err(CLAWGPU_ERR_ARG, "clawgpu_batch_start has not been called");
     if (reps < 1) return set_err(CLAWGPU_ERR_ARG, "reps must be >= 1");
-    const int seglens[] = {16, 32, 64, 128};
+    const int seglens[] = {16, 24, 32, 48, 64, 96, 128};
     const int waves_per_cu[] = {4, 8, 12, 16};
     Schedule best = b->sched;
     float best_ms = 0.f;

@@ -68,7 +68,7 @@ def _batch(specs, qs, tend, dtv, cflv, nmax, method=METHOD, mthlim=MTHLIM, arith
         if tune:
             b.advance(3, graph=graph)
             cands = b.tune(reps=1)
-            assert len(cands) == 64
+            assert len(cands) == 112  # 7 tile heights x 4 grid sizes x 2 orders x 2 budgets
             best = min(cands, key=lambda c: c[4])
             s = b.schedule()
             assert (s["seglen"], s["nblocks"], s["largest_first"], s["w3"]) == best[:4]
