@@ -469,10 +469,18 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
         for (int kk = 0; kk < TZ + 2; kk++) {
             const int kp = k0 - 1 + kk;
             Mat W[3][3];  // rows j'-1..j'+1 (t1) x planes k'-1..k'+1 (t2) of cell i
+            Mat Wn[3];    // row j'+2, loaded one slice ahead
+            double pn, un;  // p, u of row j'+1, loaded one slice ahead
 #pragma unroll
             for (int b = 0; b < 3; b++) {
                 W[1][b] = load_mat(a, L.offc(i, j0 - 2, kp + b - 1));
                 W[2][b] = load_mat(a, L.offc(i, j0 - 1, kp + b - 1));
+                Wn[b] = load_mat(a, L.offc(i, j0, kp + b - 1));
+            }
+            {
+                const long c = L.offc(i, j0 - 1, kp);
+                pn = a.q[c];
+                un = a.q[fs + c];
             }
 #pragma unroll
             for (int jj = 0; jj < TY + 2; jj++) {
@@ -481,10 +489,16 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                 for (int b = 0; b < 3; b++) {
                     W[0][b] = W[1][b];
                     W[1][b] = W[2][b];
-                    W[2][b] = load_mat(a, L.offc(i, jp + 1, kp + b - 1));
+                    W[2][b] = Wn[b];
                 }
-                const long c = L.offc(i, jp, kp);
-                const double p = a.q[c], u = a.q[fs + c];
+                const double p = pn, u = un;
+                if (jj + 1 < TY + 2) {  // next slice's data, in flight during this one
+#pragma unroll
+                    for (int b = 0; b < 3; b++) Wn[b] = load_mat(a, L.offc(i, jp + 2, kp + b - 1));
+                    const long cn = L.offc(i, jp + 1, kp);
+                    pn = a.q[cn];
+                    un = a.q[fs + cn];
+                }
                 const Mat me = W[1][1];
                 Iface f = solve<M>(lane_up(p), lane_up(u), mat_up(me), p, u, me);
                 Iface fl, fr;
@@ -515,6 +529,8 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
             Mat C3[TY + 4][3];
             Iface F[TY + 4];
             Wv CQ[TY + 4];
+            // every load of the column first, so that they are all in flight
+            // before the first solve needs one
 #pragma unroll
             for (int cc = 0; cc < TY + 4; cc++) {
                 const int j = j0 - 2 + cc;
@@ -523,6 +539,9 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                 V[cc] = a.q[2 * fs + c];
 #pragma unroll
                 for (int s = 0; s < 3; s++) C3[cc][s] = load_mat(a, L.offc(i, j, kp + s - 1));
+            }
+#pragma unroll
+            for (int cc = 0; cc < TY + 4; cc++) {
                 if (cc >= 1) F[cc] = solve<M>(P[cc - 1], V[cc - 1], C3[cc - 1][1], P[cc], V[cc], C3[cc][1]);
                 if (cc >= 3) CQ[cc - 1] = limited_cq<M, Cfg>(a, F[cc - 1], F[cc - 2], F[cc], d);
                 if (cc >= 4) {
@@ -564,6 +583,9 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                 W_[cc] = a.q[3 * fs + c];
 #pragma unroll
                 for (int s = 0; s < 3; s++) C3[cc][s] = load_mat(a, L.offc(i, jp + s - 1, k));
+            }
+#pragma unroll
+            for (int cc = 0; cc < TZ + 4; cc++) {
                 if (cc >= 1) F[cc] = solve<M>(P[cc - 1], W_[cc - 1], C3[cc - 1][1], P[cc], W_[cc], C3[cc][1]);
                 if (cc >= 3) CQ[cc - 1] = limited_cq<M, Cfg>(a, F[cc - 1], F[cc - 2], F[cc], d);
                 if (cc >= 4) {
