@@ -324,8 +324,18 @@ __device__ inline void apply_targets(const Rec& R, double qp, double qn, double 
     T.t2[8] = d2 * R.ht[1][2];
 }
 
-// Add T to the accumulator: idx(t, m) gives the LDS index of component m of
-// target t (an index in the trash row for targets outside the tile).  LANE
+// Accumulator layout: word 1 + ((kz*TY + jy)*4 + m)*64 + lane for cell
+// (kz, jy) of the tile, component m, lane -1..64.  The slots of lanes 0, 1,
+// 62 and 63 belong to halo cells whose result is never stored, so the
+// increments a lane owes lane -1 or 64 may land in a neighbouring row's slot
+// of lane 63 or 0, and increments for cells outside the tile all go to word
+// 0 (+ 64 m): a don't-care slot for every component.  Component m of a cell
+// is 64 m words after component 0.
+constexpr int kAccRow = 64;
+
+// Add T to the accumulator: idx(t) gives the LDS index of component 0 of
+// target t (the don't-care word 0 for targets outside the tile; that test is
+// uniform across the wave).  LANE
 // names the target offset that moves to another lane (0: none, 1: the t2
 // offset b = t/3 - 1, 2: the t1 offset a = t%3 - 1).  Targets are processed
 // in rounds of one lane offset: within a round every lane touches distinct
@@ -338,7 +348,7 @@ __device__ inline void scatter(double* acc, const T9& T, bool trans, int cn, int
                                Idx&& idx)
 {
     if (!trans) {
-        const int ip = idx(4, 0), in = idx(4, cn);
+        const int ip = idx(4), in = ip + kAccRow * cn;
         const double op = acc[ip], on = acc[in];
         acc[ip] = op + T.p[4];
         acc[in] = on + T.n;
@@ -358,12 +368,12 @@ __device__ inline void scatter(double* acc, const T9& T, bool trans, int cn, int
         double op[9], o1[9], o2[9];
 #pragma unroll
         for (int k = 0; k < nt; k++) {
-            ip[k] = idx(ts[k], 0);
-            i1[k] = idx(ts[k], c1);
-            i2[k] = idx(ts[k], c2);
+            ip[k] = idx(ts[k]);
+            i1[k] = ip[k] + kAccRow * c1;
+            i2[k] = ip[k] + kAccRow * c2;
         }
         const bool centre = LANE == 0 || g == 1;
-        const int in = centre ? idx(4, cn) : 0;
+        const int in = centre ? idx(4) + kAccRow * cn : 0;
 #pragma unroll
         for (int k = 0; k < nt; k++) {
             op[k] = acc[ip[k]];
@@ -411,10 +421,10 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
 {
     using M = FastMath;
     static_assert(MI || !KL, "the coefficient table needs the material table");
-    // accumulator of the tile, plus one row that receives the increments of
-    // cells outside the tile (so that the slice code has no branches)
-    constexpr int kTrash = TZ * TY * 4 * 64;
-    __shared__ double acc[kTrash + 64];
+    // accumulator of the tile (layout above kAccRow), so that the slice code
+    // has no branches
+    constexpr int kAccWords = TZ * TY * 4 * kAccRow + 2;
+    __shared__ double acc[kAccWords];
     __shared__ double lut[MI ? 2 * kMaxMat : 1];
     __shared__ Kc klut_s[KL ? kMaxKcMat * kMaxKcMat * kMaxKcMat : 1];
     const Kc* klut = klut_s;
@@ -458,9 +468,10 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
     const Layout3& L = a.L;
     const long fs = L.fs;
 #pragma unroll
-    for (int e = 0; e < TZ * TY * 4; e++) acc[e * 64 + l] = 0.0;
-#define ACC(kz, jy, m, lane) acc[(((kz) * TY + (jy)) * 4 + (m)) * 64 + (lane)]
-#define SLOTI(ok, kz, jy, m, lane) ((ok) ? (((kz) * TY + (jy)) * 4 + (m)) * 64 + (lane) : kTrash + l)
+    for (int e = 0; e < TZ * TY * 4; e++) acc[1 + e * kAccRow + l] = 0.0;
+#define ACC(kz, jy, m, lane) acc[1 + (((kz) * TY + (jy)) * 4 + (m)) * kAccRow + (lane)]
+    // component-0 slot of cell (kz, jy) at lane ln in [-1, 64]; ok is uniform
+#define SLOTI(ok, kz, jy, ln) ((ok) ? 1 + ((kz) * TY + (jy)) * 4 * kAccRow + (ln) : 0)
     const bool trans = Cfg::m3(a) > 0;
 
     // ---- x sweeps: slice (j', k'), one interface per lane (its low face)
@@ -511,10 +522,10 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                 T9 T;
                 slice_position<M, Cfg, KL>(a, f, cq, hi, cqhi, W, d, d1, d2, T, klut, a.nmat);
                 // x: n = u, t1 = v (rows, offset a), t2 = w (planes, offset b)
-                scatter<0>(acc, T, trans, 1, 2, 3, [&](int t, int m) {
+                scatter<0>(acc, T, trans, 1, 2, 3, [&](int t) {
                     const int jy = jp + (t % 3 - 1) - j0, kz = kp + (t / 3 - 1) - k0;
                     const bool ok = jy >= 0 && jy < TY && kz >= 0 && kz < TZ;
-                    return SLOTI(ok, kz, jy, m, l);
+                    return SLOTI(ok, kz, jy, l);
                 });
             }
         }
@@ -556,10 +567,10 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                     T9 T;
                     slice_position<M, Cfg, KL>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2, T, klut, a.nmat);
                     // y: n = v, t1 = w (planes, offset a), t2 = u (lanes, offset b)
-                    scatter<1>(acc, T, trans, 2, 3, 1, [&](int t, int m) {
+                    scatter<1>(acc, T, trans, 2, 3, 1, [&](int t) {
                         const int kz = kp + (t % 3 - 1) - k0, ln = l + (t / 3 - 1);
-                        const bool ok = kz >= 0 && kz < TZ && ln >= 0 && ln <= 63;
-                        return SLOTI(ok, kz, jy, m, ln);
+                        const bool ok = kz >= 0 && kz < TZ;
+                        return SLOTI(ok, kz, jy, ln);
                     });
                 }
             }
@@ -600,10 +611,10 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
                     T9 T;
                     slice_position<M, Cfg, KL>(a, F[r], CQ[r], F[r + 1], CQ[r + 1], N, d, d1, d2, T, klut, a.nmat);
                     // z: n = w, t1 = u (lanes, offset a), t2 = v (rows, offset b)
-                    scatter<2>(acc, T, trans, 3, 1, 2, [&](int t, int m) {
+                    scatter<2>(acc, T, trans, 3, 1, 2, [&](int t) {
                         const int jy = jp + (t / 3 - 1) - j0, ln = l + (t % 3 - 1);
-                        const bool ok = jy >= 0 && jy < TY && ln >= 0 && ln <= 63;
-                        return SLOTI(ok, kz, jy, m, ln);
+                        const bool ok = jy >= 0 && jy < TY;
+                        return SLOTI(ok, kz, jy, ln);
                     });
                 }
             }
