@@ -328,7 +328,7 @@ extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int mau
     c->L.nx = mx + 2 * mbc; c->L.ny = my + 2 * mbc; c->L.nz = mz + 2 * mbc;
     c->L.fs = (long)c->L.nx * c->L.ny * c->L.nz;
     c->seg = 64;
-    c->tile = 1;
+    c->tile = 4;
     c->xcd = 0;
     c->cmax = 0.0;
     const size_t planes = (size_t)c->L.fs * sizeof(double);

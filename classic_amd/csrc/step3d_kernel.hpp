@@ -342,13 +342,18 @@ constexpr int kAccRow = 64;
 // words (its own cells, or all its neighbours on the same side), so the
 // round's reads can all be issued before its writes; across rounds the
 // read-modify-writes are ordered, which keeps the increments that two lanes
-// add to one cell.
+// add to one cell.  A target whose slot folds to the constant word 0 once
+// the sweep loops are unrolled (outside the tile for every wave) is skipped
+// altogether, and with it the arithmetic that only feeds it.
+__device__ inline bool known_outside(int slot) { return __builtin_constant_p(slot) && slot == 0; }
+
 template <int LANE, class Idx>
 __device__ inline void scatter(double* acc, const T9& T, bool trans, int cn, int c1, int c2,
                                Idx&& idx)
 {
     if (!trans) {
         const int ip = idx(4), in = ip + kAccRow * cn;
+        if (known_outside(ip)) return;
         const double op = acc[ip], on = acc[in];
         acc[ip] = op + T.p[4];
         acc[in] = on + T.n;
@@ -374,20 +379,23 @@ __device__ inline void scatter(double* acc, const T9& T, bool trans, int cn, int
         }
         const bool centre = LANE == 0 || g == 1;
         const int in = centre ? idx(4) + kAccRow * cn : 0;
+        const bool cdead = centre && known_outside(idx(4));
 #pragma unroll
         for (int k = 0; k < nt; k++) {
+            if (known_outside(ip[k])) continue;
             op[k] = acc[ip[k]];
             o1[k] = acc[i1[k]];
             o2[k] = acc[i2[k]];
         }
-        const double on = centre ? acc[in] : 0.0;
+        const double on = centre && !cdead ? acc[in] : 0.0;
 #pragma unroll
         for (int k = 0; k < nt; k++) {
+            if (known_outside(ip[k])) continue;
             acc[ip[k]] = op[k] + T.p[ts[k]];
             acc[i1[k]] = o1[k] + T.t1[ts[k]];
             acc[i2[k]] = o2[k] + T.t2[ts[k]];
         }
-        if (centre) acc[in] = on + T.n;
+        if (centre && !cdead) acc[in] = on + T.n;
     }
 }
 

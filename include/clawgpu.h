@@ -340,8 +340,8 @@ int clawgpu3_claw3_hooks(clawgpu3_ctx* ctx, const double* geom, double tstart, d
 /* Cells per segment of the y/z split sweeps (tuning; default 64). */
 int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
 
-/* Tile shape of the unsplit kernel (tuning): 0 = 60x4x4, 1 = 60x4x2
- * (default), 2 = 60x2x2, 3 = 60x8x2, 4 = 60x2x4 cells per wavefront;
+/* Tile shape of the unsplit kernel (tuning): 0 = 60x4x4, 1 = 60x4x2,
+ * 2 = 60x2x2, 3 = 60x8x2, 4 = 60x2x4 (default) cells per wavefront;
  * xcd_order = 1 hands each XCD a contiguous range of tiles (default 0). */
 int clawgpu3_set_tile(clawgpu3_ctx* ctx, int tile, int xcd_order);
 
