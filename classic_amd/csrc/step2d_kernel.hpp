@@ -303,29 +303,54 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
 #pragma unroll
                 for (int m = 0; m < M; m++) bma[m] = bpa[m] = bmp[m] = bpp[m] = 0.0;
             }
-            if (J - 2 >= r0) {
-                // cell (i, J-2): its y-sweep terms are now complete
-                store = out_lane && J - 2 < r1;
+            if constexpr (Mth::fast) {
+                // Fast arithmetic sums a cell's y terms in another order:
+                // interface J-1 adds its upper-face terms to cell J-2 (now
+                // complete) and its lower-face terms to cell J-1 at once,
+                // so nothing of it is carried to the next row.
+                if (J - 2 >= r0) {
+                    store = out_lane && J - 2 < r1;
+#pragma unroll
+                    for (int m = 0; m < M; m++) {
+                        double G1 = trans ? -0.5 * dtdy * bma[m] : 0.0;
+                        double G2 = trans ? -0.5 * dtdy * bpa[m] : 0.0;
+                        double v = vA[m] - dtdy * yp.am[m] - dtdy * fa[m];
+                        v = v + dtdx * lane_up(G2) - dtdx * lane_dn(G1) - dtdx * (G2 - G1);
+                        vout[m] = v;
+                    }
+                }
 #pragma unroll
                 for (int m = 0; m < M; m++) {
-                    double qadd = qa_part[m] - dtdy * yp.am[m];
-                    double dF = fa[m] - fa_part[m];
-                    double G1 = trans ? (-0.5 * dtdy * bma[m]) - gp1[m] : 0.0;
-                    double G2 = trans ? (-0.5 * dtdy * bpa[m]) - gp2[m] : 0.0;
-                    double g2l = lane_up(G2);
-                    double g1r = lane_dn(G1);
-                    double v = vA[m] + dtdx * g2l;
-                    v = v + (qadd - dtdy * dF - dtdx * (G2 - G1));
-                    v = v - dtdx * g1r;
-                    vout[m] = v;
+                    double G1 = trans ? -0.5 * dtdy * bmp[m] : 0.0;
+                    double G2 = trans ? -0.5 * dtdy * bpp[m] : 0.0;
+                    double v = vB[m] - dtdy * yp.ap[m] + dtdy * fa[m];
+                    vB[m] = v + dtdx * lane_up(G2) - dtdx * lane_dn(G1) - dtdx * (G2 - G1);
                 }
-            }
+            } else {
+                if (J - 2 >= r0) {
+                    // cell (i, J-2): its y-sweep terms are now complete
+                    store = out_lane && J - 2 < r1;
 #pragma unroll
-            for (int m = 0; m < M; m++) {
-                qa_part[m] = 0.0 - dtdy * yp.ap[m];
-                fa_part[m] = fa[m];
-                gp1[m] = 0.5 * dtdy * bmp[m];
-                gp2[m] = 0.5 * dtdy * bpp[m];
+                    for (int m = 0; m < M; m++) {
+                        double qadd = qa_part[m] - dtdy * yp.am[m];
+                        double dF = fa[m] - fa_part[m];
+                        double G1 = trans ? (-0.5 * dtdy * bma[m]) - gp1[m] : 0.0;
+                        double G2 = trans ? (-0.5 * dtdy * bpa[m]) - gp2[m] : 0.0;
+                        double g2l = lane_up(G2);
+                        double g1r = lane_dn(G1);
+                        double v = vA[m] + dtdx * g2l;
+                        v = v + (qadd - dtdy * dF - dtdx * (G2 - G1));
+                        v = v - dtdx * g1r;
+                        vout[m] = v;
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    qa_part[m] = 0.0 - dtdy * yp.ap[m];
+                    fa_part[m] = fa[m];
+                    gp1[m] = 0.5 * dtdy * bmp[m];
+                    gp2[m] = 0.5 * dtdy * bpp[m];
+                }
             }
         }
         {
