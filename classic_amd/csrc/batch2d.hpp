@@ -121,10 +121,17 @@ __device__ __forceinline__ void step_loop(const Params& b)
     }
 }
 
-// Euler: the single-grid kernel's register budget (one wave per SIMD);
-// acoustics: three waves per SIMD, as its single-grid kernel reaches.
+// The compiler's register budget; step_kernel_w2 holds the Euler kernel
+// to two waves per SIMD (256 registers) as the single-grid kernel reaches
+// with fast arithmetic, step_kernel_w3 the acoustics kernel to three.
 template <class S, class Cfg, int G>
 __global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step_kernel(Params b)
+{
+    step_loop<S, Cfg, G>(b);
+}
+
+template <class S, class Cfg, int G>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void step_kernel_w2(Params b)
 {
     step_loop<S, Cfg, G>(b);
 }
@@ -151,6 +158,11 @@ inline void launch_group(const Params& a, bool mc, int nblk, bool w3, hipStream_
             else hipLaunchKernelGGL((step_kernel_w3<S, CfgAny, G>), dim3(nblk), dim3(256), 0, s, a);
             return;
         }
+    }
+    if constexpr (G == 0 && S::Math::fast) {
+        if (mc) hipLaunchKernelGGL((step_kernel_w2<S, CfgFix<2, 2, 4>, G>), dim3(nblk), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((step_kernel_w2<S, CfgAny, G>), dim3(nblk), dim3(256), 0, s, a);
+        return;
     }
     if (mc) hipLaunchKernelGGL((step_kernel<S, CfgFix<2, 2, 4>, G>), dim3(nblk), dim3(256), 0, s, a);
     else hipLaunchKernelGGL((step_kernel<S, CfgAny, G>), dim3(nblk), dim3(256), 0, s, a);
