@@ -27,6 +27,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "riemann2d.hpp"
 #include "lanes.hpp"
@@ -58,12 +60,183 @@ struct CfgFix {
     static constexpr int order = ORDER, trans = TRANS, lim = LIM;
 };
 
+// The same wavefront work for constant-coefficient acoustics in fast
+// arithmetic and the fixed configuration (second order, transverse 2, one
+// limiter): the solver's structure written out.  Both waves of an interface
+// are multiples of fixed eigenvectors, (-Z, 1) and (Z, 1) in (p, u_n) at
+// speeds -c and +c, so with alpha1, alpha2 their strengths
+//   * the limiter ratio w(k')·w(k) / w(k)·w(k) is alpha(k') / alpha(k);
+//   * cqxx = kappa (Z (b2 - b1), b1 + b2) with b = phi alpha and
+//     kappa = c (1 - c dt/dx) for both waves;
+//   * a fluctuation has no transverse velocity, so its transverse split
+//     depends on its pressure a alone: B^-a = (-c/2 a, c/(2Z) a),
+//     B^+a = (c/2 a, c/(2Z) a) in (p, v_t);
+// and the Courant number of every interface is c dt/dx.  The sums are the
+// generic kernel's (fast-arithmetic order); results agree with it and with
+// the reference to rounding.
+template <class Cfg>
+__device__ __forceinline__ void acoustics_wave_fast(const StepArgs& a, const int strip,
+                                                    const int seg, const int lane)
+{
+    using Mth = FastMath;
+    const Layout2 L = a.L;
+    const int mx = L.mx, my = L.my, mbc = L.mbc;
+    const int i = kStripOut * strip + lane - 1;
+    const int ic = i < mx + mbc ? i : mx + mbc;
+    const int r0 = 1 + seg * a.seglen;
+    const int r1 = min(r0 + a.seglen, my + 1);
+    const bool out_lane = lane >= 2 && lane <= kStripOut + 1 && i <= mx;
+    const bool xcfl_lane = lane >= 1 && i >= 1 && i <= mx + 1;
+    const bool ycfl_lane = i >= 0 && i <= mx + 1;
+    const double dtdx = a.dtdx, dtdy = a.dtdy;
+    const double* __restrict__ qo = a.qold + L.off(ic, 0);
+    double* __restrict__ qn = a.qnew + L.off(i, 0);
+    const long pitch = L.pitch, fs = L.fs;
+    const int rowmax = my + mbc;
+    const double zz = a.P.zz, cc = a.P.cc;
+    const double h = 0.5 / zz;             // 1 / (2Z)
+    const double hc = 0.5 * cc, gam = cc * h;  // B^+- coefficients c/2, c/(2Z)
+    const double kx = cc * (1.0 - cc * dtdx), ky = cc * (1.0 - cc * dtdy);
+    const double cz = cc * zz;
+    const double cflx = cfl_term(dtdx, cc), cfly = cfl_term(dtdy, cc);
+
+    // one y-interface: strengths of its two waves and of the 2-wave of the
+    // interface below it (the limiter's upwind neighbour)
+    struct YA {
+        double a1, a2, a2lo;
+    };
+    YA ysA{0.0, 0.0, 0.0}, ysB{0.0, 0.0, 0.0};
+    double pp = qo[(r0 - 2) * pitch], vp = qo[2 * fs + (r0 - 2) * pitch];  // cell J-1
+    double qnx[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) qnx[m] = qo[m * fs + (r0 - 1) * pitch];
+    double vA[3] = {0.0, 0.0, 0.0}, vB[3] = {0.0, 0.0, 0.0};
+    double g2p = 0.0, g2v = 0.0;  // x-row J-1's G2 owed to row J (p, v)
+    double cfl = 0.0;
+
+    auto phi = [&](double num, double al) {
+        const double r = num * Mth::rcp(al);
+        return al != 0.0 ? philim<Mth>(r, Cfg::lim) : 0.0;
+    };
+
+    auto row = [&](YA& yp, YA& yc, const int J) {
+        const double p = qnx[0], u = qnx[1], v = qnx[2];
+        {
+            const long jn = min(J + 1, rowmax);
+#pragma unroll
+            for (int m = 0; m < 3; m++) qnx[m] = qo[m * fs + jn * pitch];
+        }
+        double vC[3];
+        // ---------------------------------------------- x-row J
+        if (J <= r1) {
+            if (xcfl_lane) cfl = fmax(cfl, cflx);
+            const double dp = p - lane_up(p), du = u - lane_up(u);
+            const double a1 = (-dp + zz * du) * h, a2 = (dp + zz * du) * h;
+            const double b1 = phi(lane_dn(a1), a1) * a1;  // 1-wave: speed -c, upwind = right
+            const double b2 = phi(lane_up(a2), a2) * a2;  // 2-wave: speed +c, upwind = left
+            const double cqp = kx * (zz * (b2 - b1)), cqu = kx * (b1 + b2);
+            const double fap = 0.5 * cqp, fau = 0.5 * cqu;
+            const double amp = cz * a1, amu = -cc * a1;  // A^- dq = -c w1
+            const double app = cz * a2, apu = cc * a2;   // A^+ dq = +c w2
+            const double amt = amp + cqp, apt = app - cqp;  // pressures to split in y
+            // G1 = B^- parts, G2 = B^+ parts (p, v), the A^- side shifted left
+            const double hx = -0.5 * dtdx;
+            const double g1p = lane_dn(hx * (-hc * amt)) + hx * (-hc * apt);
+            const double g2pn = lane_dn(hx * (hc * amt)) + hx * (hc * apt);
+            const double g1v = lane_dn(hx * (gam * amt)) + hx * (gam * apt);
+            const double g2vn = g1v;  // B^- and B^+ have the same v part
+            const double amnp = lane_dn(amp), amnu = lane_dn(amu);
+            const double fanp = lane_dn(fap), fanu = lane_dn(fau);
+            {
+                const double qadd = (0.0 - dtdx * app) - dtdx * amnp;
+                double w = p + dtdy * g2p;
+                vC[0] = w + qadd - dtdx * (fanp - fap) - dtdy * (g2pn - g1p);
+            }
+            {
+                const double qadd = (0.0 - dtdx * apu) - dtdx * amnu;
+                vC[1] = u + qadd - dtdx * (fanu - fau);
+            }
+            vC[2] = v + dtdy * g2v - dtdy * (g2vn - g1v);
+            vB[0] = vB[0] - dtdy * g1p;
+            vB[2] = vB[2] - dtdy * g1v;
+            g2p = g2pn;
+            g2v = g2vn;
+        } else {
+            vC[0] = vC[1] = vC[2] = 0.0;
+        }
+
+        // ---------------------------------- y-interface J on column i
+        if (ycfl_lane && J >= 1 && J <= my + 1) cfl = fmax(cfl, cfly);
+        {
+            const double dp = p - pp, dv = v - vp;
+            yc.a1 = (-dp + zz * dv) * h;
+            yc.a2 = (dp + zz * dv) * h;
+            yc.a2lo = yp.a2;
+        }
+        pp = p;
+        vp = v;
+        double vout[3] = {0.0, 0.0, 0.0};
+        bool store = false;
+        if (J - 1 >= r0) {
+            // interface J-1 (yp) limited with its neighbours J-2 and J
+            const double b1 = phi(yc.a1, yp.a1) * yp.a1;
+            const double b2 = phi(yp.a2lo, yp.a2) * yp.a2;
+            const double cqp = ky * (zz * (b2 - b1)), cqv = ky * (b1 + b2);
+            const double fap = 0.5 * cqp, fav = 0.5 * cqv;
+            const double amp = cz * yp.a1, amv = -cc * yp.a1;
+            const double app = cz * yp.a2, apv = cc * yp.a2;
+            const double amt = amp + cqp, apt = app - cqp;  // pressures to split in x
+            const double hy = -0.5 * dtdy;
+            if (J - 2 >= r0) {
+                // cell J-2: the upper-face terms complete it
+                store = out_lane && J - 2 < r1;
+                const double G1p = hy * (-hc * amt), G2p = hy * (hc * amt), Gu = hy * (gam * amt);
+                vout[0] = vA[0] - dtdy * amp - dtdy * fap;
+                vout[0] = vout[0] + dtdx * lane_up(G2p) - dtdx * lane_dn(G1p) - dtdx * (G2p - G1p);
+                vout[1] = vA[1] + dtdx * lane_up(Gu) - dtdx * lane_dn(Gu);
+                vout[2] = vA[2] - dtdy * amv - dtdy * fav;
+            }
+            // cell J-1: the lower-face terms
+            const double G1p = hy * (-hc * apt), G2p = hy * (hc * apt), Gu = hy * (gam * apt);
+            double w = vB[0] - dtdy * app + dtdy * fap;
+            vB[0] = w + dtdx * lane_up(G2p) - dtdx * lane_dn(G1p) - dtdx * (G2p - G1p);
+            vB[1] = vB[1] + dtdx * lane_up(Gu) - dtdx * lane_dn(Gu);
+            vB[2] = vB[2] - dtdy * apv + dtdy * fav;
+        }
+        {
+            double* dst = store ? qn + (long)(J - 2) * pitch : g_store_sink + lane;
+            const long stride = store ? fs : 64;
+#pragma unroll
+            for (int m = 0; m < 3; m++) dst[m * stride] = vout[m];
+        }
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            vA[m] = vB[m];
+            vB[m] = vC[m];
+        }
+    };
+
+    const int nrow = r1 - r0 + 3;
+    const int jend = r0 - 1 + nrow + (nrow & 1);
+#pragma unroll
+    for (int m = 0; m < 3; m++) g_store_sink[m * 64 + lane] = 0.0;
+    for (int J = r0 - 1; J < jend; J += 2) {
+        row(ysA, ysB, J);
+        row(ysB, ysA, J + 1);
+    }
+    publish_cfl(cfl, a.cfl_bits);
+}
+
 // The work of one wavefront: strip `strip` (64 columns) and row segment `seg`
 // of the grid described by a.
 template <class S, class Cfg>
 __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, const int seg,
                                            const int lane)
 {
+    if constexpr (std::is_same_v<S, Acoustics2<FastMath>> && Cfg::fixed) {
+        acoustics_wave_fast<Cfg>(a, strip, seg, lane);
+        return;
+    }
     using Mth = typename S::Math;
     using Cell = typename S::Cell;
     using Cache = typename S::Cache;

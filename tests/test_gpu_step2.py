@@ -226,6 +226,28 @@ def test_fast_arith_config1_acoustics():
     assert err <= TOL
 
 
+@pytest.mark.parametrize("lim", [4, 2])
+@pytest.mark.parametrize("mthbc", [(3, 1, 2, 2), (1, 3, 3, 1)])
+def test_fast_arith_single_step_acoustics(lim, mthbc):
+    """Fast-arithmetic acoustics step against the reference arithmetic of
+    the oracle (MC: the kernel's dedicated acoustics path; superbee: the
+    generic one), random data on a grid with ragged strips and segments."""
+    mx, my = 137, 101
+    q = smooth_acoustics(mx, my, seed=7 + lim)
+    q[..., 0] += np.random.default_rng(lim).standard_normal(q.shape[:2]) * 0.3
+    method = (1, 2, 2, 0, 0, 0, 0)
+    p = _oracle_problem(SOLVER_ACOUSTICS, mx, my, method, [lim, lim], mthbc=mthbc)
+    q = O.bc2(p, q)
+    dt = 0.2 / max(mx, my)
+    q_o, cfl_o = O.step2(p, q, 1.0 / mx, 1.0 / my, dt)
+    q_g, cfl_g = gpu.step2(SOLVER_ACOUSTICS, gpu.solver_params(SOLVER_ACOUSTICS, 1.0, 4.0), q,
+                           1.0 / mx, 1.0 / my, dt, method, [lim, lim], arith=1)
+    assert cfl_g == cfl_o
+    err = max_rel_err(q_g, q_o)
+    print("fast acoustics", lim, mthbc, err)
+    assert err <= 1e-14
+
+
 @pytest.mark.parametrize("lim", [1, 2, 3, 4])
 @pytest.mark.parametrize("method", [(1, 2, 2, 0, 0, 0, 0), (1, 2, 1, 0, 0, 0, 0),
                                     (1, 1, 0, 0, 0, 0, 0)])
