@@ -33,6 +33,8 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s + %HBM peak; L1 err vs 4x-fine plain run at equal GPU time"
 WORKLOAD = "euler2d_4096x4096_roe_efix_mc_unsplit_transverse_200steps"
 BYTES_PER_CELL_UPDATE = 64  # (2*meqn + maux) * 8 with meqn = 4, maux = 0
+WORKLOAD_A = "acoustics2d_4096x4096_pulse_mc_unsplit_transverse_walls_200steps"
+BYTES_PER_CELL_UPDATE_A = 48  # meqn = 3, maux = 0
 
 
 def profile_traffic(name):
@@ -83,7 +85,7 @@ def _backend(mod):
     return "ref" if mod.ref_available() else "oracle"
 
 
-def cpu_baseline(mx, my, seed, nsteps=1):
+def cpu_baseline(mx, my, seed, nsteps=1, solver="euler"):
     """Time the reference's CPU path (single thread) on `nsteps` steps of the
     same workload; returns (Gcell-updates/s, seconds, sample, kind)."""
     try:
@@ -96,18 +98,24 @@ def cpu_baseline(mx, my, seed, nsteps=1):
     except Exception as e:
         return None, 0.0, f"oracle unavailable: {e}", None
     backend = _backend(O)
-    q, dx, dy = P.euler_quadrants(mx, my, seed=seed)
-    p = O.Problem(4, 3, mx, my, O.SOLVER_EULER, O.solver_params(gamma=1.4, efix=True),
-                  method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4, 4], mthbc=(1, 1, 1, 1))
+    if solver == "acoustics":
+        q, dx, dy = P.acoustics_pulse(mx, my)
+        p = O.Problem(3, 2, mx, my, O.SOLVER_ACOUSTICS, O.solver_params(1.0, 4.0),
+                      method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4], mthbc=(3, 3, 3, 3))
+        dt0, name = 0.9 * dx / 2.0, "acoustics"
+    else:
+        q, dx, dy = P.euler_quadrants(mx, my, seed=seed)
+        p = O.Problem(4, 3, mx, my, O.SOLVER_EULER, O.solver_params(gamma=1.4, efix=True),
+                      method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4, 4], mthbc=(1, 1, 1, 1))
+        dt0, name = 0.9 * dx / 3.0, "Euler"
     with _QuietStdout(O):
         t0 = time.perf_counter()
-        r = O.claw2(p, q, dx, dy, 0.0, 1e9, [0.9 * dx / 3.0, 1e99], [1.0, 0.9], nsteps,
-                    backend=backend)
+        r = O.claw2(p, q, dx, dy, 0.0, 1e9, [dt0, 1e99], [1.0, 0.9], nsteps, backend=backend)
         el = time.perf_counter() - t0
     steps = int(r.nv[1])
     note = REF_NOTE if backend == "ref" else PORT_NOTE
     return ((mx * my * steps) / el / 1e9, el,
-            f"{mx}x{my} Euler, {steps} claw2 step(s), 1 thread; {note}",
+            f"{mx}x{my} {name}, {steps} claw2 step(s), 1 thread; {note}",
             "reference" if backend == "ref" else "port")
 
 
@@ -322,8 +330,10 @@ def bench_batch(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["euler2d", "acoustics3d", "batch2d"], default="euler2d",
-                    help="euler2d: the headline 4096^2 Euler run; acoustics3d: the 512^3 "
+    ap.add_argument("--workload", choices=["euler2d", "acoustics2d", "acoustics3d", "batch2d"],
+                    default="euler2d",
+                    help="euler2d: the headline 4096^2 Euler run; acoustics2d: the 200^2 "
+                         "acoustics pulse problem at 4096^2; acoustics3d: the 512^3 "
                          "variable-coefficient acoustics run; batch2d: 256 uneven 2D problems "
                          "in one persistent kernel")
     ap.add_argument("--nprob", type=int, default=256)
@@ -361,19 +371,31 @@ def main():
         print(json.dumps(bench_batch(args)), flush=True)
         return
 
-    from classic_amd import SOLVER_EULER, gpu
+    from classic_amd import SOLVER_ACOUSTICS, SOLVER_EULER, gpu
     from classic_amd import problems as P
 
     n = args.n
-    q0, dx, dy = P.euler_quadrants(n, n, seed=args.seed)
-    method, mthlim, mthbc = (1, 2, 2, 0, 0, 0, 0), [4, 4, 4], (1, 1, 1, 1)
+    acoustics = args.workload == "acoustics2d"
     arith = 1 if args.arith == "fast" else 0
-    g = gpu.Grid2D(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith)
+    if acoustics:
+        # the pulse problem of the 200^2 acoustics configuration at n^2
+        q0, dx, dy = P.acoustics_pulse(n, n)
+        method, mthlim, mthbc = (1, 2, 2, 0, 0, 0, 0), [4, 4], (3, 3, 3, 3)
+        g = gpu.Grid2D(SOLVER_ACOUSTICS, n, n, [1.0, 4.0], arith=arith)
+        dt0 = 0.9 * dx / 2.0  # c = 2
+        workload, bytes_cell, traffic_name = WORKLOAD_A, BYTES_PER_CELL_UPDATE_A, "acoustics4096"
+        data = "synthetic (the acoustics pulse problem's initial data at this size)"
+    else:
+        q0, dx, dy = P.euler_quadrants(n, n, seed=args.seed)
+        method, mthlim, mthbc = (1, 2, 2, 0, 0, 0, 0), [4, 4, 4], (1, 1, 1, 1)
+        g = gpu.Grid2D(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith)
+        # first dt from the CFL-desired rule of the initial data: the retake
+        # logic corrects it on the first step if it is too large
+        dt0 = 0.9 * dx / 3.0
+        workload, bytes_cell, traffic_name = WORKLOAD, BYTES_PER_CELL_UPDATE, "euler4096"
+        data = "synthetic (seeded four-quadrant Euler initial data)"
     g.set_segment_rows(args.seg_rows)
     g.set_q(q0)
-    # first dt from the CFL-desired rule of the initial data: the retake
-    # logic corrects it on the first step if it is too large
-    dt0 = 0.9 * dx / 3.0
     # warmup: untimed fixed-dt steps on a copy of the initial data
     if args.warmup > 0:
         g.time_steps(dx, dy, 0.25 * dx, method, mthlim, mthbc, args.warmup)
@@ -393,7 +415,7 @@ def main():
     ksteps = max(10, min(args.steps, 50))
     total_ms, kern_ms, _ = g.time_steps(dx, dy, dt_last, method, mthlim, mthbc, ksteps)
     kavg_s = kern_ms / ksteps / 1e3
-    achieved = cells * BYTES_PER_CELL_UPDATE / kavg_s / 1e9
+    achieved = cells * bytes_cell / kavg_s / 1e9
     seg_rows = g.segment_rows()
     # the same kernel time with the other arithmetic, for reference
     g.set_arith(1 - arith)
@@ -411,7 +433,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        v, el, sample, kind = cpu_baseline(n, n, args.seed, args.cpu_steps)
+        # the reference's acoustics step is about 4x cheaper than its Euler
+        # step: four times the steps keep the sample at 10-30 s
+        v, el, sample, kind = cpu_baseline(n, n, args.seed,
+                                           args.cpu_steps * (4 if acoustics else 1),
+                                           "acoustics" if acoustics else "euler")
         if v is not None:
             cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": kind,
                    "sample": f"{sample} ({el:.1f} s)"}
@@ -428,11 +454,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded four-quadrant Euler initial data)",
-        "config": {"workload": WORKLOAD, "mx": n, "my": n, "seed": args.seed,
+        "data": data,
+        "config": {"workload": workload, "mx": n, "my": n, "seed": args.seed,
                    "method": list(method), "mthlim": mthlim, "mthbc": list(mthbc),
                    "cfl_desired": 0.9, "cfl_max": 1.0},
-        "hbm_frac_of_peak": value * BYTES_PER_CELL_UPDATE / HBM_PEAK_GBS,
+        "hbm_frac_of_peak": value * bytes_cell / HBM_PEAK_GBS,
         "arith": args.arith,
         "kernel_ms_avg": kavg_s * 1e3,
         "segment_rows": seg_rows,
@@ -440,7 +466,7 @@ def main():
         "step_overhead_ms_avg": (total_ms - kern_ms) / ksteps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (profile_traffic("euler4096")
+                     "traffic": (profile_traffic(traffic_name)
                                  if n == 4096 and args.arith == "fast" else None)},
         "cpu_baseline": cpu,
     }
