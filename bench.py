@@ -328,14 +328,104 @@ def bench_batch(args):
     }
 
 
+WORKLOAD_L = "learned2d_1024x1024_euler_mlp_limiter_unsplit_transverse_200steps"
+MLP_FLOP_PER_WAVE = 2 * (5 * 32 + 32 * 32 + 32)  # 5 -> 32 -> 32 -> 1 multiply-adds
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_learned(args):
+    """The learned-limiter run: 2D Euler at n^2 (1024^2 by default) with
+    limiter 6 on all three waves (features pass + BF16-MFMA network + step
+    kernel reading the phi tables every step), network weights of the
+    5-32-32-1 architecture from init_params (random, as no trained file is
+    shipped).  value = Gcell-updates/s of the learned run; the roofline is
+    that of the fused pre-pass kernel a learned step runs (features in FP64,
+    then the network on the matrix cores): 2,432 network flop per wave over
+    all interface waves of both directions / its average launch time."""
+    from classic_amd import SOLVER_EULER
+    from classic_amd import learned as LL
+    from classic_amd import problems as P
+
+    n = args.n
+    arith = 1 if args.arith == "fast" else 0
+    q0, dx, dy = P.euler_quadrants(n, n, seed=args.seed)
+    method, mthbc = (1, 2, 2, 0, 0, 0, 0), (1, 1, 1, 1)
+    mth_l, mth_mc = [6, 6, 6], [4, 4, 4]
+    dt0 = 0.9 * dx / 3.0
+    g = LL.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith)
+    g.set_params(LL.init_params(0))
+    g.set_q(q0)
+    if args.warmup > 0:
+        g.time_steps(dx, dy, 0.25 * dx, method, mth_l, mthbc, args.warmup)
+    runs = {}
+    for name, lim in (("mc", mth_mc), ("learned", mth_l)):
+        g.set_q(q0)
+        t0 = time.perf_counter()
+        out = g.claw2(dx, dy, 0.0, 1e30, [dt0, 1e99], [1.0, 0.9], args.steps, method, lim, mthbc)
+        wall = time.perf_counter() - t0
+        steps = int(out["nv"][1])
+        if steps != args.steps:
+            raise SystemExit(f"claw2 ({name}) took {steps} steps, expected {args.steps}")
+        runs[name] = (wall, float(out["dtv"][4]))
+    wall, dt_last = runs["learned"]
+    value = n * n * args.steps / wall / 1e9
+    reps = 20
+    fused_ms, infer_ms = g.time_kernels(dx, dy, dt_last, reps)
+    ksteps = 20
+    _, step_ms, _ = g.time_steps(dx, dy, dt_last, method, mth_l, mthbc, ksteps)
+    _, step_mc_ms, _ = g.time_steps(dx, dy, dt_last, method, mth_mc, mthbc, ksteps)
+    waves = 2 * g.nwaves()
+    g.close()
+    flops = waves * MLP_FLOP_PER_WAVE
+    achieved = flops / (fused_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        # the MC run of the same grid on the reference's CPU path (it has no
+        # learned limiter); 16 x the steps of a 4096^2 sample
+        v, el, sample, kind = cpu_baseline(n, n, args.seed, args.cpu_steps * 16, "euler")
+        if v is not None:
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": kind,
+                   "sample": f"{sample}, MC limiter ({el:.1f} s)"}
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state, bf16 network with f32 accumulation",
+        "data": "synthetic (seeded four-quadrant Euler initial data; random network weights)",
+        "config": {"workload": WORKLOAD_L, "mx": n, "my": n, "seed": args.seed,
+                   "method": list(method), "mthlim": mth_l, "mthbc": list(mthbc),
+                   "mlp": "5-32-32-1 GELU", "cfl_desired": 0.9, "cfl_max": 1.0},
+        "arith": args.arith,
+        "mc_ms_per_step": runs["mc"][0] / args.steps * 1e3,
+        "kernel_ms": {"fused_prepass": fused_ms, "mlp_inference_alone": infer_ms,
+                      "step_with_learned": step_ms / ksteps, "step_mc": step_mc_ms / ksteps},
+        "mlp_waves_per_launch": waves,
+        "mlp_inference_alone_tflops": flops / (infer_ms * 1e-3) / 1e12,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / MFMA_BF16_PEAK_TFLOPS,
+                     "traffic": (profile_traffic("learned1024")
+                                 if n == 1024 and args.arith == "fast" else None)},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["euler2d", "acoustics2d", "acoustics3d", "batch2d"],
+    ap.add_argument("--workload",
+                    choices=["euler2d", "acoustics2d", "acoustics3d", "batch2d", "learned2d"],
                     default="euler2d",
                     help="euler2d: the headline 4096^2 Euler run; acoustics2d: the 200^2 "
                          "acoustics pulse problem at 4096^2; acoustics3d: the 512^3 "
                          "variable-coefficient acoustics run; batch2d: 256 uneven 2D problems "
-                         "in one persistent kernel")
+                         "in one persistent kernel; learned2d: 1024^2 Euler with the MLP "
+                         "limiter")
     ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--n3", type=int, default=512)
     ap.add_argument("--cpu-n3", type=int, default=160)
@@ -343,7 +433,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="accepted steps to time (default 200 for euler2d, 100 for acoustics3d)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--n", type=int, default=None,
+                    help="grid side (default 4096; 1024 for learned2d)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -369,6 +460,11 @@ def main():
         return
     if args.workload == "batch2d":
         print(json.dumps(bench_batch(args)), flush=True)
+        return
+    if args.n is None:
+        args.n = 1024 if args.workload == "learned2d" else 4096
+    if args.workload == "learned2d":
+        print(json.dumps(bench_learned(args)), flush=True)
         return
 
     from classic_amd import SOLVER_ACOUSTICS, SOLVER_EULER, gpu

@@ -60,6 +60,7 @@ _SIGS = {
     "clawgpu_learned_write": ([_V, _i, _V, C.c_long], _i),
     "clawgpu_learned_set_target": ([_V, _D], _i),
     "clawgpu_learned_grad": ([_V, _d, _d, _d, _I, _I, _i, C.POINTER(C.c_float), _D], _i),
+    "clawgpu_learned_time": ([_V, _d, _d, _d, _i, C.POINTER(C.c_float), C.POINTER(C.c_float)], _i),
     "clawgpu_time_steps": ([_V, _d, _d, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
     "clawgpu_copy_calibration": ([C.c_long, _i, _F], _i),
     "clawgpu_batch_create": ([C.POINTER(_V), _i, _i, _I, _I, _I, _D, _I, _D, _D], _i),

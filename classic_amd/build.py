@@ -25,6 +25,9 @@ FLAGS = [
     "-Wno-unused-command-line-argument",
     "-I", os.path.join(ROOT, "include"),
 ]
+# per-source additions: the network's MFMA results stay in VGPRs (no
+# accumulator-register reads after every matrix instruction)
+FILE_FLAGS = {"learned.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 OBJ_DIR = os.path.join(HERE, "_build")
 
 
@@ -37,7 +40,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     os.makedirs(OBJ_DIR, exist_ok=True)
     srcs = sources()
     objs = [os.path.join(OBJ_DIR, os.path.basename(s) + ".o") for s in srcs]
-    cmds = [[HIPCC, *FLAGS, "-c", "-o", o, s] for s, o in zip(srcs, objs)]
+    cmds = [[HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(s), []), "-c", "-o", o, s]
+            for s, o in zip(srcs, objs)]
     jobs = jobs or min(len(cmds), int(os.environ.get("MAX_JOBS", "8")), os.cpu_count() or 1)
     running = []
     for cmd in cmds:

@@ -51,13 +51,6 @@ struct LearnedState {
     bool have_params;
 };
 
-__device__ inline unsigned short f2bf(float f)
-{
-    unsigned u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-    return (unsigned short)(u >> 16);
-}
-
 __device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 
 __device__ inline float gelu_d(float x)
@@ -94,32 +87,19 @@ struct FeatArgs {
     long ntab;
 };
 
-// One thread per (direction, table slot): solves the interface and its two
-// neighbours along the direction, and writes features, theta, gradient
-// coefficients and (as a default) the MC value of phi.
+// Features of the MW waves of interface (i, J) along direction dir: the
+// interface and its two neighbours along the direction are solved with the
+// step kernel's own arithmetic.  f receives five features per wave, th the
+// upwind ratio, nz whether the wave is nonzero; with g given, the gradient
+// coefficients (MW x meqn floats) of the nonzero waves are written there.
 template <class S>
-__global__ void features_kernel(FeatArgs a)
+__device__ inline void slot_features(const FeatArgs& a, int dir, int i, int J,
+                                     float (&f)[S::MW][5], double (&th)[S::MW], bool (&nz)[S::MW],
+                                     float* g)
 {
     using Mth = typename S::Math;
     constexpr int M = S::M, MW = S::MW;
     const Layout2 L = a.L;
-    const int mx = L.mx, my = L.my;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * a.ntab) return;
-    const int dir = (int)(t / a.ntab);
-    const long slot = t % a.ntab;
-    const int i = (int)(slot % (mx + 4)) - 1, J = (int)(slot / (mx + 4)) - 1;
-    const long wbase = ((long)dir * a.ntab + slot) * MW;
-    double* phit = dir == 0 ? a.phix : a.phiy;
-    bool ok = dir == 0 ? (i >= 1 && i <= mx + 1 && J >= 0 && J <= my + 1)
-                       : (J >= 1 && J <= my + 1 && i >= 0 && i <= mx + 1);
-    for (int mw = 0; mw < MW; mw++) {
-        for (int k = 0; k < kNF; k++) a.feat[(wbase + mw) * kNF + k] = 0.0f;
-        for (int m = 0; m < M; m++) a.g[(wbase + mw) * M + m] = 0.0f;
-        a.theta[wbase + mw] = 0.0;
-        phit[slot * MW + mw] = 0.0;
-    }
-    if (!ok) return;
     // cells k-2 .. k+1 along the direction around interface k
     double qs[4][M];
     for (int c = 0; c < 4; c++) {
@@ -150,25 +130,82 @@ __global__ void features_kernel(FeatArgs a)
             dl = dl + w[0][mw][m] * w[1][mw][m];
             dr = dr + w[1][mw][m] * w[2][mw][m];
         }
-        if (wn2 == 0.0) continue;
+        nz[mw] = wn2 != 0.0;
+        th[mw] = 0.0;
+        for (int k = 0; k < 5; k++) f[mw][k] = 0.0f;
+        if (!nz[mw]) continue;
         const double sp = s[1][mw];
-        const double th = Mth::div(sp > 0.0 ? dl : dr, wn2);
+        const double t = Mth::div(sp > 0.0 ? dl : dr, wn2);
         const double td = Mth::div(sp > 0.0 ? dr : dl, wn2);
-        float* f = a.feat + (wbase + mw) * kNF;
-        f[0] = (float)(fmin(fmax(th, -kThetaClip), kThetaClip) / kThetaClip);
-        f[1] = (float)(fmin(fmax(td, -kThetaClip), kThetaClip) / kThetaClip);
-        f[2] = (float)(sp * dtd);
-        f[3] = (float)(sqrt(wn2) / qn);
-        f[4] = (float)(sp / (smax + 1e-300));
-        a.theta[wbase + mw] = th;
-        const double c = dtd * 0.5 * fabs(sp) * (1.0 - fabs(sp) * dtd);
-        for (int m = 0; m < M; m++) a.g[(wbase + mw) * M + m] = (float)(c * w[1][mw][m]);
-        phit[slot * MW + mw] = mc_phi(th);
+        f[mw][0] = (float)(fmin(fmax(t, -kThetaClip), kThetaClip) / kThetaClip);
+        f[mw][1] = (float)(fmin(fmax(td, -kThetaClip), kThetaClip) / kThetaClip);
+        f[mw][2] = (float)(sp * dtd);
+        f[mw][3] = (float)(sqrt(wn2) / qn);
+        f[mw][4] = (float)(sp / (smax + 1e-300));
+        th[mw] = t;
+        if (g) {
+            const double c = dtd * 0.5 * fabs(sp) * (1.0 - fabs(sp) * dtd);
+            for (int m = 0; m < M; m++) g[mw * M + m] = (float)(c * w[1][mw][m]);
+        }
+    }
+}
+
+// Whether table slot (i, J) of direction dir holds an interface of the step.
+__device__ inline bool slot_live(int dir, int i, int J, int mx, int my)
+{
+    return dir == 0 ? (i >= 1 && i <= mx + 1 && J >= 0 && J <= my + 1)
+                    : (J >= 1 && J <= my + 1 && i >= 0 && i <= mx + 1);
+}
+
+// One thread per (direction, table slot): writes features, theta, gradient
+// coefficients and (as a default) the MC value of phi of the slot's waves.
+template <class S>
+__global__ void features_kernel(FeatArgs a)
+{
+    constexpr int M = S::M, MW = S::MW;
+    const int mx = a.L.mx, my = a.L.my;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * a.ntab) return;
+    const int dir = (int)(t / a.ntab);
+    const long slot = t % a.ntab;
+    const int i = (int)(slot % (mx + 4)) - 1, J = (int)(slot / (mx + 4)) - 1;
+    const long wbase = ((long)dir * a.ntab + slot) * MW;
+    double* phit = dir == 0 ? a.phix : a.phiy;
+    for (int mw = 0; mw < MW; mw++) {
+        for (int k = 0; k < kNF; k++) a.feat[(wbase + mw) * kNF + k] = 0.0f;
+        for (int m = 0; m < M; m++) a.g[(wbase + mw) * M + m] = 0.0f;
+        a.theta[wbase + mw] = 0.0;
+        phit[slot * MW + mw] = 0.0;
+    }
+    if (!slot_live(dir, i, J, mx, my)) return;
+    float f[MW][5];
+    double th[MW];
+    bool nz[MW];
+    slot_features<S>(a, dir, i, J, f, th, nz, a.g + wbase * M);
+    for (int mw = 0; mw < MW; mw++) {
+        if (!nz[mw]) continue;
+        float* fo = a.feat + (wbase + mw) * kNF;
+        for (int k = 0; k < 5; k++) fo[k] = f[mw][k];
+        a.theta[wbase + mw] = th[mw];
+        phit[slot * MW + mw] = mc_phi(th[mw]);
     }
 }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2h __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// Four floats to BF16 (round to nearest even) as an MFMA operand, two
+// hardware packed conversions.
+__device__ inline bf16x4 to_bf16x4(float a, float b, float c, float d)
+{
+    const f32x2 lo = {a, b}, hi = {c, d};
+    const u32x2 r = {__builtin_bit_cast(unsigned, __builtin_convertvector(lo, bf16x2h)),
+                     __builtin_bit_cast(unsigned, __builtin_convertvector(hi, bf16x2h))};
+    return __builtin_bit_cast(bf16x4, r);
+}
 
 struct InferArgs {
     const float* feat;
@@ -180,68 +217,169 @@ struct InferArgs {
     long ntiles;
 };
 
-// BF16-MFMA inference, 16 waves per wavefront iteration; see the file head
-// for the layout.  Lanes 0..15 write the 16 results of a tile.
+// GELU of the matrix-core network: the tanh form 0.5 x (1 + tanh(u)),
+// u = sqrt(2/pi) (x + 0.044715 x^3), written as x / (1 + 2^(-2 u log2 e))
+// with the hardware exp2 and reciprocal.  It differs from the erf form of the
+// FP32 network by less than 5e-4, below the BF16 rounding of the inputs of
+// every layer.  Large negative x gives exp2 = inf and the result -0.
+__device__ inline float gelu_mfma(float x)
+{
+    const float k = 2.30220819f;  // 2 sqrt(2/pi) log2(e)
+    const float z = x * fmaf(0.044715f * k, x * x, k);
+    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z));
+}
+
+// The network's weights as MFMA fragments of one lane (grp = lane/16,
+// col = lane%16): W1^T and W2^T in the A-operand layout (A[row=col][k =
+// 4*grp+j]); the biases of the accumulator rows the lane holds (they
+// initialise the accumulators) and the output weights of those rows.
+struct MlpFrag {
+    bf16x4 w1f[2], w2f[2][2];
+    f32x4 b1v[2], b2v[2];
+    float w3v[2][4], b3;
+};
+
+__device__ inline void mlp_load(MlpFrag& F, const float* P, int grp, int col)
+{
+    for (int t = 0; t < 2; t++) {
+        float v[4];
+        for (int j = 0; j < 4; j++) {
+            int k = 4 * grp + j;  // input feature
+            v[j] = k < 5 ? P[oW1 + k * kH + 16 * t + col] : 0.0f;
+        }
+        F.w1f[t] = to_bf16x4(v[0], v[1], v[2], v[3]);
+        for (int kt = 0; kt < 2; kt++) {
+            for (int j = 0; j < 4; j++) v[j] = P[oW2 + (16 * kt + 4 * grp + j) * kH + 16 * t + col];
+            F.w2f[t][kt] = to_bf16x4(v[0], v[1], v[2], v[3]);
+        }
+        for (int r = 0; r < 4; r++) {
+            int h = 16 * t + 4 * grp + r;
+            F.b1v[t][r] = P[oB1 + h];
+            F.b2v[t][r] = P[oB2 + h];
+            F.w3v[t][r] = P[oW3 + h];
+        }
+    }
+    F.b3 = P[oB3];
+}
+
+// Network output (before the clip) of a tile of 16 waves: x holds features
+// 4*grp .. 4*grp+3 of wave col (zero for grp >= 2); every lane of column col
+// returns wave col's output.
+__device__ inline float mlp_tile(const MlpFrag& F, bf16x4 x)
+{
+    float h1[2][4];
+    for (int t = 0; t < 2; t++) {
+        f32x4 z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w1f[t], x, F.b1v[t], 0, 0, 0);
+        for (int r = 0; r < 4; r++) h1[t][r] = gelu_mfma(z[r]);
+    }
+    bf16x4 hb[2];
+    for (int kt = 0; kt < 2; kt++) hb[kt] = to_bf16x4(h1[kt][0], h1[kt][1], h1[kt][2], h1[kt][3]);
+    float part = 0.0f;
+    for (int t = 0; t < 2; t++) {
+        f32x4 z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w2f[t][0], hb[0], F.b2v[t], 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w2f[t][1], hb[1], z, 0, 0, 0);
+        for (int r = 0; r < 4; r++) part += F.w3v[t][r] * gelu_mfma(z[r]);
+    }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    return part + F.b3;
+}
+
+// BF16-MFMA inference over stored features, 16 waves per wavefront
+// iteration; see the file head for the layout.  Lanes 0..15 write the 16
+// results of a tile.
 __global__ __launch_bounds__(256) void mlp_infer_mfma_kernel(InferArgs a)
 {
     const int lane = threadIdx.x & 63;
     const int grp = lane >> 4, col = lane & 15;
     const long wave_id = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const long nwave = (long)gridDim.x * 4;
-    const float* P = a.params;
-    // weight fragments in the A-operand layout: A[row=col][k=4*grp+j]
-    bf16x4 w1f[2], w2f[2][2];
-    float b1v[2][4], b2v[2][4], w3v[2][4];
-    for (int t = 0; t < 2; t++) {
-        for (int j = 0; j < 4; j++) {
-            int k = 4 * grp + j;  // input feature
-            float v = k < 5 ? P[oW1 + k * kH + 16 * t + col] : 0.0f;
-            w1f[t][j] = (short)f2bf(v);
-            for (int kt = 0; kt < 2; kt++)
-                w2f[t][kt][j] = (short)f2bf(P[oW2 + (16 * kt + 4 * grp + j) * kH + 16 * t + col]);
-        }
-        for (int r = 0; r < 4; r++) {
-            int h = 16 * t + 4 * grp + r;
-            b1v[t][r] = P[oB1 + h];
-            b2v[t][r] = P[oB2 + h];
-            w3v[t][r] = P[oW3 + h];
-        }
-    }
-    const float b3 = P[oB3];
+    MlpFrag F;
+    mlp_load(F, a.params, grp, col);
     const long nall = 2 * a.nw;
     for (long tile = wave_id; tile < a.ntiles; tile += nwave) {
         const long n = tile * 16 + col;  // wave handled by this lane's column
         const bool nok = n < nall;
-        bf16x4 x;
+        float v[4];
         for (int j = 0; j < 4; j++) {
             int k = 4 * grp + j;
-            float v = (nok && k < kNF) ? a.feat[n * kNF + k] : 0.0f;
-            x[j] = (short)f2bf(v);
+            v[j] = (nok && k < kNF) ? a.feat[n * kNF + k] : 0.0f;
         }
-        f32x4 h1[2];
-        for (int t = 0; t < 2; t++) {
-            f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-            z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1f[t], x, z, 0, 0, 0);
-            for (int r = 0; r < 4; r++) h1[t][r] = gelu(z[r] + b1v[t][r]);
-        }
-        bf16x4 hb[2];
-        for (int kt = 0; kt < 2; kt++)
-            for (int j = 0; j < 4; j++) hb[kt][j] = (short)f2bf(h1[kt][j]);
-        float part = 0.0f;
-        for (int t = 0; t < 2; t++) {
-            f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-            z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2f[t][0], hb[0], z, 0, 0, 0);
-            z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2f[t][1], hb[1], z, 0, 0, 0);
-            for (int r = 0; r < 4; r++) part += w3v[t][r] * gelu(z[r] + b2v[t][r]);
-        }
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
+        const bf16x4 x = to_bf16x4(v[0], v[1], v[2], v[3]);
+        const float out = mlp_tile(F, x);
         if (grp == 0 && nok) {
             const double th = a.theta[n];
             double* tab = n < a.nw ? a.phix : a.phiy;
             const long idx = n < a.nw ? n : n - a.nw;
             if (a.feat[n * kNF + 3] > 0.0f)  // nonzero wave; others keep the MC default
-                tab[idx] = clip_phi((double)(part + b3), th, nullptr);
+                tab[idx] = clip_phi((double)out, th, nullptr);
+        }
+    }
+}
+
+// The pre-pass of a step with limiter 6 in one kernel: features of every
+// interface wave (as features_kernel) go through LDS straight into the MFMA
+// network, and only the phi tables are written -- the same values as
+// features_kernel followed by mlp_infer_mfma_kernel, without the feature,
+// theta and gradient tables in HBM.  Each block takes 256 table slots per
+// iteration (the trip count is the same for all its threads, so the block
+// barriers are reached by every thread); each wavefront runs its 64 slots'
+// 64*MW waves through the network as 4*MW tiles of 16.
+template <class S>
+__global__ __launch_bounds__(256) void learned_fused_kernel(FeatArgs a, const float* params)
+{
+    constexpr int MW = S::MW, NS = 64 * MW;
+    __shared__ bf16x4 xs[4][NS][2];
+    __shared__ float outs[4][NS];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int grp = lane >> 4, col = lane & 15;
+    const int mx = a.L.mx, my = a.L.my;
+    MlpFrag F;
+    mlp_load(F, params, grp, col);
+    const long nt = 2 * a.ntab;
+    for (long base = (long)blockIdx.x * 256; base < nt; base += (long)gridDim.x * 256) {
+        const long t = base + threadIdx.x;
+        int dir = 0, i = 0, J = 0;
+        long slot = 0;
+        bool live = false;
+        if (t < nt) {
+            dir = (int)(t / a.ntab);
+            slot = t % a.ntab;
+            i = (int)(slot % (mx + 4)) - 1;
+            J = (int)(slot / (mx + 4)) - 1;
+            live = slot_live(dir, i, J, mx, my);
+        }
+        float f[MW][5];
+        double th[MW];
+        bool nz[MW];
+        for (int mw = 0; mw < MW; mw++) {
+            nz[mw] = false;
+            th[mw] = 0.0;
+            for (int k = 0; k < 5; k++) f[mw][k] = 0.0f;
+        }
+        if (live) slot_features<S>(a, dir, i, J, f, th, nz, nullptr);
+        for (int mw = 0; mw < MW; mw++) {
+            xs[wv][lane * MW + mw][0] = to_bf16x4(f[mw][0], f[mw][1], f[mw][2], f[mw][3]);
+            xs[wv][lane * MW + mw][1] = to_bf16x4(f[mw][4], 0.0f, 0.0f, 0.0f);
+        }
+        __syncthreads();
+        for (int T = 0; T < NS / 16; T++) {
+            const int sidx = 16 * T + col;
+            bf16x4 x = {0, 0, 0, 0};
+            if (grp < 2) x = xs[wv][sidx][grp];
+            const float out = mlp_tile(F, x);
+            if (grp == 0) outs[wv][sidx] = out;
+        }
+        __syncthreads();
+        if (t < nt) {
+            double* phit = dir == 0 ? a.phix : a.phiy;
+            for (int mw = 0; mw < MW; mw++) {
+                double v = 0.0;
+                if (live && nz[mw])
+                    v = f[mw][3] > 0.0f ? clip_phi((double)outs[wv][lane * MW + mw], th[mw], nullptr)
+                                        : mc_phi(th[mw]);
+                phit[slot * MW + mw] = v;
+            }
         }
     }
 }
@@ -545,13 +683,46 @@ static int enqueue_phi(clawgpu_ctx* c, LearnedState* s, int mode)
 // Called by the claw2 loop before each step attempt when every wave family
 // uses limiter 6: features of the current q at this dt, then phi by the
 // BF16 MFMA network with the TVD clip / MC fallback.
+// Features and network in one pass (learned_fused_kernel): the phi tables
+// of the step, as enqueue_features followed by enqueue_phi(mode 0).
+static int enqueue_fused(clawgpu_ctx* c, LearnedState* s, double dx, double dy, double dt)
+{
+    if (!s->have_params) return set_err(CLAWGPU_ERR_ARG, "learned limiter parameters not set");
+    FeatArgs a{};
+    a.q = c->dq[c->cur];
+    a.L = c->L;
+    a.dtdx = dt / dx;
+    a.dtdy = dt / dy;
+    a.P = c->P;
+    a.phix = c->phix;
+    a.phiy = c->phiy;
+    a.ntab = s->ntab;
+    long nb = (2 * s->ntab + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    if (c->solver == CLAWGPU_SOLVER_ACOUSTICS) {
+        if (c->arith == ARITH_FAST)
+            hipLaunchKernelGGL(learned_fused_kernel<Acoustics2<FastMath>>, dim3((int)nb), dim3(256), 0,
+                               c->stream, a, (const float*)s->params);
+        else
+            hipLaunchKernelGGL(learned_fused_kernel<Acoustics2<ExactMath>>, dim3((int)nb), dim3(256), 0,
+                               c->stream, a, (const float*)s->params);
+    } else {
+        if (c->arith == ARITH_FAST)
+            hipLaunchKernelGGL(learned_fused_kernel<Euler2<FastMath>>, dim3((int)nb), dim3(256), 0,
+                               c->stream, a, (const float*)s->params);
+        else
+            hipLaunchKernelGGL(learned_fused_kernel<Euler2<ExactMath>>, dim3((int)nb), dim3(256), 0,
+                               c->stream, a, (const float*)s->params);
+    }
+    HIPCHK(c, hipGetLastError());
+    return CLAWGPU_OK;
+}
+
 int learned_prepare(clawgpu_ctx* c, double dx, double dy, double dt)
 {
     LearnedState* s = learned_state(c);
     if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
-    int rc = enqueue_features(c, s, dx, dy, dt);
-    if (rc) return rc;
-    return enqueue_phi(c, s, 0);
+    return enqueue_fused(c, s, dx, dy, dt);
 }
 
 extern "C" int clawgpu_learned_set_params(clawgpu_ctx* c, const float* params, int n)
@@ -582,6 +753,38 @@ extern "C" int clawgpu_learned_phi(clawgpu_ctx* c, int mode)
     int rc = enqueue_phi(c, (LearnedState*)c->learned, mode);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLAWGPU_OK;
+}
+
+// Measurement helper: reps fused pre-passes (the kernel a step with limiter
+// 6 runs), then one feature pass and reps passes of the stand-alone
+// BF16-MFMA inference over its tables, on the current q, timed with HIP
+// events on the context's stream; averages per launch in milliseconds.
+extern "C" int clawgpu_learned_time(clawgpu_ctx* c, double dx, double dy, double dt, int reps,
+                                    float* fused_ms, float* infer_ms)
+{
+    if (!c || reps < 1 || !fused_ms || !infer_ms) return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    LearnedState* s = learned_state(c);
+    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    if (!s->have_params) return set_err(CLAWGPU_ERR_ARG, "learned limiter parameters not set");
+    hipEvent_t ev[4];
+    for (int k = 0; k < 4; k++) HIPCHK(c, hipEventCreate(&ev[k]));
+    int rc = CLAWGPU_OK;
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    for (int r = 0; r < reps && !rc; r++) rc = enqueue_fused(c, s, dx, dy, dt);
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    if (!rc) rc = enqueue_features(c, s, dx, dy, dt);
+    HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    for (int r = 0; r < reps && !rc; r++) rc = enqueue_phi(c, s, 0);
+    HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    HIPCHK(c, hipEventSynchronize(ev[3]));
+    float a = 0.0f, b = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&a, ev[0], ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&b, ev[2], ev[3]));
+    for (int k = 0; k < 4; k++) (void)hipEventDestroy(ev[k]);
+    if (rc) return rc;
+    *fused_ms = a / reps;
+    *infer_ms = b / reps;
     return CLAWGPU_OK;
 }
 

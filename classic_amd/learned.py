@@ -51,6 +51,15 @@ class LearnedGrid(Grid2D):
     def phi(self, mode):
         check(self.lib.clawgpu_learned_phi(self.h, mode))
 
+    def time_kernels(self, dx, dy, dt, reps):
+        """Average ms per launch of the fused pre-pass of a limiter-6 step
+        (features and network in one kernel) and of the stand-alone MFMA
+        inference over stored features, on the current q (HIP events on the
+        library's stream)."""
+        f, i = C.c_float(0.0), C.c_float(0.0)
+        check(self.lib.clawgpu_learned_time(self.h, dx, dy, dt, reps, C.byref(f), C.byref(i)))
+        return f.value, i.value
+
     def nwaves(self):
         n = C.c_long(0)
         check(self.lib.clawgpu_learned_table_size(self.h, C.byref(n)))

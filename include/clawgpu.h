@@ -200,6 +200,14 @@ int clawgpu_learned_write(clawgpu_ctx* ctx, int what, const void* in, long nbyte
 int clawgpu_learned_set_target(clawgpu_ctx* ctx, const double* q);
 int clawgpu_learned_grad(clawgpu_ctx* ctx, double dx, double dy, double dt, const int* method,
                          const int* mthbc, int mode, float* grad, double* loss);
+/* Measurement helper (no reference counterpart): reps fused pre-passes (the
+ * one kernel a step with limiter 6 runs: features and network, phi tables
+ * only), then one feature pass and reps stand-alone BF16-MFMA inference
+ * passes (phi mode 0) on the current q, timed with HIP events on the
+ * context's stream; *fused_ms and *infer_ms receive the averages per launch.
+ * Leaves the tables of one feature + inference pass. */
+int clawgpu_learned_time(clawgpu_ctx* ctx, double dx, double dy, double dt, int reps,
+                         float* fused_ms, float* infer_ms);
 
 
 /* Benchmark helper (no reference counterpart): reps plain device copies of

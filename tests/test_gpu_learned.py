@@ -84,6 +84,59 @@ def test_inference_fp32_and_mfma_vs_oracle():
     np.testing.assert_array_equal(phibf[~live], phi32[~live])
 
 
+@pytest.mark.parametrize("arith", [0, 1])
+def test_fused_prepass_equals_features_then_inference(arith):
+    """A step with limiter 6 computes its phi tables in one fused kernel
+    (features through LDS into the MFMA network): the tables equal those of
+    the separate feature pass and MFMA inference -- bit for bit in exact
+    arithmetic; in fast arithmetic the compiler may contract the features'
+    FP64 expressions differently in the two kernels, which moves a float
+    feature by an ulp now and then: tables within float rounding, a BF16
+    rounding step at most."""
+    q, dx, dy = _state()
+    n = q.shape[0] - 4
+    dt = 0.2 * dx
+    with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith) as g:
+        g.set_params(Lm.init_params(7))
+        g.set_q(q)
+        g.time_steps(dx, dy, dt, METHOD, [6, 6, 6], BC, 1)
+        fused = np.concatenate([g.read(1), g.read(2)])
+        g.set_q(q)
+        g.bc2(BC)
+        g.features(dx, dy, dt)
+        g.phi(0)
+        feat = g.read(0)
+        twopass = np.concatenate([g.read(1), g.read(2)])
+    live = feat[:, 3] > 0
+    assert live.sum() > 1000 and np.any(fused[live] != 0.0)
+    if arith == 0:
+        np.testing.assert_array_equal(fused, twopass)
+    else:
+        d = np.abs(fused - twopass)
+        close = d <= 1e-6 * np.abs(twopass) + 1e-300
+        assert np.mean(close) > 0.999 and d.max() < 5e-2, (np.mean(close), d.max())
+
+
+def test_timed_passes_leave_the_inference_tables():
+    """The measurement entry point runs the fused pre-pass, a feature pass
+    and the MFMA inference; its phi tables equal one feature + inference
+    pass."""
+    q, dx, dy = _state()
+    n = q.shape[0] - 4
+    params = Lm.init_params(7)
+    with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0]) as g:
+        g.set_params(params)
+        g.set_q(q)
+        g.bc2(BC)
+        g.features(dx, dy, 0.2 * dx)
+        g.phi(0)
+        want = np.concatenate([g.read(1), g.read(2)])
+        f_ms, i_ms = g.time_kernels(dx, dy, 0.2 * dx, 3)
+        got = np.concatenate([g.read(1), g.read(2)])
+    assert f_ms > 0.0 and i_ms > 0.0
+    np.testing.assert_array_equal(got, want)
+
+
 def test_mc_regression_gradient_vs_oracle():
     q, dx, dy = _state(n=64)
     n = 64
