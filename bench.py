@@ -163,6 +163,8 @@ def bench3(args):
     q0, aux, _ = P.acoustics3d_pulse(n, seed=args.seed)
     method, mthlim, mthbc = (1, 2, 22, 0, 0, 0, 2), (2, 2), (3, 3, 3, 3, 3, 3)
     g = Grid3D(n, n, n)
+    if args.tile_order >= 0:
+        g.set_tile(4, args.tile_order)
     g.set_aux(aux)
     del aux
     g.set_q(q0)
@@ -429,6 +431,9 @@ def main():
     ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--n3", type=int, default=512)
     ap.add_argument("--cpu-n3", type=int, default=160)
+    ap.add_argument("--tile-order", type=int, default=-1,
+                    help="acoustics3d: tile order of the unsplit kernel (bit 0 XCD-contiguous, "
+                         "bit 1 z fastest; -1: the library's default)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="accepted steps to time (default 200 for euler2d, 100 for acoustics3d)")

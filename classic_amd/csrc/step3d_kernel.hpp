@@ -89,7 +89,8 @@ struct Args3 {
     double dtdx, dtdy, dtdz;
     int m2, m3, m4, fwave, lim0, lim1;
     int ntx, nty, ntz;
-    int xcd;  // 1: give each of the 8 XCDs a contiguous range of tiles
+    int xcd;  // bit 0: each of the 8 XCDs gets a contiguous range of tiles;
+              // bit 1: tile index with z fastest (else x fastest)
 };
 
 struct Mat {
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
         }
     };
     int t = blockIdx.x;
-    if (a.xcd) {
+    if (a.xcd & 1) {
         // blocks are dealt to the XCDs round-robin; renumber so that XCD x
         // works on tiles [x*per, (x+1)*per), neighbours sharing its L2
         const int ntiles = a.ntx * a.nty * a.ntz;
@@ -468,10 +469,20 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
         t = (t & 7) * per + (t >> 3);
         if (t >= ntiles) return;
     }
-    const int tx = t % a.ntx;
-    t /= a.ntx;
-    const int ty = t % a.nty;
-    const int tz = t / a.nty;
+    int tx, ty, tz;
+    if (a.xcd & 2) {
+        // z fastest: the tiles that share a tile's halo planes and rows run
+        // close together in time
+        tz = t % a.ntz;
+        t /= a.ntz;
+        tx = t % a.ntx;
+        ty = t / a.ntx;
+    } else {
+        tx = t % a.ntx;
+        t /= a.ntx;
+        ty = t % a.nty;
+        tz = t / a.nty;
+    }
     const int i = kOut * tx + l - 1, j0 = TY * ty + 1, k0 = TZ * tz + 1;
     const Layout3& L = a.L;
     const long fs = L.fs;

@@ -198,7 +198,10 @@ class Grid3D:
         check(self.lib.clawgpu3_set_aux_table(self.h, int(bool(enable)), C.byref(n)))
         return n.value
 
-    def set_tile(self, tile, xcd_order=True):
+    def set_tile(self, tile, xcd_order=2):
+        """Tile shape (0..4) and tile order of the unsplit kernel: order bit
+        0 gives each XCD a contiguous range of tiles, bit 1 numbers tiles
+        with z fastest (the default, 2)."""
         check(self.lib.clawgpu3_set_tile(self.h, tile, int(xcd_order)))
 
     def time_steps(self, dt, nsteps, method=(1, 2, 22, 0, 0, 0, 2), mthlim=(3, 3),

@@ -350,7 +350,8 @@ int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
 
 /* Tile shape of the unsplit kernel (tuning): 0 = 60x4x4, 1 = 60x4x2,
  * 2 = 60x2x2, 3 = 60x8x2, 4 = 60x2x4 (default) cells per wavefront;
- * xcd_order = 1 hands each XCD a contiguous range of tiles (default 0). */
+ * xcd_order (tile order, default 2) bit 0 hands each XCD a contiguous
+ * range of tiles, bit 1 numbers tiles with z fastest instead of x. */
 int clawgpu3_set_tile(clawgpu3_ctx* ctx, int tile, int xcd_order);
 
 /* Material table (no reference counterpart; same values, fewer bytes):

@@ -81,16 +81,17 @@ def test_step3_matches_oracle(m3, lim, fw):
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("table", [True, False])
 def test_step3_every_tile_shape(tile, table):
-    """Every tile shape of the unsplit kernel (with the material and
-    coefficient tables where they are compiled, and reading aux) against
-    the oracle, on a grid with ragged tiles in every direction."""
+    """Every tile shape and tile order of the unsplit kernel (with the
+    material and coefficient tables where they are compiled, and reading
+    aux) against the oracle, on a grid with ragged tiles in every
+    direction."""
     rng = np.random.default_rng(40 + tile)
     p = O3.Problem3(MX, MY, MZ + 5, mthlim=(2, 2), method=(1, 2, 22, 0, 0, 0, 0))
     q, aux = _blocky_medium(p, rng, 3)
     dt = 0.3 * p.geom.dx / 2.0
     ref, cfl_ref = O3.step3(p, q, aux, dt)
     with _grid(p, aux, q) as g:
-        g.set_tile(tile, tile % 2 == 0)
+        g.set_tile(tile, (tile + int(table)) % 4)  # every tile order occurs
         g.set_aux_table(table)
         cfl = g.step3(dt, method=p.method, mthlim=(2, 2))
         out = g.get_q()
