@@ -135,8 +135,8 @@ __device__ inline void slot_features(const FeatArgs& a, int dir, int i, int J,
         for (int k = 0; k < 5; k++) f[mw][k] = 0.0f;
         if (!nz[mw]) continue;
         const double sp = s[1][mw];
-        const double t = Mth::div(sp > 0.0 ? dl : dr, wn2);
-        const double td = Mth::div(sp > 0.0 ? dr : dl, wn2);
+        const double t = Mth::ratio(sp > 0.0 ? dl : dr, wn2);
+        const double td = Mth::ratio(sp > 0.0 ? dr : dl, wn2);
         f[mw][0] = (float)(fmin(fmax(t, -kThetaClip), kThetaClip) / kThetaClip);
         f[mw][1] = (float)(fmin(fmax(td, -kThetaClip), kThetaClip) / kThetaClip);
         f[mw][2] = (float)(sp * dtd);

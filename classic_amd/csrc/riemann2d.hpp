@@ -31,6 +31,7 @@ struct ExactMath {
     static constexpr bool fast = false;
     __device__ static inline double div(double a, double b) { return a / b; }
     __device__ static inline double rcp(double b) { return 1.0 / b; }
+    __device__ static inline double ratio(double a, double b) { return a / b; }
     __device__ static inline double sqrt_(double a) { return sqrt(a); }
 };
 
@@ -50,6 +51,14 @@ struct FastMath {
         double r = rcp(b);
         double q = a * r;
         return fma(fma(-b, q, a), r, q);
+    }
+    // a/b where a few ulps do: the limiter's wave ratio, which only selects
+    // phi(r).  Hardware reciprocal refined by one Newton step.
+    __device__ static inline double ratio(double a, double b)
+    {
+        double r = __builtin_amdgcn_rcp(b);
+        r = fma(r, fma(-b, r, 1.0), r);
+        return a * r;
     }
     // hardware reciprocal square root, one coupled Newton step for sqrt and
     // 1/(2 sqrt), then a residual correction (a > 0; about 1 ulp)

@@ -115,7 +115,7 @@ __device__ __forceinline__ void acoustics_wave_fast(const StepArgs& a, const int
     double cfl = 0.0;
 
     auto phi = [&](double num, double al) {
-        const double r = num * Mth::rcp(al);
+        const double r = Mth::ratio(num, al);
         return al != 0.0 ? philim<Mth>(r, Cfg::lim) : 0.0;
     };
 
@@ -354,7 +354,7 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
                         if (!Cfg::fixed && limw(mw) == kLimTable) {
                             phi = a.phix[phi_index(ic, J, mx, my, mw, MW)];
                         } else {
-                            double r = Mth::div(sx[mw] > 0.0 ? dl : dotr[mw], wn2[mw]);
+                            double r = Mth::ratio(sx[mw] > 0.0 ? dl : dotr[mw], wn2[mw]);
                             phi = philim<Mth>(r, limw(mw));
                         }
 #pragma unroll
@@ -444,7 +444,7 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
                         if (!Cfg::fixed && limw(mw) == kLimTable) {
                             phi = a.phiy[phi_index(ic, J - 1, mx, my, mw, MW)];
                         } else {
-                            double r = Mth::div(yp.s[mw] > 0.0 ? yp.dotl[mw] : yc.dotl[mw], nn);
+                            double r = Mth::ratio(yp.s[mw] > 0.0 ? yp.dotl[mw] : yc.dotl[mw], nn);
                             phi = philim<Mth>(r, limw(mw));
                         }
 #pragma unroll
