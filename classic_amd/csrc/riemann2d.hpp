@@ -53,7 +53,8 @@ struct FastMath {
         return fma(fma(-b, q, a), r, q);
     }
     // a/b where a few ulps do: the limiter's wave ratio, which only selects
-    // phi(r).  Hardware reciprocal refined by one Newton step.
+    // phi(r).  Hardware reciprocal (relative error up to about 5e-8 on
+    // gfx950) refined by one Newton step: within about 11 ulps.
     __device__ static inline double ratio(double a, double b)
     {
         double r = __builtin_amdgcn_rcp(b);
