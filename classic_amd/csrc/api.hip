@@ -537,6 +537,17 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
     };
     auto push = [&]() -> int { return clawgpu_set_q(c, hq.data()); };
     int info = 0;
+    // Without user routines the ghost fill of the next step is enqueued
+    // right behind the Courant-number read-back (into the buffer that becomes
+    // q if the step is accepted), so it runs while the host decides; a
+    // rejected step refills the unchanged q as usual.
+    const bool overlap_bc = !h && !split;
+    bool bc_ready = false;
+    struct Ev {
+        hipEvent_t e = nullptr;
+        ~Ev() { if (e) (void)hipEventDestroy(e); }
+    } ev;
+    if (overlap_bc) HIPCHK(c, hipEventCreateWithFlags(&ev.e, hipEventDisableTiming));
     double t = tstart;
     int maxn = nv[0];
     double dt = dtv[0];
@@ -561,7 +572,9 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
         double told = t;
         if (told + dt > *tend && tstart < *tend) dt = *tend - told;
         for (;;) {
-            if (user_bc) {
+            if (bc_ready) {
+                bc_ready = false;  // filled while the host waited for the last Courant number
+            } else if (user_bc) {
                 if ((rc = pull())) return rc;
                 h->bc2(h->user, c->meqn, c->mbc, c->mx, c->my, xlower, ylower, dx, dy, hq.data(),
                        c->maux, haux, told, dt, mthbc);
@@ -601,8 +614,20 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
                 }
                 rc = enqueue_step(c, dx, dy, dt, method, mthlim, use_fwave);
                 if (rc) return rc;
-                rc = read_cfl(c, &cfl);
-                if (rc) return rc;
+                if (overlap_bc && n < maxn && told + dt < *tend) {
+                    // another step follows if this one is accepted
+                    HIPCHK(c, hipMemcpyAsync(c->hcfl, c->dcfl, sizeof(double), hipMemcpyDeviceToHost,
+                                             c->stream));
+                    HIPCHK(c, hipEventRecord(ev.e, c->stream));
+                    rc = enqueue_bc(c, c->dq[res], mthbc, true);
+                    if (rc) return rc;
+                    HIPCHK(c, hipEventSynchronize(ev.e));
+                    cfl = *c->hcfl;
+                    bc_ready = true;
+                } else {
+                    rc = read_cfl(c, &cfl);
+                    if (rc) return rc;
+                }
             }
             if (!std::isfinite(cfl)) {
                 *info_out = -5;
@@ -615,6 +640,7 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
                 break;
             }
             t = told;  // reject: dq[cur] still holds the step's initial data
+            bc_ready = false;
             if (strang_src) {  // back to the state before the source half step
                 hq = saved;
                 if ((rc = push())) return rc;
