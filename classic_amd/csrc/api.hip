@@ -777,8 +777,9 @@ extern "C" int clawgpu_time_steps(clawgpu_ctx* c, double dx, double dy, double d
 {
     // Fixed-dt stepping timed with HIP events on the library's stream, with
     // no per-step host synchronisation: total_ms spans all launches (ghost
-    // fill + step kernel per step), step_kernel_ms sums event pairs around
-    // each step-kernel launch only.  Used by the benchmark.
+    // fill, the learned limiter's pre-pass with limiter 6, step kernel per
+    // step), step_kernel_ms sums event pairs around each step-kernel launch
+    // only.  Used by the benchmark.
     if (!c || !method || !mthlim || !mthbc || !total_ms || nsteps < 1)
         return set_err(CLAWGPU_ERR_ARG, "bad argument");
     int rc = check_method(c, method, mthlim);
@@ -791,6 +792,10 @@ extern "C" int clawgpu_time_steps(clawgpu_ctx* c, double dx, double dy, double d
     for (int n = 0; n < nsteps; n++) {
         rc = enqueue_bc(c, c->dq[c->cur], mthbc, n == 0);
         if (rc) return rc;
+        if (uses_table(c, mthlim)) {  // the learned limiter's pre-pass, outside the step-kernel pair
+            rc = learned_prepare(c, dx, dy, dt);
+            if (rc) return rc;
+        }
         HIPCHK(c, hipEventRecord(ev[2 + 2 * n], c->stream));
         rc = enqueue_step(c, dx, dy, dt, method, mthlim, use_fwave);
         if (rc) return rc;

@@ -811,8 +811,9 @@ extern "C" int clawgpu_learned_read(clawgpu_ctx* c, int what, void* out, long n)
 
 extern "C" int clawgpu_learned_write(clawgpu_ctx* c, int what, const void* in, long n)
 {
-    // overwrite a table (features 0 or theta 3 or dphi 5) from the host: for
-    // the gradient checks of the training kernels
+    // overwrite a table (features 0, phi 1 / 2, theta 3 or dphi 5) from the
+    // host: for the gradient checks of the training kernels and for tests
+    // that a pass writes every entry
     if (!c || !in) return set_err(CLAWGPU_ERR_ARG, "bad argument");
     LearnedState* s = learned_state(c);
     if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
@@ -820,6 +821,8 @@ extern "C" int clawgpu_learned_write(clawgpu_ctx* c, int what, const void* in, l
     size_t bytes = 0;
     switch (what) {
     case 0: dst = s->feat; bytes = 2 * s->nw * kNF * sizeof(float); break;
+    case 1: dst = c->phix; bytes = s->nw * sizeof(double); break;
+    case 2: dst = c->phiy; bytes = s->nw * sizeof(double); break;
     case 3: dst = s->theta; bytes = 2 * s->nw * sizeof(double); break;
     case 5: dst = s->dphi; bytes = 2 * s->nw * sizeof(float); break;
     default: return set_err(CLAWGPU_ERR_ARG, "table not writable");

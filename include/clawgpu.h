@@ -186,7 +186,7 @@ int clawgpu_solver_probe(clawgpu_ctx* ctx, int ixy, int n, const double* ql, con
  *   phi:      mode 0 BF16-MFMA network, 1 FP32 network, 2 MC;
  *   read/write: tables 0 features (float, 8 per wave), 1 phi x, 2 phi y
  *             (double), 3 theta (double), 4 gradient coefficients (float,
- *             meqn per wave), 5 dL/dphi (float); waves per direction from
+ *             meqn per wave; read only), 5 dL/dphi (float); waves per direction from
  *             table_size (slot (i, J) for i in [-1, mx+2], J in [-1, my+2]);
  *   grad:     parameter gradient (host, 1281 floats) and loss of mode 0
  *             (sum over waves of (out - phi_MC)^2) or mode 1 (one unsplit

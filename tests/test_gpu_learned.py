@@ -98,9 +98,22 @@ def test_fused_prepass_equals_features_then_inference(arith):
     dt = 0.2 * dx
     with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith) as g:
         g.set_params(Lm.init_params(7))
+        # both passes must write every entry: start from tables of NaN
+        nw = g.nwaves()
+        g.write(1, np.full(nw, np.nan))
+        g.write(2, np.full(nw, np.nan))
+        g.set_q(q)
+        g.bc2(BC)
+        g.step2(dx, dy, dt, METHOD, [6, 6, 6])
+        fused = np.concatenate([g.read(1), g.read(2)])
+        # fixed-dt stepping runs the same pre-pass
+        g.write(1, np.full(nw, np.nan))
+        g.write(2, np.full(nw, np.nan))
         g.set_q(q)
         g.time_steps(dx, dy, dt, METHOD, [6, 6, 6], BC, 1)
-        fused = np.concatenate([g.read(1), g.read(2)])
+        timed = np.concatenate([g.read(1), g.read(2)])
+        g.write(1, np.full(nw, np.nan))
+        g.write(2, np.full(nw, np.nan))
         g.set_q(q)
         g.bc2(BC)
         g.features(dx, dy, dt)
@@ -109,6 +122,8 @@ def test_fused_prepass_equals_features_then_inference(arith):
         twopass = np.concatenate([g.read(1), g.read(2)])
     live = feat[:, 3] > 0
     assert live.sum() > 1000 and np.any(fused[live] != 0.0)
+    assert not np.isnan(fused).any() and not np.isnan(twopass).any()
+    np.testing.assert_array_equal(timed, fused)
     if arith == 0:
         np.testing.assert_array_equal(fused, twopass)
     else:
@@ -131,6 +146,8 @@ def test_timed_passes_leave_the_inference_tables():
         g.features(dx, dy, 0.2 * dx)
         g.phi(0)
         want = np.concatenate([g.read(1), g.read(2)])
+        g.write(1, np.full(g.nwaves(), np.nan))
+        g.write(2, np.full(g.nwaves(), np.nan))
         f_ms, i_ms = g.time_kernels(dx, dy, 0.2 * dx, 3)
         got = np.concatenate([g.read(1), g.read(2)])
     assert f_ms > 0.0 and i_ms > 0.0
