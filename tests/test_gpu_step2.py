@@ -172,6 +172,45 @@ def test_periodic_conservation_full_size():
     assert np.all(rel <= 1e-12 * nsteps), rel
 
 
+@pytest.mark.parametrize("arith", [0, 1])
+def test_step2_benchmark_size_windows_match_oracle(arith):
+    """At the benchmark size (4096^2 Euler, extrapolation BCs, MC, unsplit +
+    transverse), one GPU step from an evolved state agrees with the oracle
+    run on 32^2 windows of the same state (a cell's update reads cells
+    within two of it, so a window with two ghost layers reproduces the full
+    step on its interior): bit for bit in exact arithmetic, to 1e-12 in
+    fast.  The windows sit on the quadrant boundaries, where the solution
+    has structure after 20 steps: the centre, the left and right edges (the
+    ragged last strip) and the top edge."""
+    n, w = 4096, 32
+    q, dx, dy = euler_problem(n, n, 0)
+    par = O.solver_params(gamma=1.4, efix=True)
+    full = O.Problem(4, 3, n, n, O.SOLVER_EULER, par, mthlim=[4, 4, 4], mthbc=(1, 1, 1, 1))
+    with gpu.Grid2D(SOLVER_EULER, n, n, [1.4, 1.0], arith=1) as g:
+        g.set_q(q)
+        g.claw2(dx, dy, 0.0, 1e9, [0.9 * dx / 3.0, 1e99], [1.0, 0.9], 20, (1, 2, 2, 0, 0, 0, 0),
+                [4, 4, 4], (1, 1, 1, 1))
+        q20 = g.get_q()
+    qb = O.bc2(full, q20)
+    dt = 0.3 * dx
+    with gpu.Grid2D(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith) as g:
+        g.set_q(qb)
+        g.step2(dx, dy, dt, (1, 2, 2, 0, 0, 0, 0), [4, 4, 4])
+        out = g.get_q()
+    h = n // 2 - w // 2 + 1
+    for i0, j0 in [(h, h), (1, h), (n - w + 1, h), (h, n - w + 1)]:
+        sl = (slice(j0 - 1, j0 + w + 3), slice(i0 - 1, i0 + w + 3))
+        win = O.Problem(4, 3, w, w, O.SOLVER_EULER, par, mthlim=[4, 4, 4], mthbc=(1, 1, 1, 1))
+        ref, _ = O.step2(win, qb[sl], dx, dy, dt)
+        assert np.abs(ref[2:-2, 2:-2] - qb[sl][2:-2, 2:-2]).max() > 1e-6
+        err = max_rel_err(out[sl], ref)
+        print(f"arith {arith} window at ({i0}, {j0}): max rel err {err:.2e}", flush=True)
+        if arith == 0:
+            np.testing.assert_array_equal(out[sl][2:-2, 2:-2], ref[2:-2, 2:-2])
+        else:
+            assert err <= TOL
+
+
 def test_periodic_conservation_matches_oracle():
     """The drift of the totals equals the reference arithmetic's (256^2)."""
     mx = my = 256

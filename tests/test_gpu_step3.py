@@ -113,6 +113,44 @@ def test_step3_larger_grid_many_tiles():
     assert max_rel_err3(out, ref) <= TOL
 
 
+def test_step3_benchmark_size_windows_match_oracle():
+    """At the benchmark size (512^3, the two-layer medium, walls, unsplit
+    method 22, superbee), one GPU step on random data agrees with the oracle
+    run on 24^3 windows of the same state: a cell's update only reads cells
+    within two of it, so a window with two ghost layers taken from the full
+    state reproduces the full step on its interior.  Windows: an interior
+    one, one across the material interface at z = 0, the corner at three
+    walls and the opposite corner (the ragged last strip and tiles)."""
+    n, w, mbc = 512, 24, 2
+    rng = np.random.default_rng(512)
+    full = O3.Problem3(n, n, n, mthbc=(3, 3, 3, 3, 3, 3), mthlim=(2, 2),
+                       method=(1, 2, 22, 0, 0, 0, 0))
+    q = O3.bc3(full, rng.standard_normal(full.shape()))
+    aux = P.layered_medium_aux(n, n, n, mbc=mbc, below=(1.0, 1.0), above=(6.0, 2.0))
+    dx = 2.0 / n
+    dt = 0.4 * dx / 2.0
+    with Grid3D(n, n, n) as g:
+        g.set_aux(aux)
+        g.set_q(q)
+        g.step3(dt, method=(1, 2, 22, 0, 0, 0, 0), mthlim=(2, 2))
+        out = g.get_q()
+    print("512^3 step done", flush=True)
+    corners = [(201, 137, 310), (250, 300, n // 2 - w // 2 + 1), (1, 1, 1),
+               (n - w + 1, n - w + 1, n - w + 1)]
+    for i0, j0, k0 in corners:
+        # Fortran cell k lives at array index k + mbc - 1
+        sl = tuple(slice(c - 1, c + w + 3) for c in (k0, j0, i0))
+        lower = tuple(-1.0 + (c - 1) * dx for c in (i0, j0, k0))
+        upper = tuple(lo + w * dx for lo in lower)
+        win = O3.Problem3(w, w, w, mthbc=(3, 3, 3, 3, 3, 3), mthlim=(2, 2),
+                          method=(1, 2, 22, 0, 0, 0, 0), lower=lower, upper=upper)
+        ref, _ = O3.step3(win, q[sl], aux[sl], dt)
+        err = max_rel_err3(out[sl], ref)
+        print(f"window at ({i0}, {j0}, {k0}): max rel err {err:.2e}", flush=True)
+        assert np.abs(ref[2:-2, 2:-2, 2:-2] - q[sl][2:-2, 2:-2, 2:-2]).max() > 1e-3
+        assert err <= TOL
+
+
 def test_heterogeneous_example_frames_and_regression_sums():
     gold = np.load(os.path.join(GOLD, "hetero3d_frames_20.npz"))
     p = O3.Problem3(20, 20, 20, mthbc=(1, 1, 1, 1, 1, 0), mthlim=(3, 3))
