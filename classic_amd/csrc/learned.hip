@@ -65,6 +65,13 @@ __device__ inline double mc_phi(double th)
     return fmax(0.0, fmin(c, fmin(2.0, 2.0 * th)));
 }
 
+// superbee, max(0, min(1, 2 th), min(2, th)): the upper edge of the
+// second-order TVD region (src/2d/inlinelimiter.f90:78-79)
+__device__ inline double superbee_phi(double th)
+{
+    return fmax(0.0, fmax(fmin(1.0, 2.0 * th), fmin(2.0, th)));
+}
+
 // TVD clip with MC fallback
 __device__ inline double clip_phi(double raw, double th, bool* inside)
 {
@@ -416,7 +423,8 @@ struct TrainArgs {
     double* phix;  // mode 2: write phi tables from the FP32 forward
     double* phiy;
     long nw;
-    int mode;      // 0: regress MC (loss (out - MC)^2); 1: backprop dphi; 2: forward only
+    int mode;      // 0: regress a classical phi (loss (out - phi)^2); 1: backprop dphi; 2: forward only
+    int target;    // mode 0: 0 MC, 1 superbee
     float scale;   // 1 / number of samples (mode 0)
 };
 
@@ -452,7 +460,7 @@ __global__ __launch_bounds__(kTB) void mlp_train_kernel(TrainArgs a)
     if (live) {
         const double th = a.theta[n];
         if (a.mode == 0) {
-            double e = (double)out - mc_phi(th);
+            double e = (double)out - (a.target ? superbee_phi(th) : mc_phi(th));
             loss = e * e * a.scale;
             d3 = (float)(2.0 * e) * a.scale;
         } else if (a.mode == 1) {
@@ -525,11 +533,13 @@ struct DphiArgs {
     long ntab;
     double* lossacc;
     double scale;  // 1 / (mx * my)
+    int l1;        // 0: squared error per cell; 1: absolute error per cell and field
 };
 
 // dL/dphi of the one-step loss L = scale * sum over interior cells of
-// |qnew - q*|^2, through the normal correction flux: a wave of x-interface
-// i adds +g to cell (i, J) and -g to cell (i-1, J) per unit phi.
+// |qnew - q*|^2 (or, with l1, of sum_m |qnew_m - q*_m|), through the normal
+// correction flux: a wave of x-interface i adds +g to cell (i, J) and -g to
+// cell (i-1, J) per unit phi.
 __global__ void dphi_kernel(DphiArgs a)
 {
     const Layout2 L = a.L;
@@ -549,15 +559,23 @@ __global__ void dphi_kernel(DphiArgs a)
     for (int mw = 0; mw < MW; mw++) {
         const long w = ((long)dir * a.ntab + slot) * MW + mw;
         double d = 0.0;
-        for (int m = 0; m < M; m++) d += (double)a.g[w * M + m] * (res(ir, jr, m) - res(il, jl, m));
-        a.dphi[w] = (float)(2.0 * a.scale * d);
+        if (a.l1) {
+            auto sgn = [](double e) { return e > 0.0 ? 1.0 : (e < 0.0 ? -1.0 : 0.0); };
+            for (int m = 0; m < M; m++)
+                d += (double)a.g[w * M + m] * (sgn(res(ir, jr, m)) - sgn(res(il, jl, m)));
+            a.dphi[w] = (float)(a.scale * d);
+        } else {
+            for (int m = 0; m < M; m++)
+                d += (double)a.g[w * M + m] * (res(ir, jr, m) - res(il, jl, m));
+            a.dphi[w] = (float)(2.0 * a.scale * d);
+        }
     }
     // loss: one thread per interior cell (dir 0 slots with i, J interior)
     if (dir == 0 && i >= 1 && i <= mx && J >= 1 && J <= my) {
         double l = 0.0;
         for (int m = 0; m < M; m++) {
             double e = res(i, J, m);
-            l += e * e;
+            l += a.l1 ? fabs(e) : e * e;
         }
         atomicAdd(a.lossacc, l * a.scale);
     }
@@ -865,25 +883,27 @@ int enqueue_step_public(clawgpu_ctx* c, double dx, double dy, double dt, const i
 int enqueue_bc_public(clawgpu_ctx* c, const int* mthbc);
 
 // Gradient of a training loss with respect to the network parameters.
-//   mode 0: regression onto MC, L = mean over live waves of (out - phi_MC)^2
+//   mode 0: regression onto MC, L = sum over live waves of (out - phi_MC)^2
+//   mode 2: as mode 0 with superbee in place of MC
 //   mode 1: one-step loss of the unsplit step (method, mthlim ignored: all
 //           waves use the network through the FP32 forward, clipped) from the
 //           device q against the target set by clawgpu_learned_set_target,
 //           L = mean over interior cells of |q_new - q*|^2; the gradient
 //           flows through the normal correction fluxes.
+//   mode 3: as mode 1 with L = mean over interior cells of sum_m |q_new - q*|
 // grad (host, 1281 floats) and loss receive the results; q is unchanged.
 extern "C" int clawgpu_learned_grad(clawgpu_ctx* c, double dx, double dy, double dt,
                                     const int* method, const int* mthbc, int mode, float* grad,
                                     double* loss)
 {
-    if (!c || !grad || !loss || (mode != 0 && mode != 1) || !method || !mthbc)
+    if (!c || !grad || !loss || mode < 0 || mode > 3 || !method || !mthbc)
         return set_err(CLAWGPU_ERR_ARG, "bad argument");
     LearnedState* s = learned_state(c);
     if (!s || !s->have_params) return set_err(CLAWGPU_ERR_ARG, "parameters not set");
     HIPCHK(c, hipMemsetAsync(s->grad, 0, kNParams * sizeof(float), c->stream));
     HIPCHK(c, hipMemsetAsync(s->lossacc, 0, sizeof(double), c->stream));
     int rc;
-    if (mode == 1) {
+    if (mode == 1 || mode == 3) {
         rc = enqueue_bc_public(c, mthbc);
         if (rc) return rc;
         rc = enqueue_features(c, s, dx, dy, dt);
@@ -908,6 +928,7 @@ extern "C" int clawgpu_learned_grad(clawgpu_ctx* c, double dx, double dy, double
         d.ntab = s->ntab;
         d.lossacc = s->lossacc;
         d.scale = 1.0 / ((double)c->mx * c->my);
+        d.l1 = mode == 3;
         int nb = (int)((2 * s->ntab + 127) / 128);
         hipLaunchKernelGGL(dphi_kernel, dim3(nb), dim3(128), 0, c->stream, d);
         HIPCHK(c, hipGetLastError());
@@ -920,8 +941,9 @@ extern "C" int clawgpu_learned_grad(clawgpu_ctx* c, double dx, double dy, double
     a.grad = s->grad;
     a.lossacc = s->lossacc;
     a.nw = s->nw;
-    a.mode = mode;
-    if (mode == 0) {
+    a.mode = mode == 2 ? 0 : (mode == 3 ? 1 : mode);
+    a.target = mode == 2 ? 1 : 0;
+    if (mode != 1) {
         // mean over live waves: count them on the host from theta/features
         a.scale = 1.0f;
     }

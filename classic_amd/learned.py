@@ -12,6 +12,7 @@ measures L1 errors against the fine solution.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import time
 
 import numpy as np
@@ -105,11 +106,14 @@ def coarsen(q, f, mbc=2):
     return out
 
 
-def fine_snapshots(n_coarse, seed, nsnap, factor=4, cfl=0.8, arith=1):
+def fine_snapshots(n_coarse, seed, nsnap, factor=4, cfl=0.8, arith=1, span=None):
     """Run the plain method on the fine grid (factor * n_coarse)^2 with a
-    fixed dt at Courant number `cfl` and return coarsened snapshots
-    [(q_t, q_t+dtc)] separated by `factor` fine steps (one coarse step of
-    dtc = factor * dt_fine), plus the coarse dx and dtc."""
+    fixed dt (`cfl` / 1.6 times the Courant limit of the initial data) and
+    return coarsened snapshots [(q_t, q_t+dtc)] separated by `factor` fine
+    steps (one coarse step of dtc = factor * dt_fine), plus the coarse dx
+    and dtc.  Without `span` the pairs start after 8 coarse steps and are 3
+    coarse steps apart (early times); with `span` (a physical time) they are
+    spread evenly over (0, span]."""
     nf = n_coarse * factor
     q0, dxf, dyf = P.euler_quadrants(nf, nf, seed=seed)
     # max wave speed of the initial data
@@ -117,18 +121,60 @@ def fine_snapshots(n_coarse, seed, nsnap, factor=4, cfl=0.8, arith=1):
     p = 0.4 * (q0[2:-2, 2:-2, 3] - 0.5 * rho * (u * u + v * v))
     smax = (np.sqrt(u * u + v * v) + np.sqrt(1.4 * p / rho)).max()
     dtf = cfl * dxf / (smax * 1.6)
+    warm, gap = 8, 2
+    if span is not None:
+        per = max(1, int(round(span / (nsnap * dtf * factor))))
+        warm, gap = per, per - 1
     out = []
     with Grid2D(SOLVER_EULER, nf, nf, [1.4, 1.0], arith=arith) as g:
         g.set_q(q0)
         # let the waves develop before sampling
-        g.time_steps(dxf, dyf, dtf, METHOD, [4, 4, 4], MTHBC, 8 * factor)
-        for _ in range(nsnap):
+        g.time_steps(dxf, dyf, dtf, METHOD, [4, 4, 4], MTHBC, warm * factor)
+        for k in range(nsnap):
             a = coarsen(g.get_q(), factor)
             g.time_steps(dxf, dyf, dtf, METHOD, [4, 4, 4], MTHBC, factor)
             b = coarsen(g.get_q(), factor)
             out.append((a, b))
-            g.time_steps(dxf, dyf, dtf, METHOD, [4, 4, 4], MTHBC, 2 * factor)
+            if k + 1 < nsnap and gap > 0:
+                g.time_steps(dxf, dyf, dtf, METHOD, [4, 4, 4], MTHBC, gap * factor)
     return out, dxf * factor, dtf * factor
+
+
+def fine_pairs(n_coarse, seed, nsnap, courant, span, factor=4, arith=1):
+    """As fine_snapshots, with the fine dt set from the measured Courant
+    number: after each stretch of fine steps dt is rescaled so the next one
+    runs at `courant`, and the pairs start at span * k / nsnap, k = 1..nsnap.
+    A pair whose own steps exceed Courant number 1 raises.  Returns
+    [(q_t, q_t+dtc, dx_coarse, dtc)]."""
+    nf = n_coarse * factor
+    q0, dxf, dyf = P.euler_quadrants(nf, nf, seed=seed)
+    rho = q0[2:-2, 2:-2, 0]
+    u, v = q0[2:-2, 2:-2, 1] / rho, q0[2:-2, 2:-2, 2] / rho
+    p = 0.4 * (q0[2:-2, 2:-2, 3] - 0.5 * rho * (u * u + v * v))
+    dtf = 0.5 * dxf / (np.sqrt(u * u + v * v) + np.sqrt(1.4 * p / rho)).max()
+    lim = [4, 4, 4]
+    out, t = [], 0.0
+    with Grid2D(SOLVER_EULER, nf, nf, [1.4, 1.0], arith=arith) as g:
+        g.set_q(q0)
+        for k in range(1, nsnap + 1):
+            # advance to the pair's start, rescaling dt after each stretch
+            while True:
+                n = int((span * k / nsnap - t) / dtf)
+                if n <= 0:
+                    break
+                n = min(n, 8 * factor)
+                cfl = g.time_steps(dxf, dyf, dtf, METHOD, lim, MTHBC, n)[2]
+                if not cfl <= 1.0:
+                    raise RuntimeError(f"fine run exceeded Courant number 1 ({cfl})")
+                t += n * dtf
+                dtf *= courant / cfl
+            a = coarsen(g.get_q(), factor)
+            cfl = g.time_steps(dxf, dyf, dtf, METHOD, lim, MTHBC, factor)[2]
+            if not cfl <= 1.0:
+                raise RuntimeError(f"fine run exceeded Courant number 1 ({cfl})")
+            t += factor * dtf
+            out.append((a, coarsen(g.get_q(), factor), dxf * factor, dtf * factor))
+    return out
 
 
 class Adam:
@@ -148,45 +194,144 @@ class Adam:
 
 
 def train(n_coarse=256, seeds=(11, 12, 13), nsnap=4, pre_iters=300, iters=300, lr=2e-3,
-          seed=0, log=None):
-    """Train the network: (1) regression onto MC (the network starts as the
-    plain method), (2) the one-step loss of the unsplit step against
-    4x-coarsened fine solutions.  Returns (params, history)."""
+          seed=0, log=None, cfl=0.8, span=None, courant=None, pretarget="mc", val_seed=None,
+          val_every=25, val_tend=0.1, loss="mse", lr2=None, init=None, es_iters=0,
+          es_pairs=6, es_sigma=0.02, es_lr=0.01, es_tend=0.1):
+    """Train the network: (1) regression onto a classical limiter (pretarget
+    "mc" or "superbee": the network starts as that limiter), (2) the one-step
+    loss of the unsplit step against 4x-coarsened fine solutions
+    (fine_snapshots with cfl, span; fine_pairs when courant is given),
+    squared ("mse") or absolute ("l1") error per cell, learning rate lr2
+    (lr / 2 by default).  init: starting parameters in place of
+    init_params(seed) (for instance a stage-1 result with pre_iters 0).
+    Stage 3 (es_iters > 0) minimises the multi-step error itself: the mean
+    L1 error of n_coarse^2 runs of the training seeds to es_tend against
+    their 4x-fine plain runs, by antithetic evolution strategies (es_pairs
+    Gaussian directions of scale es_sigma per iteration, Adam step es_lr)
+    over the second hidden bias and the output layer; the parameters with
+    the smallest objective are kept.  Returns (params, history)."""
+    params, hist = _train12(n_coarse, seeds, nsnap, pre_iters, iters, lr, seed, log, cfl, span,
+                            courant, pretarget, val_seed, val_every, val_tend, loss, lr2, init)
+    if es_iters > 0:
+        params = _train_es(params.astype(np.float64), n_coarse, seeds, es_iters, es_pairs,
+                           es_sigma, es_lr, es_tend, seed, hist, log).astype(np.float32)
+    return params, hist
+
+
+# offset of b2 (the second hidden layer's bias) in the parameter vector:
+# W1 (5 x 32), b1, W2 (32 x 32), then b2, W3, b3
+_ES_FROM = 5 * H + H + H * H
+
+
+def _train_es(params, n, seeds, iters, pairs, sigma, lr, tend, seed, hist, log):
+    """Stage 3 of train: antithetic evolution strategies on the multi-step
+    L1 error (Salimans et al. 2017, with an Adam step)."""
+    refs = {}
+    for s in seeds:
+        qf, _, _ = run(4 * n, s, tend, 4)
+        refs[s] = coarsen(qf, 4)
+        del qf
+
+    def objective(p):
+        return float(np.mean([l1_error(run(n, s, tend, 6, p.astype(np.float32))[0], refs[s])
+                              for s in seeds]))
+
+    rng = np.random.default_rng(seed + 7919)
+    idx = np.arange(_ES_FROM, NPARAMS)
+    opt = Adam(len(idx), lr=lr)
+    sub = params[idx].copy()
+    best = (objective(params), params.copy())
+    hist["es"] = [best[0]]
+    for it in range(iters):
+        g = np.zeros(len(idx))
+        for _ in range(pairs):
+            e = rng.standard_normal(len(idx))
+            pp, pm = params.copy(), params.copy()
+            pp[idx] += sigma * e
+            pm[idx] -= sigma * e
+            g += (objective(pp) - objective(pm)) * e
+        sub = opt.step(sub, g / (2 * pairs * sigma))
+        params[idx] = sub
+        f = objective(params)
+        hist["es"].append(f)
+        if f < best[0]:
+            best = (f, params.copy())
+        if log:
+            log(f"es iter {it + 1} objective {f:.5e} (best {best[0]:.5e})")
+    return best[1]
+
+
+def _train12(n_coarse, seeds, nsnap, pre_iters, iters, lr, seed, log, cfl, span, courant,
+             pretarget, val_seed, val_every, val_tend, loss, lr2, init):
+    """Stages 1 and 2 of train.  With val_seed, the network kept is the one
+    (after stage 1 or at any multiple of val_every in stage 2) with the
+    smallest multi-step L1 error of a run of that seed to val_tend against
+    its 4x-fine plain run.  Returns (params, history)."""
     data = []
     for s in seeds:
-        snaps, dx, dtc = fine_snapshots(n_coarse, s, nsnap)
-        data += [(a, b, dx, dtc) for a, b in snaps]
-    params = init_params(seed).astype(np.float64)
-    hist = {"pre": [], "step": [], "step_mc": []}
+        if courant is not None:
+            data += fine_pairs(n_coarse, s, nsnap, courant, span if span else 0.1)
+        else:
+            snaps, dx, dtc = fine_snapshots(n_coarse, s, nsnap, cfl=cfl, span=span)
+            data += [(a, b, dx, dtc) for a, b in snaps]
+    params = (init_params(seed) if init is None else np.asarray(init)).astype(np.float64)
+    hist = {"pre": [], "step": [], "step_mc": [], "val": [], "val_mc": None}
+    premode = {"mc": 0, "superbee": 2}[pretarget]
+    ref = None
+    if val_seed is not None:
+        qf, _, _ = run(4 * n_coarse, val_seed, val_tend, 4)
+        ref = coarsen(qf, 4)
+        del qf
+        hist["val_mc"] = l1_error(run(n_coarse, val_seed, val_tend, 4)[0], ref)
+    best = [np.inf, None]
+
+    def validate(it, p):
+        if ref is None:
+            return
+        l1 = l1_error(run(n_coarse, val_seed, val_tend, 6, p.astype(np.float32))[0], ref)
+        hist["val"].append((it, l1))
+        if l1 < best[0]:
+            best[:] = [l1, p.copy()]
+
     t0 = time.time()
     with LearnedGrid(SOLVER_EULER, n_coarse, n_coarse, [1.4, 1.0], arith=1) as g:
-        # stage 1: MC regression on the features of all snapshots
+        # stage 1: regression onto the classical limiter over the features
+        # of all snapshots (loss per live wave)
         opt = Adam(NPARAMS, lr=lr)
+        nlive = {}
         for it in range(pre_iters):
-            a, b, dx, dtc = data[it % len(data)]
+            k = it % len(data)
+            a, b, dx, dtc = data[k]
             g.set_q(a)
             g.bc2(MTHBC)
             g.features(dx, dx, dtc)
-            nlive = max(1, int((g.read(0)[:, 3] > 0).sum()))
+            if k not in nlive:
+                nlive[k] = max(1, int((g.read(0)[:, 3] > 0).sum()))
             g.set_params(params)
-            gr, loss = g.grad(dx, dx, dtc, 0)
-            params = opt.step(params, gr / nlive)
-            hist["pre"].append(loss / nlive)
+            gr, lv = g.grad(dx, dx, dtc, premode)
+            params = opt.step(params, gr / nlive[k])
+            hist["pre"].append(lv / nlive[k])
+        validate(0, params)
         # baseline one-step loss of MC on every snapshot
         for a, b, dx, dtc in data:
             hist["step_mc"].append(one_step_loss(g, a, b, dx, dtc, None))
         # stage 2: one-step loss
-        opt = Adam(NPARAMS, lr=lr * 0.5)
+        opt = Adam(NPARAMS, lr=lr * 0.5 if lr2 is None else lr2)
+        mode = {"mse": 1, "l1": 3}[loss]
         for it in range(iters):
             a, b, dx, dtc = data[it % len(data)]
             g.set_q(a)
             g.set_target(b)
             g.set_params(params)
-            gr, loss = g.grad(dx, dx, dtc, 1)
+            gr, lv = g.grad(dx, dx, dtc, mode)
             params = opt.step(params, gr)
-            hist["step"].append(loss)
+            hist["step"].append(lv)
             if log and it % 50 == 0:
-                log(f"iter {it} loss {loss:.4e} ({time.time() - t0:.1f}s)")
+                log(f"iter {it} loss {lv:.4e} ({time.time() - t0:.1f}s)")
+            if (it + 1) % val_every == 0:
+                validate(it + 1, params)
+    if best[1] is not None:
+        params = best[1]
     return params.astype(np.float32), hist
 
 
@@ -242,34 +387,57 @@ def evaluate(params, seeds=(200, 201), n=1024, tend=0.1, fine=4):
     return out
 
 
+TRAINED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_limiter_euler1024.npy")
+
+
+def trained_params():
+    """The network shipped with the package: trained by main() with its
+    defaults (written to TRAINED with --save)."""
+    p = np.load(TRAINED, allow_pickle=False)
+    if p.shape != (NPARAMS,) or p.dtype != np.float32:
+        raise ValueError(f"{TRAINED}: expected {NPARAMS} float32 parameters")
+    return p
+
+
 def main():
+    """Train with the recipe of the shipped network (its defaults) and
+    evaluate on held-out seeds: stage 1 regression onto superbee, stage 2
+    the one-step squared error on pairs sampled over (0, 0.1] at Courant
+    number 0.9 with the network kept by the multi-step error of a
+    validation seed, stage 3 evolution strategies on the multi-step error of
+    the training seeds."""
     import argparse
     import json
-    import os
     ap = argparse.ArgumentParser(description="train and evaluate the learned limiter")
     ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--train-seeds", type=int, nargs="+", default=[100, 101, 102, 103])
+    ap.add_argument("--train-seeds", type=int, nargs="+", default=[100, 101, 102])
+    ap.add_argument("--val-seed", type=int, default=150)
     ap.add_argument("--eval-seeds", type=int, nargs="+", default=[200, 201])
-    ap.add_argument("--nsnap", type=int, default=4)
+    ap.add_argument("--nsnap", type=int, default=6)
     ap.add_argument("--pre-iters", type=int, default=300)
-    ap.add_argument("--iters", type=int, default=400)
+    ap.add_argument("--iters", type=int, default=100)
+    ap.add_argument("--es-iters", type=int, default=30)
     ap.add_argument("--tend", type=float, default=0.1)
     ap.add_argument("--params", default=None, help="load parameters instead of training")
     ap.add_argument("--save", default=None)
     args = ap.parse_args()
     if args.params:
-        params = np.load(args.params).astype(np.float32)
+        params = np.load(args.params, allow_pickle=False).astype(np.float32)
         hist = None
     else:
         params, hist = train(args.n, tuple(args.train_seeds), args.nsnap, args.pre_iters,
-                             args.iters, log=lambda m: print(m, flush=True))
+                             args.iters, lr=2e-3, lr2=5e-4, courant=0.9, span=args.tend,
+                             pretarget="superbee", val_seed=args.val_seed, val_every=10,
+                             val_tend=args.tend, es_iters=args.es_iters, es_tend=args.tend,
+                             log=lambda m: print(m, flush=True))
         if args.save:
             np.save(args.save, params)
     res = evaluate(params, tuple(args.eval_seeds), args.n, args.tend)
     rec = {"n": args.n, "results": res}
     if hist:
-        rec["train_loss_first_last"] = [hist["step"][0], hist["step"][-1]]
-        rec["train_mc_loss_mean"] = float(np.mean(hist["step_mc"]))
+        rec["val"] = hist["val"]
+        rec["val_mc"] = hist["val_mc"]
+        rec["es"] = hist.get("es")
     print(json.dumps(rec), flush=True)
 
 

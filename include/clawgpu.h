@@ -189,8 +189,10 @@ int clawgpu_solver_probe(clawgpu_ctx* ctx, int ixy, int n, const double* ql, con
  *             meqn per wave; read only), 5 dL/dphi (float); waves per direction from
  *             table_size (slot (i, J) for i in [-1, mx+2], J in [-1, my+2]);
  *   grad:     parameter gradient (host, 1281 floats) and loss of mode 0
- *             (sum over waves of (out - phi_MC)^2) or mode 1 (one unsplit
- *             step against the target q, mean squared error per cell). */
+ *             (sum over waves of (out - phi_MC)^2), mode 2 (the same with
+ *             superbee's phi in place of MC's), mode 1 (one unsplit
+ *             step against the target q, mean squared error per cell) or
+ *             mode 3 (as mode 1, mean absolute error summed over fields). */
 int clawgpu_learned_set_params(clawgpu_ctx* ctx, const float* params, int n);
 int clawgpu_learned_features(clawgpu_ctx* ctx, double dx, double dy, double dt);
 int clawgpu_learned_phi(clawgpu_ctx* ctx, int mode);

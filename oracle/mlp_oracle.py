@@ -53,6 +53,11 @@ def mc(theta):
     return np.maximum(0.0, np.minimum(np.minimum((1 + theta) / 2, 2.0), 2 * theta))
 
 
+def superbee(theta):
+    """max(0, min(1, 2 theta), min(2, theta)) (src/2d/inlinelimiter.f90:78-79)."""
+    return np.maximum(0.0, np.maximum(np.minimum(1.0, 2 * theta), np.minimum(2.0, theta)))
+
+
 def clip(raw, theta):
     """Sweby TVD region 0 <= phi <= min(2 theta, 2) (phi = 0 if theta <= 0);
     MC wherever the raw value falls outside.  Returns (phi, inside)."""
@@ -75,10 +80,11 @@ def backward(p, X, dout):
     return pack(g)
 
 
-def grad_mc_regression(p, X, theta):
-    """L = sum (out - MC(theta))^2 over the given waves."""
+def grad_mc_regression(p, X, theta, target=mc):
+    """L = sum (out - target(theta))^2 over the given waves (target: mc or
+    superbee)."""
     out, _ = forward(p, X)
-    e = out - mc(theta)
+    e = out - target(theta)
     return backward(p, X, 2 * e), float((e * e).sum())
 
 
