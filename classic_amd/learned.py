@@ -196,7 +196,7 @@ class Adam:
 def train(n_coarse=256, seeds=(11, 12, 13), nsnap=4, pre_iters=300, iters=300, lr=2e-3,
           seed=0, log=None, cfl=0.8, span=None, courant=None, pretarget="mc", val_seed=None,
           val_every=25, val_tend=0.1, loss="mse", lr2=None, init=None, es_iters=0,
-          es_pairs=6, es_sigma=0.02, es_lr=0.01, es_tend=0.1):
+          es_pairs=6, es_sigma=0.02, es_lr=0.01, es_tend=0.1, es_from=None):
     """Train the network: (1) regression onto a classical limiter (pretarget
     "mc" or "superbee": the network starts as that limiter), (2) the one-step
     loss of the unsplit step against 4x-coarsened fine solutions
@@ -208,13 +208,15 @@ def train(n_coarse=256, seeds=(11, 12, 13), nsnap=4, pre_iters=300, iters=300, l
     L1 error of n_coarse^2 runs of the training seeds to es_tend against
     their 4x-fine plain runs, by antithetic evolution strategies (es_pairs
     Gaussian directions of scale es_sigma per iteration, Adam step es_lr)
-    over the second hidden bias and the output layer; the parameters with
-    the smallest objective are kept.  Returns (params, history)."""
+    over the parameters from offset es_from on (default: the second hidden
+    bias and the output layer); the parameters with the smallest objective
+    are kept.  Returns (params, history)."""
     params, hist = _train12(n_coarse, seeds, nsnap, pre_iters, iters, lr, seed, log, cfl, span,
                             courant, pretarget, val_seed, val_every, val_tend, loss, lr2, init)
     if es_iters > 0:
         params = _train_es(params.astype(np.float64), n_coarse, seeds, es_iters, es_pairs,
-                           es_sigma, es_lr, es_tend, seed, hist, log).astype(np.float32)
+                           es_sigma, es_lr, es_tend, seed, hist, log,
+                           _ES_FROM if es_from is None else es_from).astype(np.float32)
     return params, hist
 
 
@@ -223,7 +225,7 @@ def train(n_coarse=256, seeds=(11, 12, 13), nsnap=4, pre_iters=300, iters=300, l
 _ES_FROM = 5 * H + H + H * H
 
 
-def _train_es(params, n, seeds, iters, pairs, sigma, lr, tend, seed, hist, log):
+def _train_es(params, n, seeds, iters, pairs, sigma, lr, tend, seed, hist, log, start):
     """Stage 3 of train: antithetic evolution strategies on the multi-step
     L1 error (Salimans et al. 2017, with an Adam step)."""
     refs = {}
@@ -237,7 +239,7 @@ def _train_es(params, n, seeds, iters, pairs, sigma, lr, tend, seed, hist, log):
                               for s in seeds]))
 
     rng = np.random.default_rng(seed + 7919)
-    idx = np.arange(_ES_FROM, NPARAMS)
+    idx = np.arange(start, NPARAMS)
     opt = Adam(len(idx), lr=lr)
     sub = params[idx].copy()
     best = (objective(params), params.copy())
@@ -268,7 +270,7 @@ def _train12(n_coarse, seeds, nsnap, pre_iters, iters, lr, seed, log, cfl, span,
     smallest multi-step L1 error of a run of that seed to val_tend against
     its 4x-fine plain run.  Returns (params, history)."""
     data = []
-    for s in seeds:
+    for s in (seeds if pre_iters > 0 or iters > 0 else ()):
         if courant is not None:
             data += fine_pairs(n_coarse, s, nsnap, courant, span if span else 0.1)
         else:
