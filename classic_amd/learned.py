@@ -407,7 +407,8 @@ def main():
     the one-step squared error on pairs sampled over (0, 0.1] at Courant
     number 0.9 with the network kept by the multi-step error of a
     validation seed, stage 3 evolution strategies on the multi-step error of
-    the training seeds."""
+    the training seeds over b2, W3 and b3, then (--es2-iters) over W2 as
+    well."""
     import argparse
     import json
     ap = argparse.ArgumentParser(description="train and evaluate the learned limiter")
@@ -419,6 +420,7 @@ def main():
     ap.add_argument("--pre-iters", type=int, default=300)
     ap.add_argument("--iters", type=int, default=100)
     ap.add_argument("--es-iters", type=int, default=30)
+    ap.add_argument("--es2-iters", type=int, default=130)
     ap.add_argument("--tend", type=float, default=0.1)
     ap.add_argument("--params", default=None, help="load parameters instead of training")
     ap.add_argument("--save", default=None)
@@ -432,6 +434,12 @@ def main():
                              pretarget="superbee", val_seed=args.val_seed, val_every=10,
                              val_tend=args.tend, es_iters=args.es_iters, es_tend=args.tend,
                              log=lambda m: print(m, flush=True))
+        if args.es2_iters > 0:
+            params, h2 = train(args.n, tuple(args.train_seeds), 0, 0, 0, init=params,
+                               es_iters=args.es2_iters, es_pairs=8, es_sigma=0.01, es_lr=0.005,
+                               es_tend=args.tend, es_from=5 * H + H,
+                               log=lambda m: print(m, flush=True))
+            hist["es"] = hist["es"] + h2["es"]
         if args.save:
             np.save(args.save, params)
     res = evaluate(params, tuple(args.eval_seeds), args.n, args.tend)
