@@ -343,7 +343,10 @@ def bench_learned(args):
     classic_amd.learned on 4x-coarsened plain 4096^2 runs).  value = Gcell-updates/s of the learned run; the roofline is
     that of the fused pre-pass kernel a learned step runs (features in FP64,
     then the network on the matrix cores): 2,432 network flop per wave over
-    all interface waves of both directions / its average launch time."""
+    all interface waves of both directions / its average launch time.
+    l1_vs_fine_4x: the metric's L1 part -- learned at n^2, MC at n^2 and MC
+    at (2n)^2 against the plain (4n)^2 run on held-out seed 200 to t = 0.1,
+    each with its GPU time."""
     from classic_amd import SOLVER_EULER
     from classic_amd import learned as LL
     from classic_amd import problems as P
@@ -378,6 +381,14 @@ def bench_learned(args):
     _, step_mc_ms, _ = g.time_steps(dx, dy, dt_last, method, mth_mc, mthbc, ksteps)
     waves = 2 * g.nwaves()
     g.close()
+    # second half of the metric: L1 error against the 4x-fine plain run, with
+    # the GPU time of each run (wall time of its claw2 loop, which waits for
+    # the device), on held-out seed 200 to t = 0.1
+    ev = LL.evaluate(LL.trained_params(), (200,), n, 0.1)[0]
+    l1 = {"seed": 200, "tend": 0.1, "reference": f"MC at {4 * n}^2, 4x4 block-averaged",
+          f"learned_{n}": {"l1": ev["l1_learned"], "gpu_s": ev["t_learned"], "steps": ev["steps_learned"]},
+          f"mc_{n}": {"l1": ev["l1_mc"], "gpu_s": ev["t_mc"], "steps": ev["steps_mc"]},
+          f"mc_{2 * n}": {"l1": ev["l1_mc2x"], "gpu_s": ev["t_mc2x"], "steps": ev["steps_mc2x"]}}
     flops = waves * MLP_FLOP_PER_WAVE
     achieved = flops / (fused_ms * 1e-3) / 1e12
     cpu = None
@@ -408,6 +419,7 @@ def bench_learned(args):
         "mc_ms_per_step": runs["mc"][0] / args.steps * 1e3,
         "kernel_ms": {"fused_prepass": fused_ms, "mlp_inference_alone": infer_ms,
                       "step_with_learned": step_ms / ksteps, "step_mc": step_mc_ms / ksteps},
+        "l1_vs_fine_4x": l1,
         "mlp_waves_per_launch": waves,
         "mlp_inference_alone_tflops": flops / (infer_ms * 1e-3) / 1e12,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS,
