@@ -238,12 +238,24 @@ def _train_es(params, n, seeds, iters, pairs, sigma, lr, tend, seed, hist, log, 
         return float(np.mean([l1_error(run(n, s, tend, 6, p.astype(np.float32))[0], refs[s])
                               for s in seeds]))
 
-    rng = np.random.default_rng(seed + 7919)
-    idx = np.arange(start, NPARAMS)
+    best, fs = es_minimise(objective, params, start, iters, pairs, sigma, lr,
+                           np.random.default_rng(seed + 7919), log)
+    hist["es"] = fs
+    return best
+
+
+def es_minimise(objective, params, start, iters, pairs, sigma, lr, rng, log=None):
+    """Antithetic evolution strategies over params[start:]: per iteration,
+    the gradient estimate sum_k (f(p + sigma e_k) - f(p - sigma e_k)) e_k /
+    (2 pairs sigma) over Gaussian directions e_k feeds an Adam step.  Returns
+    the parameters with the smallest objective seen and the objective after
+    each iteration (the starting value first)."""
+    params = np.array(params, dtype=np.float64)
+    idx = np.arange(start, len(params))
     opt = Adam(len(idx), lr=lr)
     sub = params[idx].copy()
     best = (objective(params), params.copy())
-    hist["es"] = [best[0]]
+    fs = [best[0]]
     for it in range(iters):
         g = np.zeros(len(idx))
         for _ in range(pairs):
@@ -255,12 +267,12 @@ def _train_es(params, n, seeds, iters, pairs, sigma, lr, tend, seed, hist, log, 
         sub = opt.step(sub, g / (2 * pairs * sigma))
         params[idx] = sub
         f = objective(params)
-        hist["es"].append(f)
+        fs.append(f)
         if f < best[0]:
             best = (f, params.copy())
         if log:
             log(f"es iter {it + 1} objective {f:.5e} (best {best[0]:.5e})")
-    return best[1]
+    return best[1], fs
 
 
 def _train12(n_coarse, seeds, nsnap, pre_iters, iters, lr, seed, log, cfl, span, courant,

@@ -75,3 +75,22 @@ def test_shipped_network_loads():
     phi, _ = MO.clip(out, th)
     hi = np.where(th > 0, np.minimum(2 * th, 2.0), 0.0)
     assert np.isfinite(out).all() and np.all(phi >= 0) and np.all(phi <= hi + 1e-15)
+
+
+def test_evolution_strategies_step_descends():
+    """The stage-3 optimiser on a known quadratic: it leaves params[:start]
+    alone, its first move points down the gradient, and 200 iterations bring
+    the objective close to the minimum it knows nothing about."""
+    from classic_amd import learned as Lm
+    rng = np.random.default_rng(4)
+    target = rng.standard_normal(12)
+
+    def f(p):
+        return float(((p[2:] - target[2:]) ** 2).sum())
+
+    p0 = np.zeros(12)
+    p0[:2] = [5.0, -3.0]
+    best, fs = Lm.es_minimise(f, p0, 2, 200, 6, 0.05, 0.05, np.random.default_rng(0))
+    np.testing.assert_array_equal(best[:2], p0[:2])
+    assert fs[1] < fs[0]
+    assert min(fs) <= 1e-2 * fs[0] and f(best) == min(fs)
