@@ -420,7 +420,7 @@ def main():
     number 0.9 with the network kept by the multi-step error of a
     validation seed, stage 3 evolution strategies on the multi-step error of
     the training seeds over b2, W3 and b3, then (--es2-iters) over W2 as
-    well."""
+    well, then (--es3-iters) the same over six training seeds."""
     import argparse
     import json
     ap = argparse.ArgumentParser(description="train and evaluate the learned limiter")
@@ -433,6 +433,8 @@ def main():
     ap.add_argument("--iters", type=int, default=100)
     ap.add_argument("--es-iters", type=int, default=30)
     ap.add_argument("--es2-iters", type=int, default=130)
+    ap.add_argument("--es3-seeds", type=int, nargs="+", default=[100, 101, 102, 103, 104, 105])
+    ap.add_argument("--es3-iters", type=int, default=52)
     ap.add_argument("--tend", type=float, default=0.1)
     ap.add_argument("--params", default=None, help="load parameters instead of training")
     ap.add_argument("--save", default=None)
@@ -452,6 +454,12 @@ def main():
                                es_tend=args.tend, es_from=5 * H + H,
                                log=lambda m: print(m, flush=True))
             hist["es"] = hist["es"] + h2["es"]
+        if args.es3_iters > 0:
+            params, h3 = train(args.n, tuple(args.es3_seeds), 0, 0, 0, init=params,
+                               es_iters=args.es3_iters, es_pairs=8, es_sigma=0.01, es_lr=0.005,
+                               es_tend=args.tend, es_from=5 * H + H, seed=2,
+                               log=lambda m: print(m, flush=True))
+            hist["es"] = hist["es"] + h3["es"]
         if args.save:
             np.save(args.save, params)
     res = evaluate(params, tuple(args.eval_seeds), args.n, args.tend)
