@@ -12,8 +12,9 @@ resident in HBM; the timed region is the device-side claw2 loop of exactly
 Prints one JSON line: value = Gcell-updates/s of the whole run; roofline =
 algorithmic HBM bytes (64 B per cell-update: read 4 fields of qold, write 4 of
 qnew) per step-kernel launch / its average duration from HIP events on the
-library's stream; cpu_baseline = the single-thread CPU oracle (same
-arithmetic) on a bounded sample.
+library's stream; cpu_baseline = the CPU oracle (the C restatement of the
+reference, same arithmetic; OpenMP build on the host's threads) on a
+bounded sample of about 10 s.
 """
 from __future__ import annotations
 
@@ -51,53 +52,40 @@ def profile_traffic(name):
     return None
 
 
-REF_NOTE = ("reference Fortran (src/2d or src/3d claw/step/flux/limiter/bc, flang -O2, built "
-            "from the reference sources into oracle/_ref) with the project's Riemann solvers")
-PORT_NOTE = "oracle C restatement compiled -O2"
+PORT_NOTE = ("the oracle's C restatement of the reference (same arithmetic, bit for bit with "
+             "the reference build in the tests), -O3 OpenMP build: each cell updated in the "
+             "serial order by bands of slices")
 
 
-class _QuietStdout:
-    """Send file descriptor 1 to /dev/null (the reference Fortran prints its
-    own messages there) so that stdout carries only the JSON line; the
-    Fortran unit is flushed before the descriptor is restored."""
-
-    def __init__(self, mod=None):
-        self.mod = mod
-
-    def __enter__(self):
-        sys.stdout.flush()
-        self.saved = os.dup(1)
-        self.null = os.open(os.devnull, os.O_WRONLY)
-        os.dup2(self.null, 1)
-        return self
-
-    def __exit__(self, *exc):
-        if self.mod is not None and self.mod.ref_available():
-            self.mod.ref().ref_flush_unit(6)
-        os.dup2(self.saved, 1)
-        os.close(self.saved)
-        os.close(self.null)
+def cpu_threads():
+    """Threads of the CPU baseline: OMP_NUM_THREADS when it is set (16 on the
+    GPU box's share of the host), capped by the CPUs this process may run on."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
-def _backend(mod):
-    """'ref' when the reference build is present (it is built where the
-    reference sources are and travels with the tree), else the oracle."""
-    return "ref" if mod.ref_available() else "oracle"
+def _sample_steps(run, target_s=10.0, cap=64):
+    """run(k) -> (accepted steps, seconds) for k steps from the same initial
+    state; a one-step probe sizes a sample of about target_s seconds, which
+    is what is reported."""
+    _, t1 = run(1)
+    k = int(min(cap, max(1, round(target_s / max(t1, 1e-3)))))
+    return run(k)
 
 
-def cpu_baseline(mx, my, seed, nsteps=1, solver="euler"):
-    """Time the reference's CPU path (single thread) on `nsteps` steps of the
-    same workload; returns (Gcell-updates/s, seconds, sample, kind)."""
+def cpu_baseline(mx, my, seed, nsteps=None, solver="euler"):
+    """Time the oracle's OpenMP build (kind "port") on claw2 steps of the same
+    workload; returns (Gcell-updates/s, seconds, sample, kind, threads)."""
     try:
         from oracle import oracle as O
         from classic_amd import problems as P
+        O.lib_omp(1)
     except Exception as e:  # pragma: no cover
-        return None, 0.0, f"oracle unavailable: {e}", None
-    try:
-        O.lib()
-    except Exception as e:
-        return None, 0.0, f"oracle unavailable: {e}", None
-    backend = _backend(O)
+        return None, 0.0, f"oracle unavailable: {e}", None, 0
+    th = cpu_threads()
     if solver == "acoustics":
         q, dx, dy = P.acoustics_pulse(mx, my)
         p = O.Problem(3, 2, mx, my, O.SOLVER_ACOUSTICS, O.solver_params(1.0, 4.0),
@@ -108,46 +96,50 @@ def cpu_baseline(mx, my, seed, nsteps=1, solver="euler"):
         p = O.Problem(4, 3, mx, my, O.SOLVER_EULER, O.solver_params(gamma=1.4, efix=True),
                       method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4, 4], mthbc=(1, 1, 1, 1))
         dt0, name = 0.9 * dx / 3.0, "Euler"
-    with _QuietStdout(O):
+
+    def run(k):
         t0 = time.perf_counter()
-        r = O.claw2(p, q, dx, dy, 0.0, 1e9, [dt0, 1e99], [1.0, 0.9], nsteps, backend=backend)
-        el = time.perf_counter() - t0
-    steps = int(r.nv[1])
-    note = REF_NOTE if backend == "ref" else PORT_NOTE
+        r = O.claw2(p, q, dx, dy, 0.0, 1e9, [dt0, 1e99], [1.0, 0.9], k, backend="omp", threads=th)
+        return int(r.nv[1]), time.perf_counter() - t0
+
+    steps, el = run(nsteps) if nsteps else _sample_steps(run)
     return ((mx * my * steps) / el / 1e9, el,
-            f"{mx}x{my} {name}, {steps} claw2 step(s), 1 thread; {note}",
-            "reference" if backend == "ref" else "port")
+            f"{mx}x{my} {name}, {steps} claw2 step(s), {th} threads; {PORT_NOTE}", "port", th)
 
 
 WORKLOAD3 = "acoustics3d_512cubed_vc_layered_superbee_method22_walls_100steps"
 BYTES_PER_CELL_UPDATE3 = 80  # (2*meqn + maux) * 8 with meqn = 4, maux = 2
 
 
-def cpu_baseline3(n, seed, nsteps=2):
-    """Time the reference's CPU path (single thread) on nsteps unsplit steps of
-    an n^3 sample of the 3D workload; returns (Gcell-updates/s, s, sample, kind)."""
+def cpu_baseline3(n, seed):
+    """Time the oracle's OpenMP build (kind "port"; the blocks of each pass of
+    the unsplit step on all threads) on unsplit step3 calls of an n^3 sample
+    of the 3D workload; returns (Gcell-updates/s, s, sample, kind, threads)."""
     try:
         from oracle import oracle3 as O3
         from classic_amd import problems as P
-        O3.lib()
+        O3.lib_omp(1)
     except Exception as e:  # pragma: no cover
-        return None, 0.0, f"oracle unavailable: {e}", None
-    backend = _backend(O3)
+        return None, 0.0, f"oracle unavailable: {e}", None, 0
+    th = cpu_threads()
     q, aux, _ = P.acoustics3d_pulse(n, seed=seed)
     p = O3.Problem3(n, n, n, mthbc=(3,) * 6, mthlim=(2, 2))
     q = O3.bc3(p, q)
     dt = 0.9 * p.geom.dx / 2.0
-    el = 0.0
-    with _QuietStdout(O3):
-        for _ in range(nsteps):
+
+    def run(k):
+        qq, el = q, 0.0
+        for _ in range(k):
             t0 = time.perf_counter()
-            q, _ = O3.step3(p, q, aux, dt, backend=backend)
+            qq, _ = O3.step3(p, qq, aux, dt, backend="omp", threads=th)
             el += time.perf_counter() - t0
-            q = O3.bc3(p, q)
-    note = REF_NOTE if backend == "ref" else PORT_NOTE
-    return (nsteps * n ** 3 / el / 1e9, el,
-            f"{n}^3 acoustics3d, {nsteps} unsplit step3 calls, 1 thread; {note}",
-            "reference" if backend == "ref" else "port")
+            qq = O3.bc3(p, qq)
+        return k, el
+
+    steps, el = _sample_steps(run)
+    return (steps * n ** 3 / el / 1e9, el,
+            f"{n}^3 acoustics3d, {steps} unsplit step3 calls, {th} threads; {PORT_NOTE}",
+            "port", th)
 
 
 def bench3(args):
@@ -188,10 +180,10 @@ def bench3(args):
     achieved = cells * BYTES_PER_CELL_UPDATE3 / kavg_s / 1e9
     cpu = None
     if not args.no_cpu:
-        v, el, sample, kind = cpu_baseline3(args.cpu_n3, args.seed)
+        v, el, sample, kind, th = cpu_baseline3(args.cpu_n3, args.seed)
         if v is not None:
-            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": kind,
-                   "sample": f"{sample} ({el:.1f} s)"}
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": th, "kind": kind,
+                   "nproc": os.cpu_count(), "sample": f"{sample} ({el:.1f} s)"}
     return {
         "metric": METRIC, "value": value, "unit": "Gcell-updates/s", "n_gpus": 1,
         "steps": steps, "warmup": args.warmup, "ms_per_step": wall / steps * 1e3,
@@ -214,17 +206,17 @@ def bench3(args):
 WORKLOAD_B = "batch2d_256problems_sides_loguniform_128_1024_euler_acoustics_mc_unsplit_transverse"
 
 
-def cpu_baseline_batch(specs, min_cells=40_000_000):
-    """Time the CPU oracle (single thread, -O2) on one claw2 step of each
+def cpu_baseline_batch(specs, min_cells=200_000_000):
+    """Time the oracle's OpenMP build (kind "port") on one claw2 step of each
     batch problem in turn until min_cells cell updates are done; returns
-    (Gcell-updates/s, s, sample)."""
+    (Gcell-updates/s, s, sample, kind, threads)."""
     try:
         from oracle import oracle as O
         from classic_amd import problems as P
-        O.lib()
+        O.lib_omp(1)
     except Exception as e:  # pragma: no cover
-        return None, 0.0, f"oracle unavailable: {e}", None
-    backend = _backend(O)
+        return None, 0.0, f"oracle unavailable: {e}", None, 0
+    th = cpu_threads()
     done, el, used = 0, 0.0, 0
     for s in specs:
         q, dx, dy = P.batch_problem_q(s)
@@ -234,18 +226,16 @@ def cpu_baseline_batch(specs, min_cells=40_000_000):
         else:
             p = O.Problem(3, 2, s["mx"], s["my"], O.SOLVER_ACOUSTICS, O.solver_params(*s["params"]),
                           method=(1, 2, 2, 0, 0, 0, 0), mthlim=[4, 4], mthbc=s["mthbc"])
-        with _QuietStdout(O):
-            t0 = time.perf_counter()
-            r = O.claw2(p, q, dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 1, backend=backend)
-            el += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = O.claw2(p, q, dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 1, backend="omp", threads=th)
+        el += time.perf_counter() - t0
         done += s["mx"] * s["my"] * int(r.nv[1])
         used += 1
-        if done >= min_cells:
+        if done >= min_cells or el >= 20.0:
             break
-    note = REF_NOTE if backend == "ref" else PORT_NOTE
     return (done / el / 1e9, el, (f"first {used} problems of the batch, 1 claw2 step each "
-                                  f"({done} cell updates), 1 thread; {note}"),
-            "reference" if backend == "ref" else "port")
+                                  f"({done} cell updates), {th} threads; {PORT_NOTE}"),
+            "port", th)
 
 
 def bench_batch(args):
@@ -292,10 +282,10 @@ def bench_batch(args):
     achieved = bytes_iter / kavg_s / 1e9
     cpu = None
     if not args.no_cpu:
-        v, el, sample, kind = cpu_baseline_batch(specs)
+        v, el, sample, kind, th = cpu_baseline_batch(specs)
         if v is not None:
-            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": kind,
-                   "sample": f"{sample} ({el:.1f} s)"}
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": th, "kind": kind,
+                   "nproc": os.cpu_count(), "sample": f"{sample} ({el:.1f} s)"}
     best = min(cands, key=lambda c: c[4])
     worst = max(cands, key=lambda c: c[4])
     return {
@@ -394,11 +384,11 @@ def bench_learned(args):
     cpu = None
     if not args.no_cpu:
         # the MC run of the same grid on the reference's CPU path (it has no
-        # learned limiter); 16 x the steps of a 4096^2 sample
-        v, el, sample, kind = cpu_baseline(n, n, args.seed, args.cpu_steps * 16, "euler")
+        # learned limiter)
+        v, el, sample, kind, th = cpu_baseline(n, n, args.seed, None, "euler")
         if v is not None:
-            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": kind,
-                   "sample": f"{sample}, MC limiter ({el:.1f} s)"}
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": th, "kind": kind,
+                   "nproc": os.cpu_count(), "sample": f"{sample}, MC limiter ({el:.1f} s)"}
     return {
         "metric": METRIC,
         "value": value,
@@ -454,7 +444,8 @@ def main():
                     help="grid side (default 4096; 1024 for learned2d)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="steps of the CPU baseline sample (0: sized to about 10 s)")
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast",
                     help="fast: FMA + reciprocal reuse, held to 1e-12 of the reference by the "
                          "tests; exact: the reference's rounding bit for bit")
@@ -546,14 +537,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        # the reference's acoustics step is about 4x cheaper than its Euler
-        # step: four times the steps keep the sample at 10-30 s
-        v, el, sample, kind = cpu_baseline(n, n, args.seed,
-                                           args.cpu_steps * (4 if acoustics else 1),
-                                           "acoustics" if acoustics else "euler")
+        v, el, sample, kind, th = cpu_baseline(n, n, args.seed, args.cpu_steps or None,
+                                               "acoustics" if acoustics else "euler")
         if v is not None:
-            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": kind,
-                   "sample": f"{sample} ({el:.1f} s)"}
+            cpu = {"value": v, "unit": "Gcell-updates/s", "cores": th, "kind": kind,
+                   "nproc": os.cpu_count(), "sample": f"{sample} ({el:.1f} s)"}
 
     rec = {
         "metric": METRIC,

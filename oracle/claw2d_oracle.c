@@ -12,7 +12,9 @@
  * Functions and the reference code each one restates:
  *   orc_limiter      src/2d/inlinelimiter.f90:25-105
  *   orc_flux2        src/2d/flux2.f90:5-235
- *   orc_step2        src/2d/step2.f90:2-251
+ *   orc_step2        src/2d/step2.f90:2-251 (x-sweeps step2_xslices :92-165,
+ *                    y-sweeps step2_yslices :166-248; orc_step2_par shares
+ *                    them among OpenMP threads by bands, same order per cell)
  *   orc_step2ds      src/2d/step2ds.f90:2-275
  *   orc_dimsp2       src/2d/dimsp2.f:4-98
  *   orc_bc2          src/2d/bc2.f:4-215
@@ -36,12 +38,17 @@
  * Array layout follows the reference: q(meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc),
  * column-major, m fastest.  Macros below take Fortran (1-based m) indices.
  */
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
 
 #include "claw2d_oracle.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ---------------------------------------------------------------- solvers */
 
@@ -583,28 +590,27 @@ static int step_work_alloc(step_work* sw, int maxm, int meqn, int mwaves, int ma
 #define Q2(a, m, i, j) (a)[((size_t)((j) + mbc - 1) * (mx + 2 * mbc) + ((i) + mbc - 1)) * meqn + ((m) - 1)]
 #define A2(a, m, i, j) (a)[((size_t)((j) + mbc - 1) * (mx + 2 * mbc) + ((i) + mbc - 1)) * maux + ((m) - 1)]
 
-/* step2.f90:2-251 (x-sweeps j=0..my+1, then y-sweeps i=0..mx+1, with the
- * reference's per-cell update expressions and their parenthesisation). */
-static int orc_step2_w(int maxm, int meqn, int mwaves, int maux, int mbc, int mx, int my,
-                       const double* qold, double* qnew, const double* aux, double dx,
-                       double dy, double dt, const int* method, const int* mthlim,
-                       double* cfl_out, int use_fwave, orc_rpn2_t rpn2, orc_rpt2_t rpt2,
-                       step_work* sw)
-{
-    const int index_capa = method[5];
-    double cfl = 0.0, cfl1d;
-    double dtdx = dt / dx, dtdy = dt / dy;
-    double *qadd = sw->qadd, *fadd = sw->fadd, *gadd = sw->gadd, *q1d = sw->q1d;
-    double *dtdx1d = sw->dtdx1d, *dtdy1d = sw->dtdy1d;
+/* step2.f90:2-251, the x-sweeps (:92-165): slices j = jlo..jhi, with the
+ * reference's per-cell update expressions and their parenthesisation.  Only
+ * updates that land in rows rlo..rhi are applied (the serial step passes the
+ * whole range); a cell therefore receives the terms of slices j-1, j, j+1 in
+ * that order whatever the band, which keeps the threaded step below equal to
+ * the serial one bit for bit. */
 #define G(a, m, k, i) (a)[(((i) + mbc - 1) * 2 + ((k) - 1)) * meqn + ((m) - 1)]
 #define D1(a, i) (a)[(i) + mbc - 1]
-    if (index_capa == 0)
-        for (int i = 1 - mbc; i <= maxm + mbc; i++) {
-            D1(dtdx1d, i) = dtdx;
-            D1(dtdy1d, i) = dtdy;
-        }
-
-    for (int j = 0; j <= my + 1; j++) {
+#define INB(r, lo, hi) ((r) >= (lo) && (r) <= (hi))
+static int step2_xslices(int maxm, int meqn, int mwaves, int maux, int mbc, int mx, int my,
+                         const double* qold, double* qnew, const double* aux, double dtdx,
+                         double dtdy, const int* method, const int* mthlim, int use_fwave,
+                         orc_rpn2_t rpn2, orc_rpt2_t rpt2, step_work* sw, int jlo, int jhi,
+                         int rlo, int rhi, double* cfl)
+{
+    const int index_capa = method[5];
+    double cfl1d;
+    double *qadd = sw->qadd, *fadd = sw->fadd, *gadd = sw->gadd, *q1d = sw->q1d;
+    double* dtdx1d = sw->dtdx1d;
+    (void)my;
+    for (int j = jlo; j <= jhi; j++) {
         for (int i = 1 - mbc; i <= mx + mbc; i++)
             for (int m = 1; m <= meqn; m++) S1(q1d, m, i) = Q2(qold, m, i, j);
         if (index_capa > 0)
@@ -621,32 +627,51 @@ static int orc_step2_w(int maxm, int meqn, int mwaves, int maux, int mbc, int mx
                       sw->aux3, method, mthlim, qadd, fadd, gadd, &cfl1d, &sw->fw, rpn2, rpt2,
                       use_fwave) != 0)
             return -1;
-        cfl = (cfl1d != cfl1d || cfl != cfl) ? NAN : fmax(cfl, cfl1d);
+        *cfl = (cfl1d != cfl1d || *cfl != *cfl) ? NAN : fmax(*cfl, cfl1d);
+        const int own0 = INB(j, rlo, rhi), ownm = INB(j - 1, rlo, rhi), ownp = INB(j + 1, rlo, rhi);
         if (index_capa == 0) {
             for (int i = 1; i <= mx; i++)
                 for (int m = 1; m <= meqn; m++) {
-                    Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + S1(qadd, m, i) -
-                                        dtdx * (S1(fadd, m, i + 1) - S1(fadd, m, i)) -
-                                        dtdy * (G(gadd, m, 2, i) - G(gadd, m, 1, i));
-                    Q2(qnew, m, i, j - 1) = Q2(qnew, m, i, j - 1) - dtdy * G(gadd, m, 1, i);
-                    Q2(qnew, m, i, j + 1) = Q2(qnew, m, i, j + 1) + dtdy * G(gadd, m, 2, i);
+                    if (own0)
+                        Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + S1(qadd, m, i) -
+                                            dtdx * (S1(fadd, m, i + 1) - S1(fadd, m, i)) -
+                                            dtdy * (G(gadd, m, 2, i) - G(gadd, m, 1, i));
+                    if (ownm) Q2(qnew, m, i, j - 1) = Q2(qnew, m, i, j - 1) - dtdy * G(gadd, m, 1, i);
+                    if (ownp) Q2(qnew, m, i, j + 1) = Q2(qnew, m, i, j + 1) + dtdy * G(gadd, m, 2, i);
                 }
         } else {
             for (int i = 1; i <= mx; i++)
                 for (int m = 1; m <= meqn; m++) {
-                    Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + S1(qadd, m, i) -
-                                        (dtdx * (S1(fadd, m, i + 1) - S1(fadd, m, i)) +
-                                         dtdy * (G(gadd, m, 2, i) - G(gadd, m, 1, i))) /
-                                            A2(aux, index_capa, i, j);
-                    Q2(qnew, m, i, j - 1) = Q2(qnew, m, i, j - 1) -
-                                            dtdy * G(gadd, m, 1, i) / A2(aux, index_capa, i, j - 1);
-                    Q2(qnew, m, i, j + 1) = Q2(qnew, m, i, j + 1) +
-                                            dtdy * G(gadd, m, 2, i) / A2(aux, index_capa, i, j + 1);
+                    if (own0)
+                        Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + S1(qadd, m, i) -
+                                            (dtdx * (S1(fadd, m, i + 1) - S1(fadd, m, i)) +
+                                             dtdy * (G(gadd, m, 2, i) - G(gadd, m, 1, i))) /
+                                                A2(aux, index_capa, i, j);
+                    if (ownm)
+                        Q2(qnew, m, i, j - 1) = Q2(qnew, m, i, j - 1) -
+                                                dtdy * G(gadd, m, 1, i) / A2(aux, index_capa, i, j - 1);
+                    if (ownp)
+                        Q2(qnew, m, i, j + 1) = Q2(qnew, m, i, j + 1) +
+                                                dtdy * G(gadd, m, 2, i) / A2(aux, index_capa, i, j + 1);
                 }
         }
     }
+    return 0;
+}
 
-    for (int i = 0; i <= mx + 1; i++) {
+/* step2.f90:166-248, the y-sweeps: slices i = ilo..ihi, updates applied to
+ * columns clo..chi only (as step2_xslices). */
+static int step2_yslices(int maxm, int meqn, int mwaves, int maux, int mbc, int mx, int my,
+                         const double* qold, double* qnew, const double* aux, double dtdx,
+                         double dtdy, const int* method, const int* mthlim, int use_fwave,
+                         orc_rpn2_t rpn2, orc_rpt2_t rpt2, step_work* sw, int ilo, int ihi,
+                         int clo, int chi, double* cfl)
+{
+    const int index_capa = method[5];
+    double cfl1d;
+    double *qadd = sw->qadd, *fadd = sw->fadd, *gadd = sw->gadd, *q1d = sw->q1d;
+    double* dtdy1d = sw->dtdy1d;
+    for (int i = ilo; i <= ihi; i++) {
         for (int j = 1 - mbc; j <= my + mbc; j++)
             for (int m = 1; m <= meqn; m++) S1(q1d, m, j) = Q2(qold, m, i, j);
         if (index_capa > 0)
@@ -663,35 +688,147 @@ static int orc_step2_w(int maxm, int meqn, int mwaves, int maux, int mbc, int mx
                       sw->aux3, method, mthlim, qadd, fadd, gadd, &cfl1d, &sw->fw, rpn2, rpt2,
                       use_fwave) != 0)
             return -1;
-        cfl = (cfl1d != cfl1d || cfl != cfl) ? NAN : fmax(cfl, cfl1d);
+        *cfl = (cfl1d != cfl1d || *cfl != *cfl) ? NAN : fmax(*cfl, cfl1d);
+        const int own0 = INB(i, clo, chi), ownm = INB(i - 1, clo, chi), ownp = INB(i + 1, clo, chi);
         if (index_capa == 0) {
             for (int j = 1; j <= my; j++)
                 for (int m = 1; m <= meqn; m++) {
-                    Q2(qnew, m, i, j) = Q2(qnew, m, i, j) +
-                                        (S1(qadd, m, j) - dtdy * (S1(fadd, m, j + 1) - S1(fadd, m, j)) -
-                                         dtdx * (G(gadd, m, 2, j) - G(gadd, m, 1, j)));
-                    Q2(qnew, m, i - 1, j) = Q2(qnew, m, i - 1, j) - dtdx * G(gadd, m, 1, j);
-                    Q2(qnew, m, i + 1, j) = Q2(qnew, m, i + 1, j) + dtdx * G(gadd, m, 2, j);
+                    if (own0)
+                        Q2(qnew, m, i, j) = Q2(qnew, m, i, j) +
+                                            (S1(qadd, m, j) - dtdy * (S1(fadd, m, j + 1) - S1(fadd, m, j)) -
+                                             dtdx * (G(gadd, m, 2, j) - G(gadd, m, 1, j)));
+                    if (ownm) Q2(qnew, m, i - 1, j) = Q2(qnew, m, i - 1, j) - dtdx * G(gadd, m, 1, j);
+                    if (ownp) Q2(qnew, m, i + 1, j) = Q2(qnew, m, i + 1, j) + dtdx * G(gadd, m, 2, j);
                 }
         } else {
             for (int j = 1; j <= my; j++)
                 for (int m = 1; m <= meqn; m++) {
-                    Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + S1(qadd, m, j) -
-                                        (dtdy * (S1(fadd, m, j + 1) - S1(fadd, m, j)) +
-                                         dtdx * (G(gadd, m, 2, j) - G(gadd, m, 1, j))) /
-                                            A2(aux, index_capa, i, j);
-                    Q2(qnew, m, i - 1, j) = Q2(qnew, m, i - 1, j) -
-                                            dtdx * G(gadd, m, 1, j) / A2(aux, index_capa, i - 1, j);
-                    Q2(qnew, m, i + 1, j) = Q2(qnew, m, i + 1, j) +
-                                            dtdx * G(gadd, m, 2, j) / A2(aux, index_capa, i + 1, j);
+                    if (own0)
+                        Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + S1(qadd, m, j) -
+                                            (dtdy * (S1(fadd, m, j + 1) - S1(fadd, m, j)) +
+                                             dtdx * (G(gadd, m, 2, j) - G(gadd, m, 1, j))) /
+                                                A2(aux, index_capa, i, j);
+                    if (ownm)
+                        Q2(qnew, m, i - 1, j) = Q2(qnew, m, i - 1, j) -
+                                                dtdx * G(gadd, m, 1, j) / A2(aux, index_capa, i - 1, j);
+                    if (ownp)
+                        Q2(qnew, m, i + 1, j) = Q2(qnew, m, i + 1, j) +
+                                                dtdx * G(gadd, m, 2, j) / A2(aux, index_capa, i + 1, j);
                 }
         }
     }
+    return 0;
+}
+
+/* without a capacity function dtdx1d/dtdy1d are the constant ratios
+ * (step2.f90:80-86) */
+static void step_work_ratios(step_work* sw, int maxm, int mbc, int index_capa, double dtdx,
+                             double dtdy)
+{
+    if (index_capa == 0)
+        for (int i = 1 - mbc; i <= maxm + mbc; i++) {
+            D1(sw->dtdx1d, i) = dtdx;
+            D1(sw->dtdy1d, i) = dtdy;
+        }
+}
+
+static int g_threads = 1;
+
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    g_threads = n < 1 ? 1 : n;
+#else
+    (void)n;
+    g_threads = 1;
+#endif
+    return g_threads;
+}
+
+int orc_threads(void) { return g_threads; }
+
+/* The same step on g_threads OpenMP threads: the x-sweeps are shared out by
+ * bands of rows (rows -1..my+2 receive updates), the y-sweeps by bands of
+ * columns; a thread runs the slices of its band plus one on each side and
+ * applies only the updates that land in its band, so each cell is updated
+ * exactly as in the serial order.  The Courant number is a max over slices
+ * (slices on band edges are counted twice, which a max does not see). */
+static int orc_step2_par(int maxm, int meqn, int mwaves, int maux, int mbc, int mx, int my,
+                         const double* qold, double* qnew, const double* aux, double dx,
+                         double dy, double dt, const int* method, const int* mthlim,
+                         double* cfl_out, int use_fwave, orc_rpn2_t rpn2, orc_rpt2_t rpt2)
+{
+#ifdef _OPENMP
+    const double dtdx = dt / dx, dtdy = dt / dy;
+    double cfl = 0.0;
+    int bad = 0;
+#pragma omp parallel num_threads(g_threads)
+    {
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        step_work sw;
+        int ok = step_work_alloc(&sw, maxm, meqn, mwaves, maux, mbc) == 0;
+        double c = 0.0;
+        if (ok) {
+            step_work_ratios(&sw, maxm, mbc, method[5], dtdx, dtdy);
+            const int nr = my + 4;
+            const int r0 = -1 + (int)((long)nr * t / nt), r1 = -2 + (int)((long)nr * (t + 1) / nt);
+            const int j0 = r0 - 1 > 0 ? r0 - 1 : 0, j1 = r1 + 1 < my + 1 ? r1 + 1 : my + 1;
+            if (r0 <= r1 && step2_xslices(maxm, meqn, mwaves, maux, mbc, mx, my, qold, qnew, aux,
+                                          dtdx, dtdy, method, mthlim, use_fwave, rpn2, rpt2, &sw,
+                                          j0, j1, r0, r1, &c))
+                ok = 0;
+        }
+#pragma omp barrier
+        if (ok) {
+            const int nc = mx + 4;
+            const int c0 = -1 + (int)((long)nc * t / nt), c1 = -2 + (int)((long)nc * (t + 1) / nt);
+            const int i0 = c0 - 1 > 0 ? c0 - 1 : 0, i1 = c1 + 1 < mx + 1 ? c1 + 1 : mx + 1;
+            if (c0 <= c1 && step2_yslices(maxm, meqn, mwaves, maux, mbc, mx, my, qold, qnew, aux,
+                                          dtdx, dtdy, method, mthlim, use_fwave, rpn2, rpt2, &sw,
+                                          i0, i1, c0, c1, &c))
+                ok = 0;
+        }
+#pragma omp critical
+        {
+            cfl = (c != c || cfl != cfl) ? NAN : fmax(cfl, c);
+            if (!ok) bad = 1;
+        }
+        if (sw.block) free(sw.block);
+    }
+    *cfl_out = cfl;
+    return bad ? -1 : 0;
+#else
+    (void)maxm; (void)meqn; (void)mwaves; (void)maux; (void)mbc; (void)mx; (void)my;
+    (void)qold; (void)qnew; (void)aux; (void)dx; (void)dy; (void)dt; (void)method;
+    (void)mthlim; (void)cfl_out; (void)use_fwave; (void)rpn2; (void)rpt2;
+    return -1;
+#endif
+}
+
+static int orc_step2_w(int maxm, int meqn, int mwaves, int maux, int mbc, int mx, int my,
+                       const double* qold, double* qnew, const double* aux, double dx,
+                       double dy, double dt, const int* method, const int* mthlim,
+                       double* cfl_out, int use_fwave, orc_rpn2_t rpn2, orc_rpt2_t rpt2,
+                       step_work* sw)
+{
+    if (g_threads > 1)
+        return orc_step2_par(maxm, meqn, mwaves, maux, mbc, mx, my, qold, qnew, aux, dx, dy, dt,
+                             method, mthlim, cfl_out, use_fwave, rpn2, rpt2);
+    const double dtdx = dt / dx, dtdy = dt / dy;
+    double cfl = 0.0;
+    step_work_ratios(sw, maxm, mbc, method[5], dtdx, dtdy);
+    if (step2_xslices(maxm, meqn, mwaves, maux, mbc, mx, my, qold, qnew, aux, dtdx, dtdy, method,
+                      mthlim, use_fwave, rpn2, rpt2, sw, 0, my + 1, INT_MIN, INT_MAX, &cfl))
+        return -1;
+    if (step2_yslices(maxm, meqn, mwaves, maux, mbc, mx, my, qold, qnew, aux, dtdx, dtdy, method,
+                      mthlim, use_fwave, rpn2, rpt2, sw, 0, mx + 1, INT_MIN, INT_MAX, &cfl))
+        return -1;
     *cfl_out = cfl;
     return 0;
+}
 #undef G
 #undef D1
-}
+#undef INB
 
 /* step2ds.f90:2-275: one direction of a dimensionally split step.  The
  * x-sweep runs over the ghost rows too; the y-sweep over ghost columns. */

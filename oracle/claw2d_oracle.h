@@ -68,6 +68,10 @@ int orc_dimsp2(int meqn, int mwaves, int maux, int mbc, int mx, int my, const do
                const int* method, const int* mthlim, const double* cflv, int use_fwave,
                int solver_id, double* cfl);
 int orc_bc2(int meqn, int mbc, int mx, int my, double* q, const int* mthbc);
+/* OpenMP threads of the unsplit steps (the OpenMP build only; 1 otherwise):
+ * the threaded steps update every cell in the serial order, bit for bit. */
+int orc_set_threads(int n);
+int orc_threads(void);
 int orc_claw2(int meqn, int mwaves, int maux, int mbc, int mx, int my, double* q,
               const double* aux, double dx, double dy, double tstart, double* tend, double* dtv,
               double* cflv, int* nv, const int* method, const int* mthlim, const int* mthbc,

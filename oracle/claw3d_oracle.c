@@ -45,6 +45,10 @@
 #include "claw2d_oracle.h"
 #include "claw3d_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 /* 1D slice arrays, Fortran indices, first index 1-mbc */
 #define S1(a, m, i) (a)[(size_t)((i) + mbc - 1) * meqn + ((m) - 1)]
 #define SW(w, m, mw, i) (w)[((size_t)((i) + mbc - 1) * mwaves + ((mw) - 1)) * meqn + ((m) - 1)]
@@ -607,10 +611,61 @@ static int slice3(int dir, int p1, int p2, const dims3* d, int maxm, int mwaves,
 #undef AT
 }
 
-/* step3.f90:127-866.  Slices are visited in the reference's serial order:
- * for each of the four (offset, offset) passes, 2x2 blocks of slices with
- * both block indices of that parity, the slices inside a block with the
- * outer index slowest. */
+/* One pass of a sweep direction of step3.f90:127-866: the 2x2 blocks of
+ * slices whose block indices (outer ob, inner ib) are oo, oo+2, ... and io,
+ * io+2, ..., the slices inside a block with the outer index slowest; slice3
+ * takes (inner, outer) = (j, k) for x, (i, k) for y, (i, j) for z.  The
+ * increments of a slice reach the slices next to it only, so the blocks of
+ * one pass touch disjoint cells: with orc_threads() > 1 they run on OpenMP
+ * threads (as the reference's own OpenMP loop does, step3.f90:186) and each
+ * cell still receives its terms in the serial order, bit for bit. */
+static int pass3(int dir, int oo, int io, int nbo, int nbin, int pomax, int pimax,
+                 const dims3* d, int maxm, int mwaves, const double* qold, double* qnew,
+                 const double* aux, double dtdx, double dtdy, double dtdz, const int* method,
+                 const int* mthlim, int use_fwave, double* cfl, orc_rpn3_t rpn3,
+                 orc_rpt3_t rpt3, orc_rptt3_t rptt3, step3_work* sw)
+{
+#define BLOCK_SLICES(W, CFL, FAIL)                                                             \
+    for (int po = 2 * ob; po <= (2 * ob + 1 < pomax ? 2 * ob + 1 : pomax); po++)               \
+        for (int pi = 2 * ib; pi <= (2 * ib + 1 < pimax ? 2 * ib + 1 : pimax); pi++)           \
+            if (slice3(dir, pi, po, d, maxm, mwaves, qold, qnew, aux, dtdx, dtdy, dtdz, method, \
+                       mthlim, use_fwave, 0, CFL, rpn3, rpt3, rptt3, W))                        \
+                FAIL;
+#ifdef _OPENMP
+    if (orc_threads() > 1) {
+        int bad = 0;
+#pragma omp parallel num_threads(orc_threads())
+        {
+            step3_work w;
+            int ok = step3_work_alloc(&w, maxm, d->meqn, mwaves, d->maux, d->mbc) == 0;
+            double c = 0.0;
+#pragma omp for collapse(2) schedule(dynamic)
+            for (int ob = oo; ob <= nbo - 1; ob += 2)
+                for (int ib = io; ib <= nbin - 1; ib += 2) {
+                    if (!ok) continue;
+                    BLOCK_SLICES(&w, &c, ok = 0)
+                }
+#pragma omp critical
+            {
+                *cfl = (c != c || *cfl != *cfl) ? NAN : fmax(*cfl, c);
+                if (!ok) bad = 1;
+            }
+            if (w.block) free(w.block);
+        }
+        return bad ? -1 : 0;
+    }
+#endif
+    for (int ob = oo; ob <= nbo - 1; ob += 2)
+        for (int ib = io; ib <= nbin - 1; ib += 2) {
+            BLOCK_SLICES(sw, cfl, return -1)
+        }
+    return 0;
+#undef BLOCK_SLICES
+}
+
+/* step3.f90:127-866: x sweeps (slices (j, k)), y sweeps ((i, k)), z sweeps
+ * ((i, j)), each as four passes over 2x2 blocks of slices in the reference's
+ * serial order. */
 static int orc3_step3_w(const dims3* d, int maxm, int mwaves, const double* qold, double* qnew,
                         const double* aux, double dx, double dy, double dz, double dt,
                         const int* method, const int* mthlim, double* cfl_out, int use_fwave,
@@ -619,36 +674,18 @@ static int orc3_step3_w(const dims3* d, int maxm, int mwaves, const double* qold
     double cfl = 0.0;
     double dtdx = dt / dx, dtdy = dt / dy, dtdz = dt / dz;
     const int nbi = (d->mx + 3) / 2, nbj = (d->my + 3) / 2, nbk = (d->mz + 3) / 2;
-    /* x sweeps: slices (j, k) */
-    for (int ko = 0; ko <= 1; ko++)
-        for (int jo = 0; jo <= 1; jo++)
-            for (int kb = ko; kb <= nbk - 1; kb += 2)
-                for (int jb = jo; jb <= nbj - 1; jb += 2)
-                    for (int k = 2 * kb; k <= (2 * kb + 1 < d->mz + 1 ? 2 * kb + 1 : d->mz + 1); k++)
-                        for (int j = 2 * jb; j <= (2 * jb + 1 < d->my + 1 ? 2 * jb + 1 : d->my + 1); j++)
-                            if (slice3(1, j, k, d, maxm, mwaves, qold, qnew, aux, dtdx, dtdy, dtdz,
-                                       method, mthlim, use_fwave, 0, &cfl, rpn3, rpt3, rptt3, sw))
-                                return -1;
-    /* y sweeps: slices (i, k) */
-    for (int ko = 0; ko <= 1; ko++)
-        for (int io = 0; io <= 1; io++)
-            for (int kb = ko; kb <= nbk - 1; kb += 2)
-                for (int ib = io; ib <= nbi - 1; ib += 2)
-                    for (int k = 2 * kb; k <= (2 * kb + 1 < d->mz + 1 ? 2 * kb + 1 : d->mz + 1); k++)
-                        for (int i = 2 * ib; i <= (2 * ib + 1 < d->mx + 1 ? 2 * ib + 1 : d->mx + 1); i++)
-                            if (slice3(2, i, k, d, maxm, mwaves, qold, qnew, aux, dtdx, dtdy, dtdz,
-                                       method, mthlim, use_fwave, 0, &cfl, rpn3, rpt3, rptt3, sw))
-                                return -1;
-    /* z sweeps: slices (i, j) */
-    for (int jo = 0; jo <= 1; jo++)
-        for (int io = 0; io <= 1; io++)
-            for (int jb = jo; jb <= nbj - 1; jb += 2)
-                for (int ib = io; ib <= nbi - 1; ib += 2)
-                    for (int j = 2 * jb; j <= (2 * jb + 1 < d->my + 1 ? 2 * jb + 1 : d->my + 1); j++)
-                        for (int i = 2 * ib; i <= (2 * ib + 1 < d->mx + 1 ? 2 * ib + 1 : d->mx + 1); i++)
-                            if (slice3(3, i, j, d, maxm, mwaves, qold, qnew, aux, dtdx, dtdy, dtdz,
-                                       method, mthlim, use_fwave, 0, &cfl, rpn3, rpt3, rptt3, sw))
-                                return -1;
+    /* (direction, outer blocks, inner blocks, last outer slice, last inner slice) */
+    const int sweeps[3][5] = {{1, nbk, nbj, d->mz + 1, d->my + 1},
+                              {2, nbk, nbi, d->mz + 1, d->mx + 1},
+                              {3, nbj, nbi, d->my + 1, d->mx + 1}};
+    for (int sdir = 0; sdir < 3; sdir++)
+        for (int oo = 0; oo <= 1; oo++)
+            for (int io = 0; io <= 1; io++)
+                if (pass3(sweeps[sdir][0], oo, io, sweeps[sdir][1], sweeps[sdir][2],
+                          sweeps[sdir][3], sweeps[sdir][4], d, maxm, mwaves, qold, qnew, aux,
+                          dtdx, dtdy, dtdz, method, mthlim, use_fwave, &cfl, rpn3, rpt3, rptt3,
+                          sw))
+                    return -1;
     *cfl_out = cfl;
     return 0;
 }

@@ -21,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "liboracle2d.so")
 REF_SO = os.path.join(HERE, "_ref", "libclawref2d.so")
+ORACLE_OMP_SO = os.path.join(HERE, "_build", "liboracle2d_omp.so")
 
 SOLVER_ACOUSTICS = 1
 SOLVER_EULER = 2
@@ -70,6 +71,27 @@ def lib():
     return _lib
 
 
+_omp = None
+
+
+def lib_omp(threads: int):
+    """The OpenMP build of the same restatement (-O3, threaded unsplit steps
+    that update every cell in the serial order, so results are the plain
+    build's bit for bit), set to `threads` threads; backend "omp" below."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(ORACLE_OMP_SO):
+            build()
+        _omp = C.CDLL(ORACLE_OMP_SO)
+        _omp.orc_set_solver_params.argtypes = [C.POINTER(SolverParams)]
+    _omp.orc_set_threads(int(threads))
+    return _omp
+
+
+def _cpu(backend, threads):
+    return lib() if backend == "oracle" else lib_omp(threads)
+
+
 def ref_available() -> bool:
     return os.path.exists(REF_SO)
 
@@ -114,13 +136,15 @@ def _prep(q):
     return q
 
 
-def step2(p: Problem, qold, dx, dy, dt, aux=None, backend="oracle"):
-    """One unsplit step; returns (qnew, cfl).  qnew starts as a copy of qold."""
+def step2(p: Problem, qold, dx, dy, dt, aux=None, backend="oracle", threads=1):
+    """One unsplit step; returns (qnew, cfl).  qnew starts as a copy of qold.
+    backend: "oracle" (the C restatement), "omp" (its OpenMP build on
+    `threads` threads) or "ref" (the reference build)."""
     qold = _prep(qold)
     qnew = qold.copy()
     cfl = C.c_double(0.0)
-    if backend == "oracle":
-        L = lib()
+    if backend in ("oracle", "omp"):
+        L = _cpu(backend, threads)
         L.orc_set_solver_params(C.byref(p.params))
         rc = L.orc_step2(p.meqn, p.mwaves, p.maux, p.mbc, p.mx, p.my, _dp(qold), _dp(qnew),
                          _aux_ptr(aux), C.c_double(dx), C.c_double(dy), C.c_double(dt),
@@ -178,8 +202,9 @@ class ClawResult:
 
 
 def claw2(p: Problem, q, dx, dy, tstart, tend, dtv, cflv, nmax, aux=None,
-          xlower=0.0, ylower=0.0, backend="oracle", trace_cap=100000):
-    """The claw2 time loop (src/2d/claw2.f:494-784).  Returns ClawResult."""
+          xlower=0.0, ylower=0.0, backend="oracle", trace_cap=100000, threads=1):
+    """The claw2 time loop (src/2d/claw2.f:494-784).  Returns ClawResult.
+    backend as for step2."""
     q = _prep(q).copy()
     te = C.c_double(tend)
     dtv_a = np.zeros(5)
@@ -187,8 +212,8 @@ def claw2(p: Problem, q, dx, dy, tstart, tend, dtv, cflv, nmax, aux=None,
     cflv_a = np.zeros(4)
     cflv_a[: len(cflv)] = cflv
     nv = np.array([nmax, 0], dtype=np.int32)
-    if backend == "oracle":
-        L = lib()
+    if backend in ("oracle", "omp"):
+        L = _cpu(backend, threads)
         L.orc_set_solver_params(C.byref(p.params))
         dtt = np.zeros(trace_cap)
         cft = np.zeros(trace_cap)

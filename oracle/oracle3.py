@@ -21,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "liboracle3d.so")
 REF_SO = os.path.join(HERE, "_ref", "libclawref3d.so")
+ORACLE_OMP_SO = os.path.join(HERE, "_build", "liboracle3d_omp.so")
 
 SOLVER_VC_ACOUSTICS = 3
 
@@ -56,6 +57,22 @@ def lib():
             subprocess.run(["make", "-s", "-C", HERE], check=True)
         _lib = C.CDLL(ORACLE_SO)
     return _lib
+
+
+_omp = None
+
+
+def lib_omp(threads: int):
+    """The OpenMP build of the same restatement (-O3; the blocks of each pass
+    of the unsplit step run on `threads` threads, each cell updated in the
+    serial order, so results are the plain build's bit for bit)."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(ORACLE_OMP_SO):
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+        _omp = C.CDLL(ORACLE_OMP_SO)
+    _omp.orc_set_threads(int(threads))
+    return _omp
 
 
 def ref_available() -> bool:
@@ -101,15 +118,17 @@ def _prep(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def step3(p: Problem3, qold, aux, dt, backend="oracle"):
-    """One unsplit (method(3) >= 0) step; returns (qnew, cfl)."""
+def step3(p: Problem3, qold, aux, dt, backend="oracle", threads=1):
+    """One unsplit (method(3) >= 0) step; returns (qnew, cfl).  backend:
+    "oracle", "omp" (the OpenMP build on `threads` threads) or "ref"."""
     qold = _prep(qold)
     aux = _prep(aux)
     qnew = qold.copy()
     cfl = C.c_double(0.0)
     g = p.geom
-    if backend == "oracle":
-        rc = lib().orc3_step3(p.meqn, p.mwaves, p.maux, p.mbc, p.mx, p.my, p.mz, _dp(qold),
+    if backend in ("oracle", "omp"):
+        L = lib() if backend == "oracle" else lib_omp(threads)
+        rc = L.orc3_step3(p.meqn, p.mwaves, p.maux, p.mbc, p.mx, p.my, p.mz, _dp(qold),
                               _dp(qnew), _dp(aux), C.c_double(g.dx), C.c_double(g.dy),
                               C.c_double(g.dz), C.c_double(dt), _ip(p.method), _ip(p.mthlim),
                               p.use_fwave, p.solver_id, C.byref(cfl))

@@ -122,31 +122,35 @@ int ref3_claw3(int meqn, int mwaves, int maux, int mbc, int mx, int my, int mz, 
     return info;
 }
 
-static int ref3_step_common(int split, int meqn, int mwaves, int maux, int mbc, int mx, int my,
-                            int mz, double* qold, double* qnew, double* aux, double dx,
+/* Work arrays of step3/dimsp3 for `nt` threads: the reference indexes every
+ * slice array by thread in its last dimension (step3.f90:26-38) and the
+ * flux3 scratch in `work` by thread (:71-113). */
+static int ref3_step_common(int split, int nt, int meqn, int mwaves, int maux, int mbc, int mx,
+                            int my, int mz, double* qold, double* qnew, double* aux, double dx,
                             double dy, double dz, double dt, int* method, int* mthlim,
                             double* cflv, int use_fwave, double* cfl)
 {
     int maxm = maxm3(mx, my, mz);
-    int n = maxm + 2 * mbc;
+    size_t n = (size_t)(maxm + 2 * mbc);
     int ma = maux > 0 ? maux : 1;
-    int mwork = n * (meqn * mwaves + mwaves + 40 * meqn) + 16;
-    size_t tot = (size_t)n * (15 * meqn + 3 + 9 * ma) + (size_t)mwork;
+    if (nt < 1) nt = 1;
+    int mwork = (int)(n * (size_t)(meqn * mwaves + mwaves + 40 * meqn) * (size_t)nt) + 16;
+    size_t tot = n * (size_t)(15 * meqn + 3 + 9 * ma) * (size_t)nt + (size_t)mwork;
     double* b = (double*)calloc(tot, sizeof(double));
     int fw = use_fwave ? 1 : 0;
     if (!b) return -3;
     double* qadd = b;
-    double* fadd = qadd + (size_t)n * meqn;
-    double* gadd = fadd + (size_t)n * meqn;
-    double* hadd = gadd + (size_t)6 * n * meqn;
-    double* q1d = hadd + (size_t)6 * n * meqn;
-    double* dtdx1d = q1d + (size_t)n * meqn;
-    double* dtdy1d = dtdx1d + n;
-    double* dtdz1d = dtdy1d + n;
-    double* aux1 = dtdz1d + n;
-    double* aux2 = aux1 + (size_t)3 * n * ma;
-    double* aux3 = aux2 + (size_t)3 * n * ma;
-    double* work = aux3 + (size_t)3 * n * ma;
+    double* fadd = qadd + n * meqn * nt;
+    double* gadd = fadd + n * meqn * nt;
+    double* hadd = gadd + 6 * n * meqn * nt;
+    double* q1d = hadd + 6 * n * meqn * nt;
+    double* dtdx1d = q1d + n * meqn * nt;
+    double* dtdy1d = dtdx1d + n * nt;
+    double* dtdz1d = dtdy1d + n * nt;
+    double* aux1 = dtdz1d + n * nt;
+    double* aux2 = aux1 + 3 * n * ma * nt;
+    double* aux3 = aux2 + 3 * n * ma * nt;
+    double* work = aux3 + 3 * n * ma * nt;
     if (!split)
         step3_(&maxm, &meqn, &mwaves, &mbc, &mx, &my, &mz, qold, qnew, aux, &dx, &dy, &dz, &dt,
                method, mthlim, cfl, qadd, fadd, gadd, hadd, q1d, dtdx1d, dtdy1d, dtdz1d, aux1,
@@ -161,11 +165,22 @@ static int ref3_step_common(int split, int meqn, int mwaves, int maux, int mbc, 
     return 0;
 }
 
+/* step3 with work arrays for nt OpenMP threads (the reference's own OpenMP
+ * loop, step3.f90:132-869, in a build compiled with -fopenmp; the caller
+ * sets the thread count through OMP_NUM_THREADS) */
+int ref3_step3_nt(int nt, int meqn, int mwaves, int maux, int mbc, int mx, int my, int mz,
+                  double* qold, double* qnew, double* aux, double dx, double dy, double dz,
+                  double dt, int* method, int* mthlim, int use_fwave, double* cfl)
+{
+    return ref3_step_common(0, nt, meqn, mwaves, maux, mbc, mx, my, mz, qold, qnew, aux, dx, dy,
+                            dz, dt, method, mthlim, NULL, use_fwave, cfl);
+}
+
 int ref3_step3(int meqn, int mwaves, int maux, int mbc, int mx, int my, int mz, double* qold,
                double* qnew, double* aux, double dx, double dy, double dz, double dt,
                int* method, int* mthlim, int use_fwave, double* cfl)
 {
-    return ref3_step_common(0, meqn, mwaves, maux, mbc, mx, my, mz, qold, qnew, aux, dx, dy, dz,
+    return ref3_step_common(0, 1, meqn, mwaves, maux, mbc, mx, my, mz, qold, qnew, aux, dx, dy, dz,
                             dt, method, mthlim, NULL, use_fwave, cfl);
 }
 
@@ -173,7 +188,7 @@ int ref3_dimsp3(int meqn, int mwaves, int maux, int mbc, int mx, int my, int mz,
                 double* qnew, double* aux, double dx, double dy, double dz, double dt,
                 int* method, int* mthlim, double* cflv, int use_fwave, double* cfl)
 {
-    return ref3_step_common(1, meqn, mwaves, maux, mbc, mx, my, mz, qold, qnew, aux, dx, dy, dz,
+    return ref3_step_common(1, 1, meqn, mwaves, maux, mbc, mx, my, mz, qold, qnew, aux, dx, dy, dz,
                             dt, method, mthlim, cflv, use_fwave, cfl);
 }
 
