@@ -13,6 +13,7 @@
 
 #include "common.hpp"
 #include "bc2d.hpp"
+#include "loop2d.hpp"
 #include "layout2d.hpp"
 #include "split2d_kernel.hpp"
 #include "../../include/clawgpu.h"
@@ -32,6 +33,22 @@ __global__ void bc2_kernel(BcArgs a)
     if (t == 0 && a.cfl_bits)
         for (int k = 0; k < 4; k++) a.cfl_bits[k] = 0ull;
     bc2_cell(a.q, a.L, a.meqn, a.mthbc, t);
+}
+
+// The decision of the last attempt of a device-side claw2 loop (attempt
+// l.k - 1), stored in l.st[l.k & 1].
+__global__ void loop2_finish_kernel(LoopArgs l)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    LoopState s;
+    loop_next(l, l.st[(l.k - 1) & 1], loop_cfl(l, l.k - 1), s);
+    l.st[l.k & 1] = s;
+}
+
+int launch_loop2_finish(const LoopArgs& l, hipStream_t s)
+{
+    hipLaunchKernelGGL(loop2_finish_kernel, dim3(1), dim3(64), 0, s, l);
+    return (int)hipGetLastError();
 }
 
 }  // namespace clawgpu
@@ -147,6 +164,9 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
         (e = hipMalloc((void**)&c->staging, abytes)) != hipSuccess ||
         (e = hipMalloc((void**)&c->dcfl, 4 * sizeof(unsigned long long))) != hipSuccess ||
         (e = hipHostMalloc((void**)&c->hcfl, 4 * sizeof(double), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipMalloc((void**)&c->dloop, 2 * sizeof(LoopState))) != hipSuccess ||
+        (e = hipMalloc((void**)&c->dcflb, 3 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipHostMalloc((void**)&c->hloop, 3 * sizeof(LoopState), hipHostMallocDefault)) != hipSuccess ||
         (general && (e = hipMalloc((void**)&c->daux, (size_t)c->L.fs * maux * sizeof(double))) != hipSuccess) ||
         (general && (e = hipMalloc((void**)&c->grec, (size_t)c->L.fs * 10 * sizeof(double))) != hipSuccess)) {
         snprintf(g_errbuf, sizeof g_errbuf, "allocation failed: %s", hipGetErrorString(e));
@@ -178,6 +198,9 @@ extern "C" int clawgpu_destroy(clawgpu_ctx* c)
     if (c->hcfl) (void)hipHostFree(c->hcfl);
     if (c->daux) (void)hipFree(c->daux);
     if (c->grec) (void)hipFree(c->grec);
+    if (c->dloop) (void)hipFree(c->dloop);
+    if (c->dcflb) (void)hipFree(c->dcflb);
+    if (c->hloop) (void)hipHostFree(c->hloop);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CLAWGPU_OK;
@@ -331,6 +354,26 @@ static int step_rows(clawgpu_ctx* c, const StepArgs& a)
     return best;
 }
 
+// Arguments of the unsplit step kernel for this grid and method (buffers,
+// dt ratios and the Courant slot left for the caller).
+static StepArgs step_args(clawgpu_ctx* c, const int* method, const int* mthlim, int use_fwave)
+{
+    StepArgs a{};
+    a.L = c->L;
+    a.order = method[1];
+    a.trans = method[2];
+    a.use_fwave = use_fwave ? 1 : 0;
+    for (int k = 0; k < kMaxWaves; k++) a.lim[k] = k < c->mwaves ? mthlim[k] : 0;
+    a.nstrips = (c->mx + kStripOut - 1) / kStripOut;
+    a.seglen = step_rows(c, a);
+    a.nsegs = (c->my + a.seglen - 1) / a.seglen;
+    a.P = c->P;
+    a.phix = c->phix;
+    a.phiy = c->phiy;
+    a.bcmap = 0;
+    return a;
+}
+
 // Enqueue one unsplit step from dq[cur] into dq[1-cur]; the Courant
 // accumulator must have been cleared (by the bc pass or explicitly).
 static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const int* method,
@@ -355,23 +398,12 @@ static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const i
         HIPCHK(c, (hipError_t)launch_step2_general(g, c->stream));
         return CLAWGPU_OK;
     }
-    StepArgs a;
+    StepArgs a = step_args(c, method, mthlim, use_fwave);
     a.qold = c->dq[c->cur];
     a.qnew = c->dq[(c->cur + 1) % 3];
-    a.L = c->L;
     a.dtdx = dt / dx;
     a.dtdy = dt / dy;
-    a.order = method[1];
-    a.trans = method[2];
-    a.use_fwave = use_fwave ? 1 : 0;
-    for (int k = 0; k < kMaxWaves; k++) a.lim[k] = k < c->mwaves ? mthlim[k] : 0;
-    a.nstrips = (c->mx + kStripOut - 1) / kStripOut;
-    a.seglen = step_rows(c, a);
-    a.nsegs = (c->my + a.seglen - 1) / a.seglen;
     a.cfl_bits = c->dcfl;
-    a.P = c->P;
-    a.phix = c->phix;
-    a.phiy = c->phiy;
     long waves = (long)a.nstrips * a.nsegs;
     int nb = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     int e = c->arith == ARITH_FAST ? launch_step2_fast(c->solver, a, nb, c->stream)
@@ -499,6 +531,94 @@ extern "C" int clawgpu_dimsp2(clawgpu_ctx* c, double dx, double dy, double dt, c
     return CLAWGPU_OK;
 }
 
+// The claw2 loop without user routines, decided on the device: each
+// attempt of a step is one launch of the loop kernel, which decides the
+// previous attempt (accept: its output buffer becomes q and dt follows the
+// Courant number; reject: dt shrinks and q stays, claw2.f:695-753) and
+// reads the ghost cells through the boundary maps, so nothing returns to
+// the host between steps.  The host keeps at most two chunks of attempts in
+// flight and reads the state back once per chunk: attempts launched after
+// the loop has finished are no-ops.  *out receives the final state.
+static int claw2_device(clawgpu_ctx* c, double dx, double dy, double tstart, double tend,
+                        const double* dtv, const double* cflv, int maxn, const int* method,
+                        const int* mthlim, const int* mthbc, int use_fwave, LoopState* out)
+{
+    StepArgs a = step_args(c, method, mthlim, use_fwave);
+    a.bcmap = 1;
+    for (int e = 0; e < 4; e++) a.mthbc[e] = mthbc[e];
+    LoopArgs l{};
+    for (int b = 0; b < 3; b++) l.q[b] = c->dq[b];
+    l.dx = dx;
+    l.dy = dy;
+    l.tstart = tstart;
+    l.tend = tend;
+    l.dtv2 = dtv[1];
+    l.cflv1 = cflv[0];
+    l.cflv2 = cflv[1];
+    l.maxn = maxn;
+    l.vardt = method[0] == 1;
+    l.st = c->dloop;
+    l.cflb = c->dcflb;
+    // state before the first attempt (claw2.f:596-601)
+    LoopState& s0 = c->hloop[2];
+    s0 = LoopState{};
+    s0.t = s0.told = tstart;
+    s0.dt = dtv[0];
+    if (tstart + s0.dt > tend && tstart < tend) s0.dt = tend - tstart;
+    s0.dtmin = s0.dtmax = dtv[0];
+    s0.cur = c->cur;
+    HIPCHK(c, hipMemcpyAsync(c->dloop, &s0, sizeof(LoopState), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->dcflb, 0, 3 * sizeof(unsigned long long), c->stream));
+    struct Evs {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Evs()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    for (int b = 0; b < 2; b++) HIPCHK(c, hipEventCreateWithFlags(&ev.e[b], hipEventDisableTiming));
+    const long waves = (long)a.nstrips * a.nsegs;
+    const int nb = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    const bool fast = c->arith == ARITH_FAST;
+    int k = 0, snap_k[2] = {0, 0}, qh = 0, qn = 0, n_seen = 0, k_known = 0;
+    for (;;) {
+        // attempts beyond those already launched that can still be needed,
+        // if none of the undecided ones is rejected
+        const long est = (long)maxn - n_seen - (k - k_known);
+        if (qn < 2 && (est > 0 || qn == 0)) {
+            const int m = est > 0 ? (int)(est < 32 ? est : 32) : 1;
+            for (int j = 0; j < m; j++) {
+                l.k = k++;
+                HIPCHK(c, (hipError_t)(fast ? launch_loop2_fast(c->solver, a, l, nb, c->stream)
+                                            : launch_loop2_exact(c->solver, a, l, nb, c->stream)));
+            }
+            const int slot = (qh + qn) % 2;
+            snap_k[slot] = k;
+            HIPCHK(c, hipMemcpyAsync(&c->hloop[slot], c->dloop + ((k - 1) & 1), sizeof(LoopState),
+                                     hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipEventRecord(ev.e[slot], c->stream));
+            qn++;
+            continue;
+        }
+        HIPCHK(c, hipEventSynchronize(ev.e[qh]));
+        const LoopState r = c->hloop[qh];  // state before attempt snap_k - 1
+        const int ks = snap_k[qh];
+        qh = (qh + 1) % 2;
+        qn--;
+        if (r.done) break;
+        n_seen = r.n;
+        k_known = ks - 1;
+    }
+    l.k = k;
+    HIPCHK(c, (hipError_t)launch_loop2_finish(l, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->hloop[2], c->dloop + (k & 1), sizeof(LoopState),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = c->hloop[2];
+    return CLAWGPU_OK;
+}
+
 // The claw2 time loop (src/2d/claw2.f:494-784) on device-resident q: per
 // step bc2 -> fused step into the other buffer -> Courant number back to the
 // host -> accept (swap buffers) or reject (keep the old buffer, which is the
@@ -568,6 +688,25 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
     if (method[5] > method[6]) { info = 5; goto done; }
     if (method[1] < method[2]) { info = 6; goto done; }
     if (maxn == 0) goto done;
+    if (!h && !split && !uses_table(c, mthlim) && !general_path(c)) {
+        LoopState f;
+        rc = claw2_device(c, dx, dy, tstart, *tend, dtv, cflv, maxn, method, mthlim, mthbc,
+                          use_fwave, &f);
+        if (rc) return rc;
+        if (f.info == -5) {
+            *info_out = -5;
+            return set_err(CLAWGPU_ERR_NONFINITE, "non-finite wave speed (Courant number)");
+        }
+        c->cur = f.cur;
+        t = f.t;
+        cflmax = f.cflmax;
+        cfl = f.cfl;
+        dtmin = f.dtmin;
+        dtmax = f.dtmax;
+        dt = f.dt;
+        nv[1] = f.n;
+        goto done;
+    }
     for (int n = 1; n <= maxn; n++) {
         double told = t;
         if (told + dt > *tend && tstart < *tend) dt = *tend - told;

@@ -117,6 +117,8 @@ __device__ __forceinline__ void step_loop(const Params& b)
         a.cfl_bits = &b.prob[k].cfl_bits;
         a.P = p.P;
         a.phix = a.phiy = nullptr;
+        a.bcmap = 0;  // the batch fills ghost cells in its own pass
+        for (int e = 0; e < 4; e++) a.mthbc[e] = 0;
         step2_wave<S, Cfg>(a, (int)(local % p.nstrips), (int)(local / p.nstrips), lane);
     }
 }

@@ -42,6 +42,38 @@ struct StepArgs {
     // and (i, J); y: between (i, J-1) and (i, J)
     const double* phix;
     const double* phiy;
+    // bcmap != 0: the ghost cells of qold are not filled; a load outside
+    // the interior reads the interior cell the bc2 rules copy (codes
+    // mthbc: 1 extrapolation, 2 periodic, 3 wall negating the normal
+    // momentum), which is the value bc2 would have stored there
+    int bcmap;
+    int mthbc[4];
+};
+
+// State of the device-side claw2 loop (src/2d/claw2.f:596-753) before one
+// attempt of a step: the buffer holding q, the attempt's dt and start time,
+// and the loop's running outputs.
+struct LoopState {
+    double t, told, dt;             // time reached, start of the attempt, its dt
+    double cflmax, dtmin, dtmax, cfl;
+    int n;     // accepted steps so far
+    int cur;   // buffer holding q (the attempt reads it, writes (cur+1)%3)
+    int done;  // nothing left to do: every later attempt is a no-op
+    int info;  // 0, or -5 for a non-finite Courant number
+};
+
+// One launch of the loop: attempt k.  st[2] holds the states before
+// attempts of alternating parity, cflb[3] the Courant maxima (as raw bits)
+// of the attempts k % 3; attempt k decides attempt k-1 from st[(k-1)&1] and
+// cflb[(k-1)%3], stores the result in st[k&1] and clears cflb[(k+1)%3].
+struct LoopArgs {
+    double* q[3];
+    double dx, dy, tstart, tend;
+    double dtv2, cflv1, cflv2;  // dt_max, cfl_max, cfl_desired
+    int maxn, vardt;
+    LoopState* st;
+    unsigned long long* cflb;
+    int k;
 };
 
 // limiter id of the table-driven (learned) limiter
@@ -65,6 +97,12 @@ int launch_sweep_fast(int solver, int dir, const SweepArgs& a, int nblocks, hipS
 // correctly rounded division); fast: FMA contraction and reciprocal reuse.
 int launch_step2_exact(int solver, const StepArgs& a, int nblocks, hipStream_t s);
 int launch_step2_fast(int solver, const StepArgs& a, int nblocks, hipStream_t s);
+// One attempt of the device-side claw2 loop (a: everything but the buffers,
+// dt ratios and Courant slot, which come from the loop state) and the
+// loop's last decision (one wave).
+int launch_loop2_exact(int solver, const StepArgs& a, const LoopArgs& l, int nblocks, hipStream_t s);
+int launch_loop2_fast(int solver, const StepArgs& a, const LoopArgs& l, int nblocks, hipStream_t s);
+int launch_loop2_finish(const LoopArgs& l, hipStream_t s);
 // Wavefronts of the step kernel launch_step2_* would pick for a that one CU
 // holds at once (0 if the occupancy query fails).
 int step2_waves_per_cu_exact(int solver, const StepArgs& a);

@@ -34,6 +34,11 @@ struct clawgpu_ctx {
     // learned limiter state (allocated on first use)
     double *phix = nullptr, *phiy = nullptr;  // phi tables read by the step kernel
     void* learned = nullptr;                   // LearnedState (learned.hip)
+    // device-side claw2 loop: states before attempts of each parity, the
+    // Courant slots of three consecutive attempts, pinned read-back copies
+    LoopState* dloop = nullptr;
+    unsigned long long* dcflb = nullptr;
+    LoopState* hloop = nullptr;  // 3 entries
     int last_hip_error;
 };
 

@@ -9,6 +9,10 @@ int launch_step2_exact(int solver, const StepArgs& a, int nblocks, hipStream_t s
 {
     return launch_step2_impl<ExactMath>(solver, a, nblocks, s);
 }
+int launch_loop2_exact(int solver, const StepArgs& a, const LoopArgs& l, int nblocks, hipStream_t s)
+{
+    return launch_loop2_impl<ExactMath>(solver, a, l, nblocks, s);
+}
 int step2_waves_per_cu_exact(int solver, const StepArgs& a)
 {
     return step2_waves_per_cu_impl<ExactMath>(solver, a);

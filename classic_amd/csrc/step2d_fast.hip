@@ -12,6 +12,10 @@ int launch_step2_fast(int solver, const StepArgs& a, int nblocks, hipStream_t s)
 {
     return launch_step2_impl<FastMath>(solver, a, nblocks, s);
 }
+int launch_loop2_fast(int solver, const StepArgs& a, const LoopArgs& l, int nblocks, hipStream_t s)
+{
+    return launch_loop2_impl<FastMath>(solver, a, l, nblocks, s);
+}
 int step2_waves_per_cu_fast(int solver, const StepArgs& a)
 {
     return step2_waves_per_cu_impl<FastMath>(solver, a);

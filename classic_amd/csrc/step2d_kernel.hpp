@@ -29,7 +29,9 @@
 
 #include <type_traits>
 
+#include "bc2d.hpp"
 #include "common.hpp"
+#include "loop2d.hpp"
 #include "riemann2d.hpp"
 #include "lanes.hpp"
 
@@ -46,6 +48,72 @@ namespace clawgpu {
 // vector-memory operations, and its wait for the prefetched next row would
 // also wait for the previous row's stores to be acknowledged.
 static __device__ double g_store_sink[kMaxWaves * 2 * 64];
+
+// The reads of one lane's column of qold.  BC = 0: the ghost cells are
+// filled; the column is clamped into the array and rows are read as stored.
+// BC = 1, 2 (the device-side loop, a.bcmap): the ghost cells were not
+// filled; a ghost column or row is read from the interior cell bc2 copies
+// into it (src/2d/bc2.f:4-215; a corner from the cell both maps give, as
+// bc2's bottom/top passes over the filled ghost columns do), and with
+// BC = 2 (some side is a wall) the wall sign flips of component 2 (x-walls)
+// and 3 (y-walls) are applied as sign-bit flips.
+template <int BC>
+struct ColumnReader {
+    const double* col;
+    long pitch, fs;
+    int my;
+    int xflip;  // sign bit flipped in component 2 of every row of this column
+    // BC > 0: source rows of the ghost rows -1, 0, my+1, my+2 (a wave reads
+    // no row outside [-1, my+2] that it uses; rows beyond are clamped) and
+    // their sign bits for component 3, all uniform, so that the row loop
+    // picks them with scalar selects
+    int gsrc[4], gflip[4];
+
+    __device__ ColumnReader(const StepArgs& a, int i)
+    {
+        const Layout2& L = a.L;
+        pitch = L.pitch;
+        fs = L.fs;
+        my = L.my;
+        int ic;
+        xflip = 0;
+        if constexpr (BC > 0) {
+            bool f;
+            const int i0 = min(max(i, 1 - L.mbc), L.mx + L.mbc);
+            ic = bc_map(i0, L.mx, L.mbc, a.mthbc[0], a.mthbc[1], f);
+            if constexpr (BC == 2) xflip = f ? (int)0x80000000u : 0;
+            const int rows[4] = {-1, 0, L.my + 1, L.my + 2};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                gsrc[e] = bc_map(rows[e], L.my, L.mbc, a.mthbc[2], a.mthbc[3], f);
+                gflip[e] = f ? (int)0x80000000u : 0;
+            }
+        } else {
+            ic = i < L.mx + L.mbc ? i : L.mx + L.mbc;
+        }
+        col = a.qold + L.off(ic, 0);
+    }
+
+    // components 0..M-1 of row j (uniform across the wave)
+    template <int M>
+    __device__ __forceinline__ void row(int j, double (&v)[M]) const
+    {
+        int jr = j, yflip = 0;
+        if constexpr (BC > 0) {
+            if (j < 1 || j > my) {
+                const int e = j < 0 ? 0 : j == 0 ? 1 : j == my + 1 ? 2 : 3;
+                jr = gsrc[e];
+                yflip = gflip[e];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) v[m] = col[m * fs + (long)jr * pitch];
+        if constexpr (BC == 2 && M >= 3) {
+            v[1] = __hiloint2double(__double2hiint(v[1]) ^ xflip, __double2loint(v[1]));
+            v[2] = __hiloint2double(__double2hiint(v[2]) ^ yflip, __double2loint(v[2]));
+        }
+    }
+};
 
 // Method flags fixed at compile time (CfgFix) or read from the arguments
 // (CfgAny).  The fixed variant serves the common case method(2)=2,
@@ -74,7 +142,7 @@ struct CfgFix {
 // and the Courant number of every interface is c dt/dx.  The sums are the
 // generic kernel's (fast-arithmetic order); results agree with it and with
 // the reference to rounding.
-template <class Cfg>
+template <class Cfg, int BC>
 __device__ __forceinline__ void acoustics_wave_fast(const StepArgs& a, const int strip,
                                                     const int seg, const int lane)
 {
@@ -82,14 +150,13 @@ __device__ __forceinline__ void acoustics_wave_fast(const StepArgs& a, const int
     const Layout2 L = a.L;
     const int mx = L.mx, my = L.my, mbc = L.mbc;
     const int i = kStripOut * strip + lane - 1;
-    const int ic = i < mx + mbc ? i : mx + mbc;
     const int r0 = 1 + seg * a.seglen;
     const int r1 = min(r0 + a.seglen, my + 1);
     const bool out_lane = lane >= 2 && lane <= kStripOut + 1 && i <= mx;
     const bool xcfl_lane = lane >= 1 && i >= 1 && i <= mx + 1;
     const bool ycfl_lane = i >= 0 && i <= mx + 1;
     const double dtdx = a.dtdx, dtdy = a.dtdy;
-    const double* __restrict__ qo = a.qold + L.off(ic, 0);
+    const ColumnReader<BC> rd(a, i);
     double* __restrict__ qn = a.qnew + L.off(i, 0);
     const long pitch = L.pitch, fs = L.fs;
     const int rowmax = my + mbc;
@@ -106,10 +173,10 @@ __device__ __forceinline__ void acoustics_wave_fast(const StepArgs& a, const int
         double a1, a2, a2lo;
     };
     YA ysA{0.0, 0.0, 0.0}, ysB{0.0, 0.0, 0.0};
-    double pp = qo[(r0 - 2) * pitch], vp = qo[2 * fs + (r0 - 2) * pitch];  // cell J-1
     double qnx[3];
-#pragma unroll
-    for (int m = 0; m < 3; m++) qnx[m] = qo[m * fs + (r0 - 1) * pitch];
+    rd.row(r0 - 2, qnx);
+    double pp = qnx[0], vp = qnx[2];  // cell J-1
+    rd.row(r0 - 1, qnx);
     double vA[3] = {0.0, 0.0, 0.0}, vB[3] = {0.0, 0.0, 0.0};
     double g2p = 0.0, g2v = 0.0;  // x-row J-1's G2 owed to row J (p, v)
     double cfl = 0.0;
@@ -121,11 +188,7 @@ __device__ __forceinline__ void acoustics_wave_fast(const StepArgs& a, const int
 
     auto row = [&](YA& yp, YA& yc, const int J) {
         const double p = qnx[0], u = qnx[1], v = qnx[2];
-        {
-            const long jn = min(J + 1, rowmax);
-#pragma unroll
-            for (int m = 0; m < 3; m++) qnx[m] = qo[m * fs + jn * pitch];
-        }
+        rd.row(min(J + 1, rowmax), qnx);
         double vC[3];
         // ---------------------------------------------- x-row J
         if (J <= r1) {
@@ -228,13 +291,13 @@ __device__ __forceinline__ void acoustics_wave_fast(const StepArgs& a, const int
 }
 
 // The work of one wavefront: strip `strip` (64 columns) and row segment `seg`
-// of the grid described by a.
-template <class S, class Cfg>
+// of the grid described by a; BC: how ghost cells are read (ColumnReader).
+template <class S, class Cfg, int BC = 0>
 __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, const int seg,
                                            const int lane)
 {
     if constexpr (std::is_same_v<S, Acoustics2<FastMath>> && Cfg::fixed) {
-        acoustics_wave_fast<Cfg>(a, strip, seg, lane);
+        acoustics_wave_fast<Cfg, BC>(a, strip, seg, lane);
         return;
     }
     using Mth = typename S::Math;
@@ -244,7 +307,7 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
     const Layout2 L = a.L;
     const int mx = L.mx, my = L.my, mbc = L.mbc;
     const int i = kStripOut * strip + lane - 1;
-    const int ic = i < mx + mbc ? i : mx + mbc;  // clamped column for loads
+    const int ic = i < mx + mbc ? i : mx + mbc;  // clamped column (learned-limiter tables)
     const int r0 = 1 + seg * a.seglen;
     const int r1 = min(r0 + a.seglen, my + 1);  // output rows r0 .. r1-1
     const bool out_lane = lane >= 2 && lane <= kStripOut + 1 && i <= mx;
@@ -256,7 +319,7 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
     const bool tr2 = second && trn == 2;
     const bool trans = trn > 0;
     const bool fwave = Cfg::fixed ? false : a.use_fwave != 0;
-    const double* __restrict__ qo = a.qold + L.off(ic, 0);
+    const ColumnReader<BC> rd(a, i);
     double* __restrict__ qn = a.qnew + L.off(i, 0);
     const long pitch = L.pitch, fs = L.fs;
     const int rowmax = my + mbc;
@@ -272,8 +335,7 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
     YS ysA, ysB;
     {
         double q0[M];
-#pragma unroll
-        for (int m = 0; m < M; m++) q0[m] = qo[m * fs + (r0 - 2) * pitch];
+        rd.row(r0 - 2, q0);
         ysA.cell = S::cell(P, q0);
 #pragma unroll
         for (int mw = 0; mw < MW; mw++) {
@@ -285,8 +347,7 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
         for (int m = 0; m < M; m++) ysA.am[m] = ysA.ap[m] = 0.0;
     }
     double qnx[M];
-#pragma unroll
-    for (int m = 0; m < M; m++) qnx[m] = qo[m * fs + (r0 - 1) * pitch];
+    rd.row(r0 - 1, qnx);
 
     double vA[M], vB[M], g2prev[M];
     double qa_part[M], fa_part[M], gp1[M], gp2[M];
@@ -304,11 +365,7 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
         double qc[M];
 #pragma unroll
         for (int m = 0; m < M; m++) qc[m] = qnx[m];
-        {
-            const long jn = min(J + 1, rowmax);
-#pragma unroll
-            for (int m = 0; m < M; m++) qnx[m] = qo[m * fs + jn * pitch];
-        }
+        rd.row(min(J + 1, rowmax), qnx);
         yc.cell = S::cell(P, qc);
         const Cell& cc = yc.cell;
         double vC[M];
@@ -563,6 +620,50 @@ __global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(S
     step2_wave<S, Cfg>(a, wid % a.nstrips, wid / a.nstrips, lane);
 }
 
+// A value equal in all lanes, moved to scalar registers.
+__device__ __forceinline__ double uniform_double(double x)
+{
+    return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
+                            __builtin_amdgcn_readfirstlane(__double2loint(x)));
+}
+
+// Attempt l.k of the device-side claw2 loop: every wave first decides the
+// previous attempt (loop_next) from the state before it and its Courant
+// number, which gives the buffers and dt of this attempt; wave 0 of block 0
+// stores that state and clears the Courant slot of the next attempt.  A
+// finished loop makes the attempt a no-op.  The ghost cells are read
+// through the boundary maps (a.bcmap), so no ghost-fill pass runs between
+// attempts.
+// (FAST arithmetic: held to two waves per SIMD, the unsplit kernel's
+// occupancy, which the loop's few extra live values would otherwise cost.)
+template <class S, class Cfg, int BC>
+__global__ __launch_bounds__(256, S::Math::fast ? 2 : CLAWGPU_MIN_WAVES) void step2_loop_kernel(StepArgs a, LoopArgs l)
+{
+    LoopState s;
+    if (l.k == 0)
+        s = l.st[0];
+    else
+        loop_next(l, l.st[(l.k - 1) & 1], loop_cfl(l, l.k - 1), s);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (l.k > 0) l.st[l.k & 1] = s;
+        l.cflb[(l.k + 1) % 3] = 0ull;
+    }
+    // the decision is computed in vector registers; its results are the
+    // same in every lane, and moving them to scalar registers keeps the
+    // step's constants (buffers, dt ratios) out of the vector budget
+    if (__builtin_amdgcn_readfirstlane(s.done)) return;
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (wid >= a.nstrips * a.nsegs) return;  // uniform per wave
+    const int cur = __builtin_amdgcn_readfirstlane(s.cur);
+    a.qold = l.q[cur];
+    a.qnew = l.q[(cur + 1) % 3];
+    a.dtdx = uniform_double(s.dt / l.dx);
+    a.dtdy = uniform_double(s.dt / l.dy);
+    a.cfl_bits = l.cflb + l.k % 3;
+    step2_wave<S, Cfg, BC>(a, wid % a.nstrips, wid / a.nstrips, lane);
+}
+
 // Solver-level evaluation for tests: interface k between states ql[k]
 // (left, reference layout q(meqn)) and qr[k]; outputs wave(meqn, mwaves),
 // s(mwaves), amdq, apdq, and the transverse split of asdq[k] (imp = 1 uses
@@ -647,6 +748,37 @@ static void launch_cfg(const StepArgs& a, int nblocks, hipStream_t s)
         hipLaunchKernelGGL((step2_unsplit_kernel<S, CfgFix<2, 2, 4>>), dim3(nblocks), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL((step2_unsplit_kernel<S, CfgAny>), dim3(nblocks), dim3(256), 0, s, a);
+}
+
+template <class S, int BC>
+static void launch_loop_bc(const StepArgs& a, const LoopArgs& l, int nblocks, hipStream_t s)
+{
+    if (fixed_cfg<S>(a))
+        hipLaunchKernelGGL((step2_loop_kernel<S, CfgFix<2, 2, 4>, BC>), dim3(nblocks), dim3(256), 0, s, a, l);
+    else
+        hipLaunchKernelGGL((step2_loop_kernel<S, CfgAny, BC>), dim3(nblocks), dim3(256), 0, s, a, l);
+}
+
+template <class S>
+static void launch_loop_cfg(const StepArgs& a, const LoopArgs& l, int nblocks, hipStream_t s)
+{
+    bool walls = false;
+    for (int e = 0; e < 4; e++) walls = walls || a.mthbc[e] == 3;
+    if (walls)
+        launch_loop_bc<S, 2>(a, l, nblocks, s);
+    else
+        launch_loop_bc<S, 1>(a, l, nblocks, s);
+}
+
+template <class Mth>
+static int launch_loop2_impl(int solver, const StepArgs& a, const LoopArgs& l, int nblocks,
+                             hipStream_t s)
+{
+    if (solver == 1)
+        launch_loop_cfg<Acoustics2<Mth>>(a, l, nblocks, s);
+    else
+        launch_loop_cfg<Euler2<Mth>>(a, l, nblocks, s);
+    return (int)hipGetLastError();
 }
 
 template <class Mth>
