@@ -87,6 +87,9 @@ struct Acoustics2 {
         double q[M];
     };
     static constexpr int XSHIFT = 3;  // leading doubles of Cell an x-interface needs
+    // the largest wave speed magnitude of an interface is that of its first
+    // or its last wave (here both are c)
+    static constexpr bool kOuterSpeedsBound = true;
     struct Cache {};
 
     __device__ static inline Cell cell(const SolverParams&, const double* q)
@@ -172,6 +175,9 @@ struct Euler2 {
     };
     // an x-interface needs all but vel[1] of its left cell (Cell order)
     static constexpr int XSHIFT = 10;
+    // |u| <= max(|u - a|, |u + a|) for a >= 0, also after rounding (fl is
+    // monotonic): the middle wave never sets the Courant number
+    static constexpr bool kOuterSpeedsBound = true;
     struct Cache {
         double u, v, enth, a, g1a2, euv, u2v2, h2a;  // h2a = 1/(2a) (fast math only)
     };

@@ -290,6 +290,268 @@ __device__ __forceinline__ void acoustics_wave_fast(const StepArgs& a, const int
     publish_cfl(cfl, a.cfl_bits);
 }
 
+// FAST arithmetic in the fixed configuration (second order, transverse 2,
+// limiter Cfg::lim on every wave, no f-waves): the sweep of step2_wave with
+// the arithmetic rearranged to issue fewer instructions --
+//  * the limiter factor phi scales the wave's second-order coefficient
+//    |s|(1 - |s| dt/dx) instead of its components (a zero wave keeps
+//    phi = 1, as the reference skips it);
+//  * an interface's Courant number is dt/dx max(|s_1|, |s_mw|), which bounds
+//    every wave speed for these solvers (S::kOuterSpeedsBound); a NaN speed
+//    stays visible through a running sum and is published as +inf;
+//  * the transverse splits move between lanes unscaled and enter the sums
+//    through the one constant -dt/dx dt/dy / 2;
+//  * the normal fluctuation and the correction flux are combined before
+//    they move (amdq + cq/2 to the left cell, apdq - cq/2 to the right).
+// Same terms as the reference, summed in another order: the tests hold it to
+// the 1e-12 bar.
+template <class S, class Cfg, int BC>
+__device__ __forceinline__ void step2_wave_fast(const StepArgs& a, const int strip, const int seg,
+                                                const int lane)
+{
+    using Mth = typename S::Math;
+    using Cell = typename S::Cell;
+    using Cache = typename S::Cache;
+    constexpr int M = S::M, MW = S::MW;
+    static_assert(Mth::fast && Cfg::fixed && S::kOuterSpeedsBound, "fast fixed path");
+    const Layout2 L = a.L;
+    const int mx = L.mx, my = L.my, mbc = L.mbc;
+    const int i = kStripOut * strip + lane - 1;
+    const int r0 = 1 + seg * a.seglen;
+    const int r1 = min(r0 + a.seglen, my + 1);  // output rows r0 .. r1-1
+    const bool out_lane = lane >= 2 && lane <= kStripOut + 1 && i <= mx;
+    const bool xcfl_lane = lane >= 1 && i >= 1 && i <= mx + 1;
+    const bool ycfl_lane = i >= 0 && i <= mx + 1;
+    const double dtdx = a.dtdx, dtdy = a.dtdy;
+    const double kt = -0.5 * dtdx * dtdy;  // transverse terms of either sweep
+    const ColumnReader<BC> rd(a, i);
+    double* __restrict__ qn = a.qnew + L.off(i, 0);
+    const long pitch = L.pitch, fs = L.fs;
+    const int rowmax = my + mbc;
+    const SolverParams P = a.P;
+
+    struct YS {
+        Cell cell;  // cell (i, J)
+        double w[MW][M], s[MW], am[M], ap[M], dotl[MW];
+        Cache cache;
+    };
+    YS ysA, ysB;
+    {
+        double q0[M];
+        rd.row(r0 - 2, q0);
+        ysA.cell = S::cell(P, q0);
+#pragma unroll
+        for (int mw = 0; mw < MW; mw++) {
+            ysA.s[mw] = ysA.dotl[mw] = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; m++) ysA.w[mw][m] = 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) ysA.am[m] = ysA.ap[m] = 0.0;
+    }
+    double qnx[M];
+    rd.row(r0 - 1, qnx);
+    double vA[M], vB[M], b2prev[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) vA[m] = vB[m] = b2prev[m] = 0.0;
+    double cfl = 0.0, cfl_sum = 0.0;
+
+    // second-order coefficient of a wave of speed s, limiter factor included
+    auto coef = [&](double s, double num, double wn2, double dtd) {
+        double phi = philim<Mth>(Mth::ratio(num, wn2), Cfg::lim);
+        phi = wn2 != 0.0 ? phi : 1.0;
+        const double sa = fabs(s);
+        return (sa * (1.0 - sa * dtd)) * phi;
+    };
+    auto courant = [&](const double* s, double dtd, bool lane_ok) {
+        double t = dtd * fmax(fabs(s[0]), fabs(s[MW - 1]));
+        t = lane_ok ? t : 0.0;
+        cfl = fmax(cfl, t);
+        cfl_sum = cfl_sum + t;
+    };
+
+    auto row = [&](YS& yp, YS& yc, const int J) {
+        double qc[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) qc[m] = qnx[m];
+        rd.row(min(J + 1, rowmax), qnx);
+        yc.cell = S::cell(P, qc);
+        const Cell& cc = yc.cell;
+        double vC[M];
+
+        // ------------------------------------------------ x-row J (flux2, ixy=1)
+        if (J <= r1) {
+            Cell cl;
+            {
+                const double* src = reinterpret_cast<const double*>(&cc);
+                double* dst = reinterpret_cast<double*>(&cl);
+#pragma unroll
+                for (int k = 0; k < S::XSHIFT; k++) dst[k] = lane_up(src[k]);
+            }
+            double wx[MW][M], sx[MW], amx[M], apx[M];
+            Cache cx;
+            S::template rpn<1, 2>(P, cl, cc, wx, sx, amx, apx, cx);
+            courant(sx, dtdx, xcfl_lane);
+            double cq[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) cq[m] = 0.0;
+            double dr[MW], nn[MW];
+#pragma unroll
+            for (int mw = 0; mw < MW; mw++) {
+                // wave mw of the interface to the right (lane+1)
+                double wr[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) wr[m] = lane_dn(wx[mw][m]);
+                double d = 0.0, n2 = 0.0;
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    n2 = n2 + wx[mw][m] * wx[mw][m];
+                    d = d + wx[mw][m] * wr[m];
+                }
+                dr[mw] = d;
+                nn[mw] = n2;
+            }
+            double dl[MW];  // dotr of the interface to the left (lane-1)
+#pragma unroll
+            for (int mw = 0; mw < MW; mw++) dl[mw] = lane_up(dr[mw]);
+#pragma unroll
+            for (int mw = 0; mw < MW; mw++) {
+                const double c = coef(sx[mw], sx[mw] > 0.0 ? dl[mw] : dr[mw], nn[mw], dtdx);
+#pragma unroll
+                for (int m = 0; m < M; m++) cq[m] = cq[m] + c * wx[mw][m];
+            }
+            double amt[M], apt[M], lf[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                amt[m] = amx[m] + cq[m];
+                apt[m] = apx[m] - cq[m];
+                lf[m] = amx[m] + 0.5 * cq[m];   // to the left cell
+                apx[m] = apx[m] - 0.5 * cq[m];  // to the right cell
+            }
+            double bma[M], bpa[M], bmp[M], bpp[M];
+            S::template rpt<1, 2>(P, cx, amt, bma, bpa);
+            S::template rpt<1, 2>(P, cx, apt, bmp, bpp);
+            // what the interface to the right (lane+1) owes this cell,
+            // exchanged two components at a time
+#pragma unroll
+            for (int m = 0; m < M; m += 2) {
+                double lfr[2], bmar[2], bpar[2];
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    lfr[e] = lane_dn(lf[m + e]);
+                    bmar[e] = lane_dn(bma[m + e]);
+                    bpar[e] = lane_dn(bpa[m + e]);
+                }
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int c = m + e;
+                    const double b1 = bmar[e] + bmp[c];  // G1 / (-dt/2dx)
+                    const double b2 = bpar[e] + bpp[c];  // G2 / (-dt/2dx)
+                    double v2 = qc[c] + kt * b2prev[c];
+                    v2 = v2 - dtdx * apx[c];
+                    v2 = v2 - dtdx * lfr[e];
+                    vC[c] = v2 - kt * (b2 - b1);
+                    vB[c] = vB[c] - kt * b1;
+                    b2prev[c] = b2;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < M; m++) vC[m] = 0.0;
+        }
+
+        double vout[M];  // cell (i, J-2) when it is complete and this lane owns it
+        bool store = false;
+#pragma unroll
+        for (int m = 0; m < M; m++) vout[m] = 0.0;
+
+        // ------------------------------ y-interface J (flux2, ixy=2) on column i
+        S::template rpn<2, 1>(P, yp.cell, cc, yc.w, yc.s, yc.am, yc.ap, yc.cache);
+        if (J >= 1 && J <= my + 1) courant(yc.s, dtdy, ycfl_lane);
+#pragma unroll
+        for (int mw = 0; mw < MW; mw++) {
+            double dr = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; m++) dr = dr + yp.w[mw][m] * yc.w[mw][m];
+            yc.dotl[mw] = dr;  // dotr of interface J-1 = dotl of interface J
+        }
+        if (J - 1 >= r0) {
+            // interface J-1 (held in yp), limited with its neighbours J-2 and J
+            double cq[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) cq[m] = 0.0;
+#pragma unroll
+            for (int mw = 0; mw < MW; mw++) {
+                double nn = 0.0;
+#pragma unroll
+                for (int m = 0; m < M; m++) nn = nn + yp.w[mw][m] * yp.w[mw][m];
+                const double c =
+                    coef(yp.s[mw], yp.s[mw] > 0.0 ? yp.dotl[mw] : yc.dotl[mw], nn, dtdy);
+#pragma unroll
+                for (int m = 0; m < M; m++) cq[m] = cq[m] + c * yp.w[mw][m];
+            }
+            double amt[M], apt[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                amt[m] = yp.am[m] + cq[m];
+                apt[m] = yp.ap[m] - cq[m];
+            }
+            double bma[M], bpa[M], bmp[M], bpp[M];
+            S::template rpt<2, 1>(P, yp.cache, amt, bma, bpa);
+            S::template rpt<2, 1>(P, yp.cache, apt, bmp, bpp);
+            // B^- parts of the columns to the right, B^+ parts of those to the
+            // left, two components at a time
+            if (J - 2 >= r0) store = out_lane && J - 2 < r1;  // cell (i, J-2) completes
+#pragma unroll
+            for (int m = 0; m < M; m += 2) {
+                double bmar[2], bmpr[2], bpal[2], bppl[2];
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    bmar[e] = lane_dn(bma[m + e]);
+                    bmpr[e] = lane_dn(bmp[m + e]);
+                    bpal[e] = lane_up(bpa[m + e]);
+                    bppl[e] = lane_up(bpp[m + e]);
+                }
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int c = m + e;
+                    if (J - 2 >= r0) {
+                        // cell (i, J-2): its upper face
+                        const double t = (bpal[e] - bmar[e]) - (bpa[c] - bma[c]);
+                        const double v2 = vA[c] - dtdy * (yp.am[c] + 0.5 * cq[c]);
+                        vout[c] = v2 + kt * t;
+                    }
+                    // cell (i, J-1): its lower face
+                    const double t = (bppl[e] - bmpr[e]) - (bpp[c] - bmp[c]);
+                    const double v2 = vB[c] - dtdy * (yp.ap[c] - 0.5 * cq[c]);
+                    vB[c] = v2 + kt * t;
+                }
+            }
+        }
+        {
+            double* dst = store ? qn + (long)(J - 2) * pitch : g_store_sink + lane;
+            const long stride = store ? fs : 64;
+#pragma unroll
+            for (int m = 0; m < M; m++) dst[m * stride] = vout[m];
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            vA[m] = vB[m];
+            vB[m] = vC[m];
+        }
+    };
+
+    const int nrow = r1 - r0 + 3;
+    const int jend = r0 - 1 + nrow + (nrow & 1);
+#pragma unroll
+    for (int m = 0; m < M; m++) g_store_sink[m * 64 + lane] = 0.0;
+    for (int J = r0 - 1; J < jend; J += 2) {
+        row(ysA, ysB, J);
+        row(ysB, ysA, J + 1);
+    }
+    publish_cfl(cfl_sum == cfl_sum ? cfl : __builtin_huge_val(), a.cfl_bits);
+}
+
 // The work of one wavefront: strip `strip` (64 columns) and row segment `seg`
 // of the grid described by a; BC: how ghost cells are read (ColumnReader).
 template <class S, class Cfg, int BC = 0>
@@ -298,6 +560,9 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
 {
     if constexpr (std::is_same_v<S, Acoustics2<FastMath>> && Cfg::fixed) {
         acoustics_wave_fast<Cfg, BC>(a, strip, seg, lane);
+        return;
+    } else if constexpr (S::Math::fast && Cfg::fixed) {
+        step2_wave_fast<S, Cfg, BC>(a, strip, seg, lane);
         return;
     }
     using Mth = typename S::Math;
