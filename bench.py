@@ -38,6 +38,26 @@ WORKLOAD_A = "acoustics2d_4096x4096_pulse_mc_unsplit_transverse_walls_200steps"
 BYTES_PER_CELL_UPDATE_A = 48  # meqn = 3, maux = 0
 
 
+def measured_ceilings(ndoubles=1 << 27):
+    """Ceilings measured on this device: the best of the library's copy kernels
+    (1 GiB read + 1 GiB written per copy) and the FP64 FMA throughput of
+    independent chains on every SIMD."""
+    from classic_amd import _lib
+
+    lib = _lib.load()
+    best = None
+    for v in range(4):
+        ms = C.c_float(0.0)
+        _lib.check(lib.clawgpu_copy_bandwidth(ndoubles, v, 5, C.byref(ms)))
+        gbs = 2 * ndoubles * 8 / (ms.value * 1e-3) / 1e9
+        if best is None or gbs > best[0]:
+            best = (gbs, v)
+    ms = C.c_float(0.0)
+    tf = C.c_double(0.0)
+    _lib.check(lib.clawgpu_fp64_fma_peak(20000, C.byref(ms), C.byref(tf)))
+    return {"copy_GB_s": best[0], "copy_variant": best[1], "fp64_fma_TFLOP_s": tf.value}
+
+
 def profile_traffic(name):
     """HBM bytes per step-kernel launch measured with rocprofv3 counters
     (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a
@@ -178,6 +198,7 @@ def bench3(args):
     g.close()
     kavg_s = kern_ms / ksteps / 1e3
     achieved = cells * BYTES_PER_CELL_UPDATE3 / kavg_s / 1e9
+    ceil = measured_ceilings()
     cpu = None
     if not args.no_cpu:
         v, el, sample, kind, th = cpu_baseline3(args.cpu_n3, args.seed)
@@ -199,6 +220,8 @@ def bench3(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profile_traffic("acoustics3d512") if n == 512 else None},
+        "measured_ceilings": ceil,
+        "frac_of_measured_copy": achieved / ceil["copy_GB_s"],
         "cpu_baseline": cpu,
     }
 
@@ -525,6 +548,7 @@ def main():
     g.set_arith(1 - arith)
     _, kern_other, _ = g.time_steps(dx, dy, dt_last, method, mthlim, mthbc, ksteps)
     g.close()
+    ceil = measured_ceilings()
     copy_cal = None
     if args.copy_calibration:
         from classic_amd import _lib
@@ -569,6 +593,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (profile_traffic(traffic_name)
                                  if n == 4096 and args.arith == "fast" else None)},
+        "measured_ceilings": ceil,
+        "frac_of_measured_copy": achieved / ceil["copy_GB_s"],
         "cpu_baseline": cpu,
     }
     if copy_cal:

@@ -63,6 +63,8 @@ _SIGS = {
     "clawgpu_learned_time": ([_V, _d, _d, _d, _i, C.POINTER(C.c_float), C.POINTER(C.c_float)], _i),
     "clawgpu_time_steps": ([_V, _d, _d, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),
     "clawgpu_copy_calibration": ([C.c_long, _i, _F], _i),
+    "clawgpu_copy_bandwidth": ([C.c_long, _i, _i, _F], _i),
+    "clawgpu_fp64_fma_peak": ([_i, _F, _D], _i),
     "clawgpu_batch_create": ([C.POINTER(_V), _i, _i, _I, _I, _I, _D, _I, _D, _D], _i),
     "clawgpu_batch_destroy": ([_V], _i),
     "clawgpu_batch_set_q": ([_V, _i, _D], _i),

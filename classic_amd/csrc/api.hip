@@ -852,7 +852,128 @@ __global__ void calib_copy(const double2* __restrict__ s, double2* __restrict__ 
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (long)gridDim.x * blockDim.x)
         d[e] = s[e];
 }
+// a copy with U 16-byte loads in flight per lane (block-contiguous chunks of
+// U * blockDim pairs), nontemporal when NT
+template <int U, bool NT>
+__global__ void copy_unrolled(const double2* __restrict__ s, double2* __restrict__ d, long n2)
+{
+    const long step = (long)gridDim.x * blockDim.x * U;
+    for (long e = (long)blockIdx.x * blockDim.x * U + threadIdx.x; e < n2; e += step) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long k = e + (long)u * blockDim.x;
+            if (k < n2) {
+                if constexpr (NT) {
+                    v[u].x = __builtin_nontemporal_load(&s[k].x);
+                    v[u].y = __builtin_nontemporal_load(&s[k].y);
+                } else {
+                    v[u] = s[k];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long k = e + (long)u * blockDim.x;
+            if (k < n2) {
+                if constexpr (NT) {
+                    __builtin_nontemporal_store(v[u].x, &d[k].x);
+                    __builtin_nontemporal_store(v[u].y, &d[k].y);
+                } else {
+                    d[k] = v[u];
+                }
+            }
+        }
+    }
+}
+
+// eight independent FMA chains per lane; the sum is stored so that nothing
+// is optimised away
+__global__ void fp64_fma_chains(double* out, int iters, double a, double b)
+{
+    double x[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) x[c] = 1e-3 * (threadIdx.x + c);
+    for (int k = 0; k < iters; k++) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) x[c] = fma(x[c], a, b);
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) t += x[c];
+    if (t == -1.0) out[blockIdx.x] = t;  // never true for these inputs
+}
 }  // namespace
+
+extern "C" int clawgpu_copy_bandwidth(long ndoubles, int variant, int reps, float* ms_avg)
+{
+    if (ndoubles < 2 || reps < 1 || !ms_avg || variant < 0 || variant > 3)
+        return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    if (variant == 0) return clawgpu_copy_calibration(ndoubles, reps, ms_avg);
+    ndoubles &= ~1L;
+    double *a = nullptr, *b = nullptr;
+    clawgpu_ctx* c = nullptr;
+    HIPCHK(c, hipMalloc((void**)&a, ndoubles * sizeof(double)));
+    if (hipMalloc((void**)&b, ndoubles * sizeof(double)) != hipSuccess) {
+        (void)hipFree(a);
+        return set_err(CLAWGPU_ERR_HIP, "allocation failed");
+    }
+    hipLaunchKernelGGL(calib_fill, dim3(4096), dim3(256), 0, 0, a, ndoubles);
+    const long n2 = ndoubles / 2;
+    auto launch = [&]() {
+        if (variant == 1)
+            hipLaunchKernelGGL((copy_unrolled<4, false>), dim3(4096), dim3(256), 0, 0, (const double2*)a, (double2*)b, n2);
+        else if (variant == 2)
+            hipLaunchKernelGGL((copy_unrolled<4, true>), dim3(4096), dim3(256), 0, 0, (const double2*)a, (double2*)b, n2);
+        else
+            hipLaunchKernelGGL((copy_unrolled<8, true>), dim3(2048), dim3(256), 0, 0, (const double2*)a, (double2*)b, n2);
+    };
+    launch();  // warm
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++) launch();
+    HIPCHK(c, hipEventRecord(e1, 0));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    *ms_avg = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_fp64_fma_peak(int iters, float* ms, double* tflops)
+{
+    if (iters < 1 || !ms || !tflops) return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    clawgpu_ctx* c = nullptr;
+    int dev = 0, ncu = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int nblocks = ncu * 8;  // 8 waves per SIMD of 256-lane blocks
+    double* out = nullptr;
+    HIPCHK(c, hipMalloc((void**)&out, nblocks * sizeof(double)));
+    hipLaunchKernelGGL(fp64_fma_chains, dim3(nblocks), dim3(256), 0, 0, out, iters / 10 + 1,
+                       0.999999, 1e-7);  // warm
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, 0));
+    hipLaunchKernelGGL(fp64_fma_chains, dim3(nblocks), dim3(256), 0, 0, out, iters, 0.999999, 1e-7);
+    HIPCHK(c, hipEventRecord(e1, 0));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    *tflops = 2.0 * 8.0 * (double)iters * nblocks * 256.0 / (t * 1e-3) / 1e12;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    return CLAWGPU_OK;
+}
 
 extern "C" int clawgpu_copy_calibration(long ndoubles, int reps, float* ms_avg)
 {

@@ -295,7 +295,9 @@ struct Euler2 {
                 double km = qmu * qmu;
                 double tm = gg1 * (qme * qm0 - 0.5 * (km + qmv * qmv));
                 bool tr1 = s0 < 0.0 && qmu > 0.0 && tm >= 0.0 && km > tm;
-                double f1 = s[0] < 0.0 ? s[0] : 0.0;
+                // without a transonic 1-wave the left-going part is min(s1, 0):
+                // a supersonic interface (s0 >= 0 and s1 > 0) gives 0 as well
+                double f1 = fmin(s[0], 0.0);
                 if (tr1) {
                     double ri = Mth::rcp(qm0);
                     double pm = g1 * (qme - 0.5 * (km + qmv * qmv) * ri);
@@ -309,16 +311,17 @@ struct Euler2 {
                 double kn = qnu * qnu;
                 double tn = gg1 * (qne * qn0 - 0.5 * (kn + qnv * qnv));
                 bool tr3 = left2 && s3 > 0.0 && qnu < 0.0 && tn >= 0.0 && kn > tn;
-                double f3 = s[2] < 0.0 ? s[2] : 0.0;
+                // s1 <= s2 <= s3 after rounding (u - a <= u <= u + a, a >= 0):
+                // where the 2-wave is not left-going (or the interface is
+                // supersonic) min(s, 0) of the 2- and 3-waves is already 0
+                double f3 = fmin(s[2], 0.0);
                 if (tr3) {
                     double ri = Mth::rcp(qn0);
                     double pn = g1 * (qne - 0.5 * (kn + qnv * qnv) * ri);
                     double s2 = qnu * ri + Mth::sqrt_(gamma * pn * ri);
                     f3 = s2 * (s3 - s[2]) / (s3 - s2);
                 }
-                f1 = super ? 0.0 : f1;
-                const double f2 = left2 ? s[1] : 0.0;
-                f3 = left2 ? f3 : 0.0;
+                const double f2 = fmin(s[1], 0.0);
                 const double g1c = s[0] - f1, g2c = s[1] - f2, g3c = s[2] - f3;
 #pragma unroll
                 for (int m = 0; m < M; m++) {

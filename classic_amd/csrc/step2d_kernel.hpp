@@ -359,7 +359,10 @@ __device__ __forceinline__ void step2_wave_fast(const StepArgs& a, const int str
     // second-order coefficient of a wave of speed s, limiter factor included
     auto coef = [&](double s, double num, double wn2, double dtd) {
         double phi = philim<Mth>(Mth::ratio(num, wn2), Cfg::lim);
-        phi = wn2 != 0.0 ? phi : 1.0;
+        // minmod, superbee and MC give a finite phi even for the 0/0 ratio of
+        // a zero wave, whose components are then zero whatever phi is; van
+        // Leer and Beam-Warming would give NaN there
+        if constexpr (Cfg::lim == 3 || Cfg::lim == 5) phi = wn2 != 0.0 ? phi : 1.0;
         const double sa = fabs(s);
         return (sa * (1.0 - sa * dtd)) * phi;
     };

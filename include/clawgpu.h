@@ -217,6 +217,19 @@ int clawgpu_learned_time(clawgpu_ctx* ctx, double dx, double dy, double dt, int 
  * traffic for calibrating the FETCH_SIZE / WRITE_SIZE counters. */
 int clawgpu_copy_calibration(long ndoubles, int reps, float* ms_avg);
 
+/* Measurement helpers (no reference counterpart): the ceilings the step
+ * kernels are measured against on the device in use.
+ * clawgpu_copy_bandwidth: reps device copies of ndoubles doubles with copy
+ *   kernel `variant` (0: the calibration copy above, one 16-byte load per
+ *   lane per iteration; 1: four 16-byte loads in flight per lane; 2: as 1
+ *   with nontemporal loads and stores; 3: eight in flight, nontemporal);
+ *   *ms_avg = average time per copy (2 x 8 x ndoubles bytes move).
+ * clawgpu_fp64_fma_peak: FP64 fused multiply-add throughput, eight
+ *   independent chains per lane over every SIMD of the device, iters
+ *   steps per chain; *tflops = 2 flops per FMA / time. */
+int clawgpu_copy_bandwidth(long ndoubles, int variant, int reps, float* ms_avg);
+int clawgpu_fp64_fma_peak(int iters, float* ms, double* tflops);
+
 /* --------------------------------------------------------- batched 2D */
 
 /* Many independent 2D grids advanced together (no reference counterpart as
