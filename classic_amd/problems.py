@@ -94,12 +94,15 @@ def acoustics3d_heterogeneous(n=20, mbc=2):
     return q, layered_medium_aux(n, n, n, mbc=mbc)
 
 
-def acoustics3d_pulse(n=512, seed=0, mbc=2, width=0.01):
+def acoustics3d_pulse(n=512, seed=0, mbc=2, width=0.01, center=None):
     """3D acoustics on [-1,1]^3, q = (p, u, v, w): p = exp(-|x - x0|^2 / width^2)
-    with x0 drawn from default_rng(seed).uniform(-0.5, 0.5, 3), u = v = w = 0;
-    aux = (Z, c): rho = 1, c = 1 (Z = 1) for z < 0 and rho = 3, c = 2 (Z = 6)
-    above.  Returns (q, aux, x0)."""
-    x0 = np.random.default_rng(seed).uniform(-0.5, 0.5, 3)
+    with x0 drawn from default_rng(seed).uniform(-0.5, 0.5, 3) (or the given
+    center), u = v = w = 0; aux = (Z, c): rho = 1, c = 1 (Z = 1) for z < 0
+    and rho = 3, c = 2 (Z = 6) above.  Returns (q, aux, x0)."""
+    if center is None:
+        x0 = np.random.default_rng(seed).uniform(-0.5, 0.5, 3)
+    else:
+        x0 = np.asarray(center, dtype=np.float64)
     xc, _ = cell_centers(-1.0, 1.0, n, mbc)
     q = np.zeros((n + 2 * mbc,) * 3 + (4,))
     ex = np.exp(-((xc - x0[0]) ** 2) / width ** 2)

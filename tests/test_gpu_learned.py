@@ -51,6 +51,107 @@ def test_table_limiter_with_mc_fallback_equals_mc(arith):
         assert max_rel_err(res[1][0], res[0][0]) <= 1e-13
 
 
+def _oracle_features(q, dx, dy, dt):
+    from oracle import oracle as O
+
+    return MO.features(q, dx, dy, dt, O.solver_params(gamma=1.4, efix=True))
+
+
+def _device_features(q, dx, dy, dt, arith):
+    n = q.shape[0] - 4
+    with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith) as g:
+        g.set_params(Lm.init_params(7))
+        g.set_q(q)
+        g.bc2(BC)
+        qb = g.get_q()  # ghost cells filled by the device bc2 (bit for bit the oracle's)
+        g.features(dx, dy, dt)
+        nw = g.nwaves()
+        ntab = nw // 3
+        feat = g.read(0).reshape(2, ntab, 3, 8)[..., :5]
+        theta = g.read(3).reshape(2, ntab, 3)
+    return qb, feat, theta
+
+
+@pytest.mark.parametrize("arith", [0, 1])
+def test_features_vs_oracle(arith):
+    """The feature pre-pass (slot_features: three Riemann solves per table
+    slot) against the oracle's restatement on the 96^2 evolved state.  Exact
+    arithmetic: theta bit for bit, the float features equal.  Fast
+    arithmetic: see _fast_close."""
+    q, dx, dy = _state()
+    dt = 0.2 * dx
+    qb, feat, theta = _device_features(q, dx, dy, dt, arith)
+    ofeat, oth, onz = _oracle_features(qb, dx, dy, dt)
+    assert onz.sum() > 5000
+    live = feat[..., 3] > 0
+    np.testing.assert_array_equal(live, onz & (ofeat[..., 3] > 0))
+    if arith == 0:
+        np.testing.assert_array_equal(theta, oth)
+        np.testing.assert_array_equal(feat, ofeat)
+    else:
+        _fast_close(theta, oth, feat, ofeat)
+
+
+def _fast_close(theta, oth, feat, ofeat):
+    """Fast arithmetic computes a wave of strength eps from O(1) states to
+    about 1e-16 absolute, so theta, a ratio of the wave's dot products, is
+    only as good as 1e-16 / eps relatively; and beyond +-3 nothing uses it
+    (the feature is clamped there, MC is 2 and the TVD bound is 2).  So theta
+    is compared clamped to [-3, 3] on waves whose strength relative to the
+    two states (feature 3) is at least 1e-6: within 1e-9; the float features
+    of every wave within 1e-6."""
+    sig = ofeat[..., 3] >= 1e-6
+    ct, co = np.clip(theta, -3.0, 3.0), np.clip(oth, -3.0, 3.0)
+    dth = np.abs(ct - co)[sig] / np.maximum(np.abs(co[sig]), 1.0)
+    d = np.abs(feat.astype(np.float64) - ofeat.astype(np.float64)) / np.maximum(np.abs(ofeat), 1.0)
+    print(f"theta ({int(sig.sum())} significant waves): max {dth.max():.2e}; "
+          f"features: max {d.max():.2e}", flush=True)
+    assert dth.max() <= 1e-9
+    assert d.max() <= 1e-6
+
+
+def test_features_vs_oracle_1024():
+    """The same at the learned workload's size (1024^2, a few steps into the
+    four-quadrant run), fast arithmetic as the benchmark runs it."""
+    n = 1024
+    q, dx, dy = P.euler_quadrants(n, n, seed=200)
+    with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0], arith=1) as g:
+        g.set_q(q)
+        g.claw2(dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 8, METHOD, [4, 4, 4], BC)
+        q = g.get_q()
+    dt = 0.25 * dx
+    qb, feat, theta = _device_features(q, dx, dy, dt, 1)
+    ofeat, oth, onz = _oracle_features(qb, dx, dy, dt)
+    assert onz.sum() > 50_000
+    _fast_close(theta, oth, feat, ofeat)
+
+
+@pytest.mark.parametrize("arith", [0, 1])
+def test_fused_prepass_phi_vs_oracle_network(arith):
+    """The phi tables a limiter-6 step computes (fused pre-pass: features,
+    BF16 MFMA network, TVD clip with MC fallback) against the oracle network
+    on the oracle's features: within BF16 rounding, except where the clip
+    decision flips at the region's edge; zero waves get phi = 0."""
+    q, dx, dy = _state()
+    n = q.shape[0] - 4
+    dt = 0.2 * dx
+    params = Lm.init_params(7)
+    with Lm.LearnedGrid(SOLVER_EULER, n, n, [1.4, 1.0], arith=arith) as g:
+        g.set_params(params)
+        g.set_q(q)
+        g.bc2(BC)
+        qb = g.get_q()
+        g.step2(dx, dy, dt, METHOD, [6, 6, 6])
+        phi = np.concatenate([g.read(1), g.read(2)]).reshape(2, -1, 3)
+    ofeat, oth, onz = _oracle_features(qb, dx, dy, dt)
+    live = onz & (ofeat[..., 3] > 0)
+    raw, _ = MO.forward(params.astype(np.float64), ofeat[live].astype(np.float64))
+    ref, _ = MO.clip(raw, oth[live])
+    d = np.abs(phi[live] - ref)
+    assert np.median(d) < 1e-2 and np.mean(d < 5e-2) > 0.99, (np.median(d), np.mean(d < 5e-2))
+    assert not phi[~onz].any()
+
+
 def _tables(g):
     nw = g.nwaves()
     return g.read(0), g.read(3), np.concatenate([g.read(1), g.read(2)]), nw

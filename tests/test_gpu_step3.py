@@ -248,3 +248,66 @@ def test_material_table_limit():
         assert len({(a, b) for a, b in aux.reshape(-1, 2)}) == nmat
         with _grid(p, aux, q) as g:
             assert g.set_aux_table(True) == want
+
+
+def test_config3_full_size_100_steps_symmetry_and_last_step():
+    """Config 3 at its size and length: 512^3 layered acoustics, walls,
+    superbee, unsplit method 22, CFL 0.9, 100 accepted steps of the claw3
+    loop, with the pulse centred in x and y (at z = 0.3, above the
+    interface) so that the medium and the data are mirror-symmetric in x and
+    y.  (i) The run takes its 100 steps; (ii) after them the solution is
+    mirror-symmetric in x (p, v, w even, u odd) and in y (p, u, w even, v
+    odd) to 1e-12 of each field's largest value -- an index or tile-edge
+    slip anywhere breaks it; (iii) the 100th step agrees with the oracle run
+    on 24^3 windows of the 99-step state (on the wave fronts and across the
+    material interface)."""
+    n, w, mbc = 512, 24, 2
+    q, aux, _ = P.acoustics3d_pulse(n, center=(0.0, 0.0, 0.3), width=0.05)
+    dx = 2.0 / n
+    method, mthlim, mthbc = (1, 2, 22, 0, 0, 0, 2), (2, 2), (3, 3, 3, 3, 3, 3)
+    with Grid3D(n, n, n) as g:
+        g.set_aux(aux)
+        g.set_q(q)
+        del q
+        r = g.claw3(0.0, 1e30, [0.9 * dx / 2.0, 1e99], [1.0, 0.9], 99, method=method,
+                    mthlim=mthlim, mthbc=mthbc)
+        assert r.info == 11 and r.nv[1] == 99  # stopped by the step count, as asked
+        q99 = g.get_q()
+        r2 = g.claw3(r.tend, 1e30, [r.dtv[4], 1e99], [1.0, 0.9], 1, method=method,
+                     mthlim=mthlim, mthbc=mthbc)
+        assert r2.nv[1] == 1
+        dt = r2.dtv[2]
+        q100 = g.get_q()
+    print(f"512^3: 100 steps to t = {r2.tend:.4f}", flush=True)
+    inner = q100[2:-2, 2:-2, 2:-2]
+    for axis, odd in ((2, 1), (1, 2)):  # array axes (z, y, x, m): x is 2, y is 1
+        flip = np.flip(inner, axis=axis)
+        for m in range(4):
+            top = np.abs(inner[..., m]).max()
+            sgn = -1.0 if m == odd else 1.0
+            err = np.abs(inner[..., m] - sgn * flip[..., m]).max() / top
+            print(f"mirror axis {axis} field {m}: {err:.2e} of {top:.3e}", flush=True)
+            assert err <= TOL
+        del flip
+    # windows of the last step: where the pressure and where the vertical
+    # velocity peak, the centre column across the interface at z = 0, and the
+    # front where it crosses the interface off-centre
+    def around(field):
+        kz, jy, ix = np.unravel_index(np.argmax(np.abs(inner[..., field])), inner.shape[:3])
+        return tuple(min(max(int(c) + 1 - w // 2, 1), n - w + 1) for c in (ix, jy, kz))
+
+    h = n // 2 - w // 2 + 1
+    corners = [around(0), around(3), (h, h, h), (h + 40, h, h)]
+    full = O3.Problem3(n, n, n, mthbc=mthbc, mthlim=mthlim, method=method)
+    q99 = O3.bc3(full, q99)
+    for i0, j0, k0 in corners:
+        sl = tuple(slice(c - 1, c + w + 3) for c in (k0, j0, i0))
+        lower = tuple(-1.0 + (c - 1) * dx for c in (i0, j0, k0))
+        upper = tuple(lo + w * dx for lo in lower)
+        win = O3.Problem3(w, w, w, mthbc=mthbc, mthlim=mthlim, method=method, lower=lower,
+                          upper=upper)
+        ref, _ = O3.step3(win, q99[sl], aux[sl], dt)
+        err = max_rel_err3(q100[sl], ref)
+        print(f"window at ({i0}, {j0}, {k0}): max rel err {err:.2e}", flush=True)
+        assert np.abs(ref[2:-2, 2:-2, 2:-2] - q99[sl][2:-2, 2:-2, 2:-2]).max() > 1e-8
+        assert err <= TOL
