@@ -45,6 +45,9 @@ _SIGS = {
     "clawgpu_set_aux": ([_V, _D], _i),
     "clawgpu_bc2": ([_V, _I], _i),
     "clawgpu_step2": ([_V, _d, _d, _d, _I, _I, _i, _D], _i),
+    # device pointers (qold, qnew, cfl) and the stream as plain addresses
+    "clawgpu_step2_noalloc": ([_i, _D, _i, _i, _i, _i, _i, C.c_long, C.c_long, _V, _V, _d, _d, _d,
+                               _I, _I, _i, _V, _V], _i),
     "clawgpu_dimsp2": ([_V, _d, _d, _d, _I, _I, _D, _i, _D], _i),
     "clawgpu_claw2": ([_V, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
     "clawgpu_claw2_hooks": ([_V, _d, _d, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),

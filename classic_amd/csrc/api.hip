@@ -319,13 +319,31 @@ static bool uses_table(const clawgpu_ctx* c, const int* mthlim)
     return false;
 }
 
-// Rows per wavefront segment of the unsplit step when none was set.  Every
+// Rows per wavefront segment of the unsplit step when none was set (also used
+// by clawgpu_step2_noalloc, noalloc.hip).  Every
 // wave of the kernel does the same work, a segment of s rows costing s + 3
 // rows (halo rows, padded to an even count), and the chip runs the waves in
 // rounds of (CUs x waves one CU holds).  The choice minimises rounds x rows
 // per wave over s in [8, 72] (longer segments measured slower than this
 // count predicts); 4096^2 Euler gets 70 rows: 4 rounds of 74 rows instead
 // of 9 rounds of 36 with 32-row segments.
+int segment_rows_for(int ncu, int waves_per_cu, int nstrips, int my)
+{
+    const long slots = (long)(ncu > 0 ? ncu : 1) * (waves_per_cu > 0 ? waves_per_cu : kWavesPerBlock);
+    int best = 32;
+    long best_cost = -1;
+    for (int rows = 8; rows <= 72; rows++) {
+        const long waves = (long)nstrips * ((my + rows - 1) / rows);
+        const long rounds = (waves + slots - 1) / slots;
+        const long cost = rounds * (rows + 3 + ((rows + 3) & 1));
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = rows;
+        }
+    }
+    return best;
+}
+
 static int step_rows(clawgpu_ctx* c, const StepArgs& a)
 {
     if (c->seglen > 0) return c->seglen;
@@ -335,18 +353,7 @@ static int step_rows(clawgpu_ctx* c, const StepArgs& a)
         return c->auto_rows;
     const int wpc = c->arith == ARITH_FAST ? step2_waves_per_cu_fast(c->solver, a)
                                            : step2_waves_per_cu_exact(c->solver, a);
-    const long slots = (long)(c->ncu > 0 ? c->ncu : 1) * (wpc > 0 ? wpc : kWavesPerBlock);
-    int best = 32;
-    long best_cost = -1;
-    for (int rows = 8; rows <= 72; rows++) {
-        const long waves = (long)a.nstrips * ((c->my + rows - 1) / rows);
-        const long rounds = (waves + slots - 1) / slots;
-        const long cost = rounds * (rows + 3 + ((rows + 3) & 1));
-        if (best_cost < 0 || cost < best_cost) {
-            best_cost = cost;
-            best = rows;
-        }
-    }
+    const int best = segment_rows_for(c->ncu, wpc, a.nstrips, c->my);
     c->auto_key[0] = c->arith;
     c->auto_key[1] = (int)mc;
     c->auto_key[2] = c->my;

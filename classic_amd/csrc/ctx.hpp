@@ -50,6 +50,10 @@ inline int set_err(int code, const char* msg)
     return code;
 }
 
+// Rows per wavefront segment of the unsplit step for a grid of nstrips
+// strips and my rows on ncu compute units holding waves_per_cu waves each.
+int segment_rows_for(int ncu, int waves_per_cu, int nstrips, int my);
+
 // SolverParams of a 2D solver from the (rho, bulk) / (gamma, efix) argument
 // pair of clawgpu_create; CLAWGPU_OK or an error code with the message set.
 int solver_params(int solver_id, const double* params, int nparams, SolverParams* P);

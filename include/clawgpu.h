@@ -99,6 +99,27 @@ int clawgpu_bc2(clawgpu_ctx* ctx, const int* mthbc);
 int clawgpu_step2(clawgpu_ctx* ctx, double dx, double dy, double dt, const int* method,
                   const int* mthlim, int use_fwave, double* cfl);
 
+/* No counterpart in the reference: clawgpu_step2 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No context: the caller passes device
+ * buffers and the call allocates nothing, its only scratch (the Courant
+ * accumulator) being a 256-byte array the library holds per GPU until the
+ * process ends.  One unsplit step of solver_id (ACOUSTICS or EULER, params
+ * as clawgpu_create, arith EXACT or FAST) from qold (ghost cells set) into
+ * the interior cells of qnew, both in field-plane layout: element (m, i, j),
+ * 1-based cell indices, at m*plane + (j+mbc-1)*pitch + (i+mbc-1), with
+ * pitch >= mx+2*mbc, plane >= pitch*(my+2*mbc), qold != qnew.  The Courant
+ * number is written to the device double *cfl.  Everything is enqueued on
+ * `stream` (a hipStream_t of the current device; NULL = the null stream) and
+ * nothing waits: read *cfl after the stream is done.  mx, my <= 1048576,
+ * 2 <= mbc <= 64; method and mthlim as clawgpu_step2 with limiters 0..5.
+ * Calls that overlap in time on one GPU (unordered streams or host threads)
+ * share the scratch and race; calls ordered on one stream do not. */
+int clawgpu_step2_noalloc(int solver_id, const double* params, int nparams, int arith, int mx,
+                          int my, int mbc, long pitch, long plane, const double* qold,
+                          double* qnew, double dx, double dy, double dt, const int* method,
+                          const int* mthlim, int use_fwave, double* cfl, void* stream);
+
 /* One dimensionally split step, the contract of dimsp2
  * (src/2d/dimsp2.f:4-98) with step2ds (src/2d/step2ds.f90:2-275):
  * method(3) = -1 Godunov (x then y) or -2 Strang (x dt/2, y dt, x dt/2); the
