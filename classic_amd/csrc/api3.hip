@@ -685,7 +685,7 @@ extern "C" int clawgpu3_set_segment(clawgpu3_ctx* c, int cells)
 
 extern "C" int clawgpu3_set_tile(clawgpu3_ctx* c, int tile, int xcd_order)
 {
-    if (!c || tile < 0 || tile > 4) return set_err(CLAWGPU_ERR_ARG, "tile must be 0..4");
+    if (!c || tile < 0 || tile > 6) return set_err(CLAWGPU_ERR_ARG, "tile must be 0..6");
     if (xcd_order < 0 || xcd_order > 3) return set_err(CLAWGPU_ERR_ARG, "tile order must be 0..3");
     c->tile = tile;
     c->xcd = xcd_order;

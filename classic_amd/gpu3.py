@@ -199,9 +199,11 @@ class Grid3D:
         return n.value
 
     def set_tile(self, tile, xcd_order=2):
-        """Tile shape (0..4) and tile order of the unsplit kernel: order bit
-        0 gives each XCD a contiguous range of tiles, bit 1 numbers tiles
-        with z fastest (the default, 2)."""
+        """Tile shape of the unsplit kernel (0..4: tile kernels of 4x4, 4x2,
+        2x2, 8x2, 2x4 y-z cells; 5, 6: z-marching kernel with 10 or 6 rows
+        per block, which needs the material table and otherwise runs shape
+        4) and tile order: order bit 0 gives each XCD a contiguous range of
+        tiles, bit 1 numbers tiles with z fastest (tile kernels only)."""
         check(self.lib.clawgpu3_set_tile(self.h, tile, int(xcd_order)))
 
     def time_steps(self, dt, nsteps, method=(1, 2, 22, 0, 0, 0, 2), mthlim=(3, 3),

@@ -101,6 +101,64 @@ def test_step3_every_tile_shape(tile, table):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("tile", [5, 6])
+@pytest.mark.parametrize("m3,lim,nmat", [(22, (2, 2), 2), (22, (3, 4), 7), (21, (1, 4), 3),
+                                         (20, (4, 4), 2), (11, (0, 0), 3), (10, (0, 0), 5),
+                                         (0, (3, 3), 2)])
+def test_step3_zmarch_matches_oracle(tile, m3, lim, nmat):
+    """The z-marching kernel (rows of 10 or 6 per block, 64 planes per
+    block) against the oracle for every method(3) family, with the
+    coefficient table (<= 4 materials) and without it: 13 rows (ragged
+    row tiles), 70 planes (a second, ragged z segment) and 70 columns (a
+    ragged second strip)."""
+    rng = np.random.default_rng(60 + tile + m3)
+    order = 1 if lim == (0, 0) else 2
+    p = O3.Problem3(70, 13, 70, mthlim=lim, method=(1, order, m3, 0, 0, 0, 0),
+                    mthbc=(3, 1, 2, 2, 3, 3))
+    q, aux = _blocky_medium(p, rng, nmat)
+    dt = 0.3 * p.geom.dx / 2.0
+    ref, cfl_ref = O3.step3(p, q, aux, dt)
+    for order_bits in (0, 1):
+        with _grid(p, aux, q) as g:
+            g.set_tile(tile, order_bits)
+            assert g.set_aux_table(True) == nmat
+            cfl = g.step3(dt, method=p.method, mthlim=lim)
+            out = g.get_q()
+        assert cfl == cfl_ref
+        err = max_rel_err3(out, ref)
+        print(f"zmarch tile {tile} m3 {m3} nmat {nmat} order {order_bits}: max rel err {err:.2e}",
+              flush=True)
+        assert err <= TOL
+
+
+@pytest.mark.parametrize("tile", [5, 6])
+@pytest.mark.parametrize("m3,lim", [(22, (2, 2)), (21, (3, 4)), (11, (4, 4)), (20, (1, 1))])
+def test_step3_zmarch_layered_medium(tile, m3, lim):
+    """Materials that vary along z only (layers of 1 to 5 planes, three
+    materials): most waves see one material in every 3x3 neighbourhood and
+    take the kernel's one-material form of the transverse records, the
+    waves at a layer interface the general form."""
+    rng = np.random.default_rng(80 + tile + m3)
+    p = O3.Problem3(70, 13, 70, mthlim=lim, method=(1, 2, m3, 0, 0, 0, 0),
+                    mthbc=(3, 3, 3, 3, 3, 3))
+    q = O3.bc3(p, rng.standard_normal(p.shape()))
+    pairs = np.array([[1.0, 1.0], [6.0, 2.0], [2.5, 0.7]])
+    nz = p.shape(2)[0]
+    layer = np.repeat(np.arange(nz), rng.integers(1, 6, nz))[:nz] % 3
+    aux = np.ascontiguousarray(np.broadcast_to(pairs[layer][:, None, None, :], p.shape(2)))
+    dt = 0.3 * p.geom.dx / 2.0
+    ref, cfl_ref = O3.step3(p, q, aux, dt)
+    with _grid(p, aux, q) as g:
+        g.set_tile(tile, 1)
+        assert g.set_aux_table(True) == 3
+        cfl = g.step3(dt, method=p.method, mthlim=lim)
+        out = g.get_q()
+    assert cfl == cfl_ref
+    err = max_rel_err3(out, ref)
+    print(f"zmarch layered tile {tile} m3 {m3}: max rel err {err:.2e}", flush=True)
+    assert err <= TOL
+
+
 def test_step3_larger_grid_many_tiles():
     rng = np.random.default_rng(9)
     p = O3.Problem3(125, 34, 23, mthlim=(3, 4))
