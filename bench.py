@@ -175,8 +175,11 @@ def bench3(args):
     q0, aux, _ = P.acoustics3d_pulse(n, seed=args.seed)
     method, mthlim, mthbc = (1, 2, 22, 0, 0, 0, 2), (2, 2), (3, 3, 3, 3, 3, 3)
     g = Grid3D(n, n, n)
-    if args.tile_order >= 0:
-        g.set_tile(4, args.tile_order)
+    if args.tile >= 0 or args.tile_order >= 0:
+        tile, order = g.get_tile()
+        g.set_tile(args.tile if args.tile >= 0 else tile,
+                   args.tile_order if args.tile_order >= 0 else order)
+    tile_used = g.get_tile()
     g.set_aux(aux)
     del aux
     g.set_q(q0)
@@ -214,6 +217,7 @@ def bench3(args):
                    "my": n, "mz": n, "seed": args.seed, "method": list(method),
                    "mthlim": list(mthlim), "mthbc": list(mthbc), "cfl_desired": 0.9,
                    "cfl_max": 1.0},
+        "tile_shape": tile_used[0], "tile_order": tile_used[1],
         "hbm_frac_of_peak": value * BYTES_PER_CELL_UPDATE3 / HBM_PEAK_GBS,
         "kernel_ms_avg": kavg_s * 1e3,
         "step_overhead_ms_avg": (total_ms - kern_ms) / ksteps,
@@ -456,6 +460,9 @@ def main():
     ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--n3", type=int, default=512)
     ap.add_argument("--cpu-n3", type=int, default=160)
+    ap.add_argument("--tile", type=int, default=-1,
+                    help="acoustics3d: tile shape of the unsplit kernel (0..4 tile kernels, "
+                         "5, 6 z-marching kernel; -1: the library's default)")
     ap.add_argument("--tile-order", type=int, default=-1,
                     help="acoustics3d: tile order of the unsplit kernel (bit 0 XCD-contiguous, "
                          "bit 1 z fastest; -1: the library's default)")

@@ -456,6 +456,22 @@ extern "C" int clawgpu_step2(clawgpu_ctx* c, double dx, double dy, double dt, co
 static int enqueue_sweep(clawgpu_ctx* c, int dir, int from, int to, double dtd, const int* method,
                          const int* mthlim, int use_fwave, int slot)
 {
+    if (general_path(c)) {
+        g2::SplitArgs g;
+        g.qin = c->dq[from];
+        g.qout = c->dq[to];
+        g.aux = c->daux;
+        g.L = c->L;
+        g.afs = c->L.fs;
+        g.dtd = dtd;
+        g.capa = method[5];
+        g.order = method[1];
+        g.fwave = use_fwave ? 1 : 0;
+        g.lim0 = mthlim[0];
+        g.cfl_bits = c->dcfl + slot;
+        HIPCHK(c, (hipError_t)launch_split_sweep_general(dir, g, c->stream));
+        return CLAWGPU_OK;
+    }
     SweepArgs a;
     a.qin = c->dq[from];
     a.qout = c->dq[to];
@@ -521,9 +537,9 @@ extern "C" int clawgpu_dimsp2(clawgpu_ctx* c, double dx, double dy, double dt, c
     if (!c || !method || !mthlim || !cflv || !cfl) return set_err(CLAWGPU_ERR_ARG, "null argument");
     if (method[2] != -1 && method[2] != -2) return set_err(CLAWGPU_ERR_ARG, "method(3) must be -1 or -2");
     if (method[1] != 1 && method[1] != 2) return set_err(CLAWGPU_ERR_ARG, "method(2) must be 1 or 2");
-    if (general_path(c))
-        return set_err(CLAWGPU_ERR_UNSUPPORTED, "split steps are not supported for solvers with aux arrays");
-    if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported");
+    if (method[5] != 0 && !general_path(c))
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported by this solver");
+    if (method[5] < 0 || method[5] > c->maux) return set_err(CLAWGPU_ERR_ARG, "method(6) must be 0..maux");
     for (int mw = 0; mw < c->mwaves; mw++)
         if (mthlim[mw] < 0 || mthlim[mw] > 5) return set_err(CLAWGPU_ERR_ARG, "invalid limiter method");
     HIPCHK(c, hipMemsetAsync(c->dcfl, 0, 4 * sizeof(unsigned long long), c->stream));
@@ -650,10 +666,13 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
         return set_err(CLAWGPU_ERR_ARG, "method(3) must be -2, -1, 0, 1 or 2");
     if (split && uses_table(c, mthlim))
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "the learned limiter runs with the unsplit step only");
-    if (split && general_path(c))
-        return set_err(CLAWGPU_ERR_UNSUPPORTED, "split steps are not supported for solvers with aux arrays");
     if (method[5] != 0 && !general_path(c))
-        return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported");
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported by this solver");
+    if (split && (method[5] < 0 || method[5] > c->maux))
+        return set_err(CLAWGPU_ERR_ARG, "method(6) must be 0..maux");
+    if (split)
+        for (int mw = 0; mw < c->mwaves; mw++)
+            if (mthlim[mw] < 0 || mthlim[mw] > 5) return set_err(CLAWGPU_ERR_ARG, "invalid limiter method");
     if (!user_bc && check_bc(mthbc) != 0)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides)");
     std::vector<double> hq, saved;  // host copies of q for the hooks

@@ -328,8 +328,12 @@ extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int mau
     c->L.nx = mx + 2 * mbc; c->L.ny = my + 2 * mbc; c->L.nz = mz + 2 * mbc;
     c->L.fs = (long)c->L.nx * c->L.ny * c->L.nz;
     c->seg = 64;
-    c->tile = 4;
-    c->xcd = 2;  // z fastest: more halo re-reads served by the caches
+    // z-marching blocks of 60x6 columns, each XCD on a contiguous range of
+    // blocks (8.2 ms per 512^3 layered step against 13.2 ms for the best
+    // tile kernel); without a material table the tile kernel 60x2x4 runs,
+    // its tiles numbered z fastest (more halo re-reads served by the caches)
+    c->tile = 6;
+    c->xcd = 3;
     c->cmax = 0.0;
     const size_t planes = (size_t)c->L.fs * sizeof(double);
     int rc = CLAWGPU_OK;
@@ -689,6 +693,14 @@ extern "C" int clawgpu3_set_tile(clawgpu3_ctx* c, int tile, int xcd_order)
     if (xcd_order < 0 || xcd_order > 3) return set_err(CLAWGPU_ERR_ARG, "tile order must be 0..3");
     c->tile = tile;
     c->xcd = xcd_order;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_get_tile(const clawgpu3_ctx* c, int* tile, int* xcd_order)
+{
+    if (!c || !tile || !xcd_order) return set_err(CLAWGPU_ERR_ARG, "null argument");
+    *tile = c->tile;
+    *xcd_order = c->xcd;
     return CLAWGPU_OK;
 }
 

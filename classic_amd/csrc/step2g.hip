@@ -29,6 +29,71 @@ namespace g2 {
 
 __device__ inline double lim_phi(double r, int lim) { return philim<ExactMath>(r, lim); }
 
+// One slice position of flux2 for the variable-coefficient advection
+// solver: cells p-2..p+2 along the slice, the four interfaces p-1..p+2
+// between them (t = 0..3: between local cells t and t+1), the Courant
+// contribution of interface p (and of n+1 for the last position) and the
+// second-order corrections of interfaces p (cq[1]) and p+1 (cq[2]).
+struct Pos {
+    double q[5], dtd[5], w[4], s[4], am[4], ap[4], cq[3];
+    double cfl;
+};
+
+// off(pp) gives the offset of cell pp of the slice; dtb = dt/dx or dt/dy
+// (divided by the capacity per cell when capa > 0, flux2.f90:38-45).
+template <class Off>
+__device__ inline void eval_position(const double* q, const double* aux, long afs, const double* vel,
+                                     Off off, int p, int n, bool live, double dtb, int capa,
+                                     int order, int lim0, int fwave, Pos& o)
+{
+    double un[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const long c = off(p - 2 + k);
+        o.q[k] = q[c];
+        un[k] = vel[c];
+        o.dtd[k] = capa > 0 ? dtb / aux[(long)(capa - 1) * afs + c] : dtb;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        o.w[t] = o.q[t + 1] - o.q[t];
+        o.s[t] = un[t + 1];
+        o.am[t] = fmin(o.s[t], 0.0) * o.w[t];
+        o.ap[t] = fmax(o.s[t], 0.0) * o.w[t];
+    }
+    // Courant number: interface p (t = 1) and, for the last position, n+1
+    // (flux2.f90:123-132)
+    o.cfl = 0.0;
+    if (live) {
+        const double* dtd = o.dtd;
+        const double* sp = o.s;
+        o.cfl = fmax(dtd[2] * sp[1], -dtd[1] * sp[1]);
+        if (p == n) o.cfl = fmax(o.cfl, fmax(dtd[3] * sp[2], -dtd[2] * sp[2]));
+        if (o.cfl != o.cfl || sp[1] != sp[1] || (p == n && sp[2] != sp[2])) o.cfl = __builtin_huge_val();
+    }
+    // second-order corrections of interfaces p (t = 1) and p+1 (t = 2)
+    // (flux2.f90:134-183)
+    o.cq[0] = o.cq[1] = o.cq[2] = 0.0;
+    if (order == 2) {
+#pragma unroll
+        for (int t = 1; t <= 2; t++) {
+            double wl = o.w[t];
+            if (lim0 != 0) {
+                const double wn2 = o.w[t] * o.w[t];
+                if (wn2 != 0.0) {
+                    const double dotl = o.w[t - 1] * o.w[t], dotr = o.w[t] * o.w[t + 1];
+                    const double r = (o.s[t] > 0.0 ? dotl : dotr) / wn2;
+                    wl = lim_phi(r, lim0) * o.w[t];
+                }
+            }
+            const double dtdxave = 0.5 * (o.dtd[t] + o.dtd[t + 1]);
+            const double sa = fabs(o.s[t]);
+            const double sw = fwave ? copysign(1.0, o.s[t]) : sa;
+            o.cq[t] = 0.0 + sw * (1.0 - sa * dtdxave) * wl;
+        }
+    }
+}
+
 // The x (DIR = 1) or y (DIR = 2) records of slice position (p, s): p runs
 // along the slice, s is the slice index.
 template <int DIR>
@@ -45,52 +110,14 @@ __global__ __launch_bounds__(256) void records_kernel(GenArgs a)
     auto off = [&](int pp, int ss) -> long { return DIR == 1 ? L.off(pp, ss) : L.off(ss, pp); };
     const double* vel = a.aux + (DIR == 1 ? 0 : a.afs);   // normal edge velocity (aux 1 or 2)
     const double* tvel = a.aux + (DIR == 1 ? a.afs : 0);  // transverse edge velocity
-    double q[5], dtd[5];
-    double un[5];
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-        const long c = off(pc - 2 + k, sc);
-        q[k] = a.q[c];
-        un[k] = vel[c];
-        dtd[k] = a.capa > 0 ? dtb / a.aux[(long)(a.capa - 1) * a.afs + c] : dtb;
-    }
-    // interfaces t = 0..3 (I = p-1+t) between local cells t and t+1
-    double w[4], s[4], am[4], ap[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        w[t] = q[t + 1] - q[t];
-        s[t] = un[t + 1];
-        am[t] = fmin(s[t], 0.0) * w[t];
-        ap[t] = fmax(s[t], 0.0) * w[t];
-    }
-    // Courant number: interface p (t = 1) and, for the last position, n+1
-    double cfl = 0.0;
-    if (live) {
-        cfl = fmax(dtd[2] * s[1], -dtd[1] * s[1]);
-        if (p == n) cfl = fmax(cfl, fmax(dtd[3] * s[2], -dtd[2] * s[2]));
-        if (cfl != cfl || s[1] != s[1] || (p == n && s[2] != s[2])) cfl = __builtin_huge_val();
-    }
-    publish_cfl(cfl, a.cfl_bits);
-    // second-order corrections of interfaces p (t = 1) and p+1 (t = 2)
-    double cq[3] = {0.0, 0.0, 0.0};
-    if (a.order == 2) {
-#pragma unroll
-        for (int t = 1; t <= 2; t++) {
-            double wl = w[t];
-            if (a.lim0 != 0) {
-                const double wn2 = w[t] * w[t];
-                if (wn2 != 0.0) {
-                    const double dotl = w[t - 1] * w[t], dotr = w[t] * w[t + 1];
-                    const double r = (s[t] > 0.0 ? dotl : dotr) / wn2;
-                    wl = lim_phi(r, a.lim0) * w[t];
-                }
-            }
-            const double dtdxave = 0.5 * (dtd[t] + dtd[t + 1]);
-            const double sa = fabs(s[t]);
-            const double sw = a.fwave ? copysign(1.0, s[t]) : sa;
-            cq[t] = 0.0 + sw * (1.0 - sa * dtdxave) * wl;
-        }
-    }
+    Pos ps;
+    eval_position(a.q, a.aux, a.afs, vel, [&](int pp) { return off(pp, sc); }, pc, n, live, dtb,
+                  a.capa, a.order, a.lim0, a.fwave, ps);
+    publish_cfl(ps.cfl, a.cfl_bits);
+    const double* dtd = ps.dtd;
+    const double* am = ps.am;
+    const double* ap = ps.ap;
+    const double* cq = ps.cq;
     const double fl = 0.5 * cq[1], fh = 0.5 * cq[2];
     const double qadd = (0.0 - dtd[2] * ap[1]) - dtd[2] * am[2];
     double g1 = 0.0, g2 = 0.0;
@@ -156,6 +183,51 @@ __global__ __launch_bounds__(256) void combine_kernel(GenArgs a)
     a.qn[c] = v;
 }
 
+
+// One sweep of the dimensionally split step (step2ds.f90:96-163 for x,
+// :186-255 for y) for the variable-coefficient advection solver: every
+// slice, ghost slices included, updates its cells 1..n by
+//   q + qadd - dtd (fadd(p+1) - fadd(p))   [/ capa(p) with a capacity],
+// with no transverse terms; the other cells are copied, so that ghost data
+// flow to the next sweep as in the reference's single array.  Lanes run
+// along x in both directions (coalesced rows).
+template <int DIR>
+__global__ __launch_bounds__(256) void split_sweep_kernel(SplitArgs a)
+{
+    const Layout2& L = a.L;
+    const int mbc = L.mbc;
+    const int n = DIR == 1 ? L.mx : L.my;
+    const int ns = DIR == 1 ? L.my : L.mx;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x + 1 - mbc;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y + 1 - mbc;
+    const int p = DIR == 1 ? ix : iy, sl = DIR == 1 ? iy : ix;
+    const bool in_grid = p <= n + mbc && sl <= ns + mbc;
+    const bool live = in_grid && p >= 1 && p <= n;
+    const int pc = live ? p : 1, sc = in_grid ? sl : 1 - mbc;
+    auto off = [&](int pp) -> long { return DIR == 1 ? L.off(pp, sc) : L.off(sc, pp); };
+    const double* vel = a.aux + (DIR == 1 ? 0 : a.afs);
+    Pos o;
+    eval_position(a.qin, a.aux, a.afs, vel, off, pc, n, live, a.dtd, a.capa, a.order, a.lim0,
+                  a.fwave, o);
+    publish_cfl(o.cfl, a.cfl_bits);
+    if (!in_grid) return;
+    if (!live) {
+        const long c = off(p);
+        a.qout[c] = a.qin[c];
+        return;
+    }
+    // qadd(p): A^+ of interface p, then A^- of interface p+1 (flux2.f90:113-121)
+    const double qadd = (0.0 - o.dtd[2] * o.ap[1]) - o.dtd[2] * o.am[2];
+    const double fl = 0.5 * o.cq[1], fh = 0.5 * o.cq[2];
+    const long c = off(p);
+    double v;
+    if (a.capa == 0)
+        v = o.q[2] + qadd - a.dtd * (fh - fl);
+    else
+        v = o.q[2] + qadd - a.dtd * (fh - fl) / a.aux[(long)(a.capa - 1) * a.afs + c];
+    a.qout[c] = v;
+}
+
 }  // namespace g2
 
 int launch_step2_general(const g2::GenArgs& a, hipStream_t s)
@@ -167,6 +239,18 @@ int launch_step2_general(const g2::GenArgs& a, hipStream_t s)
     hipLaunchKernelGGL(g2::records_kernel<2>, gy, blk, 0, s, a);
     const long n = (long)(a.L.mx + 2 * a.L.mbc) * (a.L.my + 2 * a.L.mbc);
     hipLaunchKernelGGL(g2::combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    return (int)hipGetLastError();
+}
+
+int launch_split_sweep_general(int dir, const g2::SplitArgs& a, hipStream_t s)
+{
+    const dim3 blk(64, 4);
+    const int nx = a.L.mx + 2 * a.L.mbc, ny = a.L.my + 2 * a.L.mbc;
+    const dim3 g((nx + 63) / 64, (ny + 3) / 4);
+    if (dir == 1)
+        hipLaunchKernelGGL(g2::split_sweep_kernel<1>, g, blk, 0, s, a);
+    else
+        hipLaunchKernelGGL(g2::split_sweep_kernel<2>, g, blk, 0, s, a);
     return (int)hipGetLastError();
 }
 

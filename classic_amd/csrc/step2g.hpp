@@ -21,9 +21,23 @@ struct GenArgs {
     unsigned long long* cfl_bits;
 };
 
+// one sweep of the dimensionally split step (step2ds)
+struct SplitArgs {
+    const double* __restrict__ qin;
+    double* __restrict__ qout;
+    const double* __restrict__ aux;
+    Layout2 L;
+    long afs;
+    double dtd;  // dt/dx (x sweep) or dt/dy (y sweep) of this sweep
+    int capa, order, fwave, lim0;
+    unsigned long long* cfl_bits;
+};
+
 }  // namespace g2
 
 // variable-coefficient advection, meqn = mwaves = 1
 int launch_step2_general(const g2::GenArgs& a, hipStream_t s);
+// dir 1: x sweep, 2: y sweep
+int launch_split_sweep_general(int dir, const g2::SplitArgs& a, hipStream_t s);
 
 }  // namespace clawgpu

@@ -206,6 +206,12 @@ class Grid3D:
         tiles, bit 1 numbers tiles with z fastest (tile kernels only)."""
         check(self.lib.clawgpu3_set_tile(self.h, tile, int(xcd_order)))
 
+    def get_tile(self):
+        """(tile shape, tile order) of the unsplit kernel."""
+        t, o = C.c_int(0), C.c_int(0)
+        check(self.lib.clawgpu3_get_tile(self.h, C.byref(t), C.byref(o)))
+        return t.value, o.value
+
     def time_steps(self, dt, nsteps, method=(1, 2, 22, 0, 0, 0, 2), mthlim=(3, 3),
                    mthbc=(3, 3, 3, 3, 3, 3), use_fwave=False):
         tot = C.c_float(0.0)

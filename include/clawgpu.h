@@ -385,10 +385,15 @@ int clawgpu3_claw3_hooks(clawgpu3_ctx* ctx, const double* geom, double tstart, d
 int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
 
 /* Tile shape of the unsplit kernel (tuning): 0 = 60x4x4, 1 = 60x4x2,
- * 2 = 60x2x2, 3 = 60x8x2, 4 = 60x2x4 (default) cells per wavefront;
- * xcd_order (tile order, default 2) bit 0 hands each XCD a contiguous
- * range of tiles, bit 1 numbers tiles with z fastest instead of x. */
+ * 2 = 60x2x2, 3 = 60x8x2, 4 = 60x2x4 cells per wavefront;
+ * 5, 6 (default) = z-marching blocks of 60x10 or 60x6 columns (these need
+ * the material table and otherwise run shape 4);
+ * xcd_order (tile order, default 3) bit 0 hands each XCD a contiguous
+ * range of tiles, bit 1 numbers tiles with z fastest instead of x (tile
+ * kernels only). */
 int clawgpu3_set_tile(clawgpu3_ctx* ctx, int tile, int xcd_order);
+/* The tile shape and order set (tuning). */
+int clawgpu3_get_tile(const clawgpu3_ctx* ctx, int* tile, int* xcd_order);
 
 /* Material table (no reference counterpart; same values, fewer bytes):
  * clawgpu3_set_aux finds the distinct (Z, c) pairs of aux; when there are at

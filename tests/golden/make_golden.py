@@ -7,7 +7,7 @@ solvers, on small problems, and stores inputs and outputs as .npz fixtures.
 The fixtures let the CPU tests pin the oracle on machines where the
 reference tree or flang is absent.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [name ...]   (all fixtures when no name is given)
 """
 from __future__ import annotations
 
@@ -121,7 +121,23 @@ def main():
     out["annulus_40x120"] = dict(q=np.array(frames), t=np.array([0.25, 0.5]),
                                  regression_sums=reg)
 
+    # 8) the annulus example taken by dimensional splitting (method(3) = -1
+    #    Godunov, -2 Strang) with its capacity function, to t = 0.25
+    split = {}
+    for m3 in (-1, -2):
+        p = O.Problem(1, 1, 40, 120, O.SOLVER_VC_ADVECTION, O.solver_params(), maux=3,
+                      method=(1, 2, m3, 0, 0, 3, 3), mthlim=[3], mthbc=(1, 1, 2, 2))
+        r = O.claw2(p, P.annulus_hump(40, 120), dxc, dyc, 0.0, 0.25, [0.1, 1e9], [1.0, 0.9],
+                    50000, aux=aux, backend="ref")
+        key = f"split{-m3}"
+        split.update({f"q_{key}": r.q[2:-2, 2:-2], f"dtv_{key}": r.dtv, f"nv_{key}": r.nv,
+                      f"info_{key}": r.info, f"tend_{key}": r.tend})
+    out["annulus_split_40x120"] = split
+
+    want = set(sys.argv[1:])  # names to write (all when none are given)
     for name, rec in out.items():
+        if want and name not in want:
+            continue
         np.savez_compressed(os.path.join(HERE, name + ".npz"),
                             **{k: np.asarray(v) for k, v in rec.items()})
         print("wrote", name)

@@ -29,6 +29,30 @@ def max_rel_err(a, b, mbc=2):
     return worst
 
 
+CELL_FLOOR = 1e-3
+
+
+def cell_rel_errs(a, b, mbc=2, floor=CELL_FLOOR):
+    """Relative error cell by cell over the interior, per field:
+    max |a - b| / max(|b|, floor * scale_m), scale_m as in max_rel_err.
+    Unlike max_rel_err, a cell far below its field's largest value (a
+    low-density corner, the tail of a pulse) is held to its own magnitude
+    down to floor times the field's scale; floor = 0 holds every cell to
+    itself (meaningful for fields bounded away from zero: density, energy,
+    the pressure of a pulse on a background)."""
+    nd = 3 if a.ndim == 4 else 2
+    sl = (slice(mbc, -mbc),) * nd
+    a = a[sl]
+    b = b[sl]
+    top = max(np.abs(b[..., m]).max() for m in range(b.shape[-1]))
+    out = []
+    for m in range(a.shape[-1]):
+        scale = max(np.abs(b[..., m]).max(), FIELD_FLOOR * top)
+        den = np.maximum(np.abs(b[..., m]), floor * scale)
+        out.append(float((np.abs(a[..., m] - b[..., m]) / np.where(den > 0, den, 1.0)).max()))
+    return out
+
+
 def smooth_acoustics(mx, my, seed=0, mbc=2):
     rng = np.random.default_rng(seed)
     q = np.zeros((my + 2 * mbc, mx + 2 * mbc, 3))

@@ -82,3 +82,47 @@ def test_annulus_example_bitwise_and_regression_sums():
             assert t == gold["t"][k]
             np.testing.assert_array_equal(qk[2:-2, 2:-2], gold["q"][k])
             assert abs(qk[2:-2, 2:-2, 0].sum() - gold["regression_sums"][k]) < 1e-10
+
+
+@pytest.mark.parametrize("m3", [-1, -2])
+@pytest.mark.parametrize("capa", [0, 3])
+@pytest.mark.parametrize("order,lim,fw", [(2, 4, 0), (2, 3, 0), (1, 0, 0), (2, 2, 1), (2, 1, 0)])
+def test_dimsp2_general_bitwise(m3, capa, order, lim, fw):
+    """Split step with aux arrays and the capacity function (step2ds with
+    index_capa > 0) against the oracle, bit for bit, Courant number
+    included; grid with ragged 64-lane rows and 4-row blocks."""
+    mx, my = 131, 70
+    q, aux = _random(mx, my, 7 * -m3 + 10 * order + lim + capa, capa)
+    method = (1, order, m3, 0, 0, capa, 3)
+    p = O.Problem(1, 1, mx, my, O.SOLVER_VC_ADVECTION, O.solver_params(), maux=3, method=method,
+                  mthlim=[lim], mthbc=(1, 1, 2, 2), use_fwave=bool(fw))
+    q = O.bc2(p, q)
+    dx, dy, dt = 1.0 / mx, 1.0 / my, 0.2 / my
+    ref, cfl_ref = O.dimsp2(p, q, dx, dy, dt, [1.0, 0.9], aux=aux)
+    with gpu.Grid2D(SOLVER_VC_ADVECTION, mx, my, [], maux=3) as g:
+        g.set_aux(aux)
+        g.set_q(q)
+        cfl = g.dimsp2(dx, dy, dt, method, [lim], (1.0, 0.9), use_fwave=bool(fw))
+        out = g.get_q()
+    assert cfl == cfl_ref
+    np.testing.assert_array_equal(out[2:-2, 2:-2], ref[2:-2, 2:-2])
+
+
+@pytest.mark.parametrize("m3", [-1, -2])
+def test_annulus_example_split_matches_reference_run(m3):
+    """The annulus example by Godunov / Strang splitting with its capacity
+    function, to t = 0.25 through the claw2 loop, against the reference
+    build's run (fixture): same steps, same dt, q bit for bit."""
+    gold = np.load(os.path.join(GOLD, "annulus_split_40x120.npz"))
+    aux, dxc, dyc = O.annulus_aux(40, 120)
+    key = f"split{-m3}"
+    with gpu.Grid2D(SOLVER_VC_ADVECTION, 40, 120, [], maux=3) as g:
+        g.set_aux(aux)
+        g.set_q(P.annulus_hump(40, 120))
+        r = g.claw2(dxc, dyc, 0.0, 0.25, [0.1, 1e9], [1.0, 0.9], 50000, (1, 2, m3, 0, 0, 3, 3),
+                    [3], (1, 1, 2, 2))
+        qk = g.get_q()
+    assert r["info"] == 0 and r["tend"] == float(gold["tend_" + key])
+    np.testing.assert_array_equal(r["nv"], gold["nv_" + key])
+    np.testing.assert_array_equal(r["dtv"], gold["dtv_" + key])
+    np.testing.assert_array_equal(qk[2:-2, 2:-2], gold["q_" + key])

@@ -12,7 +12,7 @@ import pytest
 from classic_amd import SOLVER_ACOUSTICS, SOLVER_EULER, gpu
 from classic_amd import problems as P
 from oracle import oracle as O
-from tests.helpers import euler_problem, max_rel_err, smooth_acoustics
+from tests.helpers import cell_rel_errs, euler_problem, max_rel_err, smooth_acoustics
 
 pytestmark = pytest.mark.gpu
 
@@ -245,9 +245,14 @@ def test_fast_arith_euler_200_steps(seed):
     assert out["nv"][1] == r.nv[1] == 200
     err = max_rel_err(q_g, r.q)
     dterr = abs(out["dtv"][4] - r.dtv[4]) / r.dtv[4]
-    print("fast euler 200 steps", seed, "max rel err", err, "dt rel err", dterr)
+    cerr = cell_rel_errs(q_g, r.q)
+    own = cell_rel_errs(q_g, r.q, floor=0.0)
+    print("fast euler 200 steps", seed, "max rel err", err, "cell rel err", cerr,
+          "density/energy cell by cell", own[0], own[3], "dt rel err", dterr)
     assert err <= TOL
     assert dterr <= TOL
+    # density and energy stay positive: every cell within the bar of itself
+    assert own[0] <= TOL and own[3] <= TOL
 
 
 def test_fast_arith_config1_acoustics():
@@ -261,8 +266,11 @@ def test_fast_arith_config1_acoustics():
         q_g = g.get_q()
     assert out["nv"][1] == r.nv[1]
     err = max_rel_err(q_g, r.q)
-    print("fast config1 max rel err", err)
+    cerr = cell_rel_errs(q_g, r.q)
+    own = cell_rel_errs(q_g, r.q, floor=0.0)
+    print("fast config1 max rel err", err, "cell rel err", cerr, "pressure cell by cell", own[0])
     assert err <= TOL
+    assert own[0] <= TOL  # p = 1 + pulse > 0
 
 
 @pytest.mark.parametrize("lim", [4, 2])
@@ -321,8 +329,12 @@ def test_fast_vs_exact_full_size_200_steps():
     assert o0["nv"][1] == o1["nv"][1] == 200
     err = max_rel_err(q1, q0)
     dterr = abs(o1["dtv"][4] - o0["dtv"][4]) / o0["dtv"][4]
-    print("fast vs exact 4096^2 x 200:", err, dterr)
+    cerr = cell_rel_errs(q1, q0)
+    own = cell_rel_errs(q1, q0, floor=0.0)
+    print("fast vs exact 4096^2 x 200:", err, "cell rel err", cerr, "density/energy cell by cell",
+          own[0], own[3], dterr)
     assert err <= TOL and dterr <= TOL
+    assert own[0] <= TOL and own[3] <= TOL
 
 
 def _toro1(mx, my, along_y=False):

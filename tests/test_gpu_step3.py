@@ -12,7 +12,7 @@ import pytest
 from classic_amd import problems as P
 from classic_amd.gpu3 import Grid3D
 from oracle import oracle3 as O3
-from tests.helpers import max_rel_err3, random_layered
+from tests.helpers import cell_rel_errs, max_rel_err3, random_layered
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -280,8 +280,10 @@ def _blocky_medium(p, rng, nmat):
 def test_material_table_bitwise_with_aux_path(m3, lim, nmat):
     """The one-byte material index reads the same doubles as aux, and with
     at most 4 materials the tabulated transverse coefficients are the bits
-    kcoef computes: the unsplit step is bit for bit the same with the tables
-    on and off, and within 1e-12 of the oracle."""
+    kcoef computes: the unsplit tile kernel is bit for bit the same with the
+    tables on and off, and within 1e-12 of the oracle.  (The z-marching
+    kernel, the default, reads the table only: without it the tile kernel
+    runs.)"""
     rng = np.random.default_rng(40 + m3 + nmat)
     p = O3.Problem3(MX, MY, MZ, mthlim=lim, method=(1, 2, m3, 0, 0, 0, 0))
     q, aux = _blocky_medium(p, rng, nmat)
@@ -290,6 +292,7 @@ def test_material_table_bitwise_with_aux_path(m3, lim, nmat):
     outs = []
     for table in (True, False):
         with _grid(p, aux, q) as g:
+            g.set_tile(4, 2)
             assert g.set_aux_table(table) == nmat
             cfl = g.step3(dt, method=p.method, mthlim=lim)
             outs.append(g.get_q())
@@ -369,3 +372,34 @@ def test_config3_full_size_100_steps_symmetry_and_last_step():
         print(f"window at ({i0}, {j0}, {k0}): max rel err {err:.2e}", flush=True)
         assert np.abs(ref[2:-2, 2:-2, 2:-2] - q99[sl][2:-2, 2:-2, 2:-2]).max() > 1e-8
         assert err <= TOL
+
+
+def test_config3_full_size_zmarch_vs_tile_kernel_100_steps():
+    """Config 3 at its size and length run by the two unsplit kernels, each
+    held to the oracle on small grids above: the z-marching kernel (the
+    default) and the 60x2x4 tile kernel, which share no code beyond the
+    interface solves and sum the increments in other orders.  Both take the
+    same 100 steps to the same dt; the states agree to 1e-12 per field."""
+    n = 512
+    q, aux, _ = P.acoustics3d_pulse(n, seed=0)
+    dx = 2.0 / n
+    method, mthlim, mthbc = (1, 2, 22, 0, 0, 0, 2), (2, 2), (3, 3, 3, 3, 3, 3)
+    res = []
+    for tile, order in ((6, 3), (4, 2)):
+        with Grid3D(n, n, n) as g:
+            g.set_tile(tile, order)
+            g.set_aux(aux)
+            g.set_q(q)
+            r = g.claw3(0.0, 1e30, [0.9 * dx / 2.0, 1e99], [1.0, 0.9], 100, method=method,
+                        mthlim=mthlim, mthbc=mthbc)
+            assert r.nv[1] == 100
+            res.append((r, g.get_q()))
+        print(f"tile {tile}: 100 steps to t = {r.tend:.6f}", flush=True)
+    del q, aux
+    (ra, qa), (rb, qb) = res
+    dterr = abs(ra.dtv[4] - rb.dtv[4]) / rb.dtv[4]
+    err = max_rel_err3(qa, qb)
+    cerr = max(cell_rel_errs(qa, qb))
+    print(f"z-marching vs tile kernel 512^3 x 100: max rel err {err:.2e}, cell rel err "
+          f"{cerr:.2e}, dt rel err {dterr:.2e}", flush=True)
+    assert err <= TOL and dterr <= TOL

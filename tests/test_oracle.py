@@ -468,3 +468,41 @@ def test_live_reference_vc_advection_random(method):
     b, cb = O.step2(p, q, 0.05, 0.07, 0.01, aux=aux, backend="ref")
     np.testing.assert_array_equal(a[2:-2, 2:-2], b[2:-2, 2:-2])
     assert ca == cb
+
+
+@pytest.mark.parametrize("m3", [-1, -2])
+def test_golden_annulus_split_with_capacity(m3):
+    """The annulus example by dimensional splitting (dimsp2/step2ds with
+    aux and capacity) against the reference build's run to t = 0.25."""
+    g = np.load(os.path.join(GOLD, "annulus_split_40x120.npz"))
+    aux, dxc, dyc = O.annulus_aux(40, 120)
+    p = O.Problem(1, 1, 40, 120, O.SOLVER_VC_ADVECTION, O.solver_params(), maux=3,
+                  method=(1, 2, m3, 0, 0, 3, 3), mthlim=[3], mthbc=(1, 1, 2, 2))
+    r = O.claw2(p, P.annulus_hump(40, 120), dxc, dyc, 0.0, 0.25, [0.1, 1e9], [1.0, 0.9], 50000,
+                aux=aux)
+    key = f"split{-m3}"
+    assert r.info == int(g["info_" + key]) == 0 and r.tend == float(g["tend_" + key])
+    np.testing.assert_array_equal(r.nv, g["nv_" + key])
+    np.testing.assert_array_equal(r.dtv, g["dtv_" + key])
+    np.testing.assert_array_equal(r.q[2:-2, 2:-2], g["q_" + key])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref absent")
+@pytest.mark.parametrize("m3", [-1, -2])
+@pytest.mark.parametrize("capa", [0, 3])
+@pytest.mark.parametrize("order,lim,fw", [(2, 4, 0), (2, 3, 0), (1, 0, 0), (2, 2, 1)])
+def test_live_reference_vc_advection_split(m3, capa, order, lim, fw):
+    """One split step (x sweeps over the ghost rows, y over the ghost
+    columns, with capacity-scaled dt/dx) against the reference build."""
+    rng = np.random.default_rng(10 * -m3 + capa + order + lim)
+    mx, my = 23, 17
+    aux = np.stack([rng.uniform(-1, 1, (my + 4, mx + 4)), rng.uniform(-1, 1, (my + 4, mx + 4)),
+                    rng.uniform(0.5, 1.5, (my + 4, mx + 4))], -1)
+    p = O.Problem(1, 1, mx, my, O.SOLVER_VC_ADVECTION, O.solver_params(), maux=3,
+                  method=(1, order, m3, 0, 0, capa, 3), mthlim=[lim], mthbc=(1, 1, 2, 2),
+                  use_fwave=bool(fw))
+    q = O.bc2(p, rng.standard_normal((my + 4, mx + 4, 1)))
+    a, ca = O.dimsp2(p, q, 0.05, 0.07, 0.01, [1.0, 0.9], aux=aux)
+    b, cb = O.dimsp2(p, q, 0.05, 0.07, 0.01, [1.0, 0.9], aux=aux, backend="ref")
+    np.testing.assert_array_equal(a[2:-2, 2:-2], b[2:-2, 2:-2])
+    assert ca == cb
