@@ -27,6 +27,7 @@
 #include <new>
 
 #include "ctx.hpp"
+#include "lanes.hpp"
 #include "riemann2d.hpp"
 
 namespace clawgpu {
@@ -94,6 +95,41 @@ struct FeatArgs {
     long ntab;
 };
 
+// The five features of one wave from its interface's data: dl, dr the dot
+// products of the wave with the same family's waves of the interfaces to
+// the left and right along the direction, wn2 its squared norm, sp its
+// speed, qn the sum of the two cells' state norms plus 1e-300, smax the
+// interface's largest speed magnitude, dtd = dt/dx (or dt/dy).
+template <class Mth>
+__device__ inline void wave_features(double dl, double dr, double wn2, double sp, double qn,
+                                     double smax, double dtd, float (&f)[5], double& th, bool& nz)
+{
+    nz = wn2 != 0.0;
+    th = 0.0;
+    for (int k = 0; k < 5; k++) f[k] = 0.0f;
+    if (!nz) return;
+    const double t = Mth::ratio(sp > 0.0 ? dl : dr, wn2);
+    const double td = Mth::ratio(sp > 0.0 ? dr : dl, wn2);
+    const double tc = fmin(fmax(t, -kThetaClip), kThetaClip);
+    const double tdc = fmin(fmax(td, -kThetaClip), kThetaClip);
+    if constexpr (Mth::fast) {
+        // the float features to a few double ulps: products with
+        // reciprocals instead of correctly rounded divisions
+        constexpr double inv_clip = 1.0 / kThetaClip;
+        f[0] = (float)(tc * inv_clip);
+        f[1] = (float)(tdc * inv_clip);
+        f[3] = (float)Mth::ratio(sqrt(wn2), qn);
+        f[4] = (float)Mth::ratio(sp, smax + 1e-300);
+    } else {
+        f[0] = (float)(tc / kThetaClip);
+        f[1] = (float)(tdc / kThetaClip);
+        f[3] = (float)(sqrt(wn2) / qn);
+        f[4] = (float)(sp / (smax + 1e-300));
+    }
+    f[2] = (float)(sp * dtd);
+    th = t;
+}
+
 // Features of the MW waves of interface (i, J) along direction dir: the
 // interface and its two neighbours along the direction are solved with the
 // step kernel's own arithmetic.  f receives five features per wave, th the
@@ -137,19 +173,9 @@ __device__ inline void slot_features(const FeatArgs& a, int dir, int i, int J,
             dl = dl + w[0][mw][m] * w[1][mw][m];
             dr = dr + w[1][mw][m] * w[2][mw][m];
         }
-        nz[mw] = wn2 != 0.0;
-        th[mw] = 0.0;
-        for (int k = 0; k < 5; k++) f[mw][k] = 0.0f;
-        if (!nz[mw]) continue;
         const double sp = s[1][mw];
-        const double t = Mth::ratio(sp > 0.0 ? dl : dr, wn2);
-        const double td = Mth::ratio(sp > 0.0 ? dr : dl, wn2);
-        f[mw][0] = (float)(fmin(fmax(t, -kThetaClip), kThetaClip) / kThetaClip);
-        f[mw][1] = (float)(fmin(fmax(td, -kThetaClip), kThetaClip) / kThetaClip);
-        f[mw][2] = (float)(sp * dtd);
-        f[mw][3] = (float)(sqrt(wn2) / qn);
-        f[mw][4] = (float)(sp / (smax + 1e-300));
-        th[mw] = t;
+        wave_features<Mth>(dl, dr, wn2, sp, qn, smax, dtd, f[mw], th[mw], nz[mw]);
+        if (!nz[mw]) continue;
         if (g) {
             const double c = dtd * 0.5 * fabs(sp) * (1.0 - fabs(sp) * dtd);
             for (int m = 0; m < M; m++) g[mw * M + m] = (float)(c * w[1][mw][m]);
@@ -328,65 +354,147 @@ __global__ __launch_bounds__(256) void mlp_infer_mfma_kernel(InferArgs a)
 // interface wave (as features_kernel) go through LDS straight into the MFMA
 // network, and only the phi tables are written -- the same values as
 // features_kernel followed by mlp_infer_mfma_kernel, without the feature,
-// theta and gradient tables in HBM.  Each block takes 256 table slots per
-// iteration (the trip count is the same for all its threads, so the block
-// barriers are reached by every thread); each wavefront runs its 64 slots'
-// 64*MW waves through the network as 4*MW tiles of 16.
+// theta and gradient tables in HBM.
+//
+// Work: a wavefront takes a line of table slots along one direction (a
+// table row for the x-interfaces, a table column for the y-interfaces; the
+// same code, with the lanes stepping through rows instead of columns for y)
+// and 61 consecutive slots of it.  Lane l holds the cell at position
+// p = 61*chunk + l - 3 along the line and solves the interface between
+// cells p-1 (from lane l-1) and p once; lanes 2..62 own the slots p, whose
+// neighbouring interfaces come from lanes l-1 and l+1 by cross-lane shifts.
+// So every interface is solved once (plus 3 halo lanes of 64), instead of
+// three times in features_kernel.
+//
+// Only a wave that is nonzero, has a nonzero float strength feature and
+// theta > 0 needs the network: for every other wave the clip's fallback
+// gives phi = MC(theta) (which is 0 for theta <= 0) whatever the network
+// says, and a zero wave keeps phi = 0.  Each wavefront compacts its waves
+// that need the network into an LDS queue (ballot + prefix count) and runs
+// only the queue through the MFMA tiles, 16 waves per tile; the rest are
+// written directly.  On evolved Euler states a small fraction of the waves
+// is nonzero (exactly constant regions give exact zero waves), so most of
+// the network work is skipped; the tables are the same as without the queue.
+constexpr int kLine = 61;  // slots per wavefront along its line
+
 template <class S>
-__global__ __launch_bounds__(256) void learned_fused_kernel(FeatArgs a, const float* params)
+__global__ __launch_bounds__(64) void learned_line_kernel(FeatArgs a, const float* params,
+                                                          int ntask)
 {
-    constexpr int MW = S::MW, NS = 64 * MW;
-    __shared__ bf16x4 xs[4][NS][2];
-    __shared__ float outs[4][NS];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    using Mth = typename S::Math;
+    using Cell = typename S::Cell;
+    constexpr int M = S::M, MW = S::MW, NS = 64 * MW;
+    __shared__ bf16x4 xs[NS][2];
+    __shared__ double qth[NS];
+    __shared__ int qix[NS];  // lane * MW + mw of the queued wave
+    const int lane = threadIdx.x;
     const int grp = lane >> 4, col = lane & 15;
-    const int mx = a.L.mx, my = a.L.my;
+    const Layout2 L = a.L;
+    const int mx = L.mx, my = L.my, W = mx + 4;
+    const int nchx = (mx + 4 + kLine - 1) / kLine, nchy = (my + 4 + kLine - 1) / kLine;
+    const int nxt = (my + 4) * nchx;  // x tasks: table rows x chunks
+    const SolverParams P = a.P;
     MlpFrag F;
     mlp_load(F, params, grp, col);
-    const long nt = 2 * a.ntab;
-    for (long base = (long)blockIdx.x * 256; base < nt; base += (long)gridDim.x * 256) {
-        const long t = base + threadIdx.x;
-        int dir = 0, i = 0, J = 0;
-        long slot = 0;
-        bool live = false;
-        if (t < nt) {
-            dir = (int)(t / a.ntab);
-            slot = t % a.ntab;
-            i = (int)(slot % (mx + 4)) - 1;
-            J = (int)(slot / (mx + 4)) - 1;
-            live = slot_live(dir, i, J, mx, my);
+    // Blocks are dealt to the 8 XCDs round-robin; XCD x works through the
+    // tasks [x*per, (x+1)*per) in order, so that the y lines of adjacent
+    // table columns (tasks in a row, chunk-major) share its L2: their strided
+    // loads and 24-byte table writes fall into the same cache lines.
+    const int per = (ntask + 7) / 8;
+    const int xcd = blockIdx.x & 7, nper = gridDim.x >> 3;
+    for (int u = blockIdx.x >> 3; u < per; u += nper) {
+        const int task = xcd * per + u;
+        if (task >= ntask) break;  // uniform over the block
+        const bool ydir = task >= nxt;
+        const int t = ydir ? task - nxt : task;
+        // x: line = table row J, row-major over chunks; y: line = table
+        // column i, chunk-major over columns
+        const int line = ydir ? t % (mx + 4) - 1 : t / nchx - 1;
+        const int chunk = ydir ? t / (mx + 4) : t % nchx;
+        const int n = ydir ? my : mx;
+        const int p = kLine * chunk + lane - 3;
+        const int pc = min(max(p, -1), n + 2);
+        const long cell0 = ydir ? L.off(line, pc) : L.off(pc, line);
+        double q[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) q[m] = a.q[m * L.fs + cell0];
+        const Cell ce = S::cell(P, q);
+        double n2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; m++) n2 = n2 + q[m] * q[m];
+        const double cn = sqrt(n2);
+        Cell cl;
+        {
+            const double* src = reinterpret_cast<const double*>(&ce);
+            double* dst = reinterpret_cast<double*>(&cl);
+#pragma unroll
+            for (int k = 0; k < S::XSHIFT; k++) dst[k] = lane_up(src[k]);
         }
-        float f[MW][5];
-        double th[MW];
-        bool nz[MW];
+        double w[MW][M], s[MW], am[M], ap[M];
+        typename S::Cache ch;
+        if (ydir)
+            S::template rpn<2, 1>(P, cl, ce, w, s, am, ap, ch);
+        else
+            S::template rpn<1, 2>(P, cl, ce, w, s, am, ap, ch);
+        const double qn = (lane_up(cn) + cn) + 1e-300;
+        double smax = 0.0;
+#pragma unroll
+        for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[mw]));
+        const bool own = lane >= 2 && lane <= kLine + 1 && p <= n + 2;
+        const bool live = own && (ydir ? (p >= 1 && p <= my + 1 && line >= 0 && line <= mx + 1)
+                                       : (p >= 1 && p <= mx + 1 && line >= 0 && line <= my + 1));
+        const int i = ydir ? line : p, J = ydir ? p : line;
+        const long e0 = ((long)(J + 1) * W + (i + 1)) * MW;
+        double* tab = ydir ? a.phiy : a.phix;
+        const double dtd = ydir ? a.dtdy : a.dtdx;
+        int count = 0;  // queued waves (uniform)
+#pragma unroll
         for (int mw = 0; mw < MW; mw++) {
-            nz[mw] = false;
-            th[mw] = 0.0;
-            for (int k = 0; k < 5; k++) f[mw][k] = 0.0f;
-        }
-        if (live) slot_features<S>(a, dir, i, J, f, th, nz, nullptr);
-        for (int mw = 0; mw < MW; mw++) {
-            xs[wv][lane * MW + mw][0] = to_bf16x4(f[mw][0], f[mw][1], f[mw][2], f[mw][3]);
-            xs[wv][lane * MW + mw][1] = to_bf16x4(f[mw][4], 0.0f, 0.0f, 0.0f);
-        }
-        __syncthreads();
-        for (int T = 0; T < NS / 16; T++) {
-            const int sidx = 16 * T + col;
-            bf16x4 x = {0, 0, 0, 0};
-            if (grp < 2) x = xs[wv][sidx][grp];
-            const float out = mlp_tile(F, x);
-            if (grp == 0) outs[wv][sidx] = out;
-        }
-        __syncthreads();
-        if (t < nt) {
-            double* phit = dir == 0 ? a.phix : a.phiy;
-            for (int mw = 0; mw < MW; mw++) {
-                double v = 0.0;
-                if (live && nz[mw])
-                    v = f[mw][3] > 0.0f ? clip_phi((double)outs[wv][lane * MW + mw], th[mw], nullptr)
-                                        : mc_phi(th[mw]);
-                phit[slot * MW + mw] = v;
+            double wn2 = 0.0, dl = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const double wl = lane_up(w[mw][m]);
+                wn2 = wn2 + w[mw][m] * w[mw][m];
+                dl = dl + wl * w[mw][m];
             }
+            const double dr = lane_dn(dl);
+            float f[5];
+            double th;
+            bool nz;
+            wave_features<Mth>(dl, dr, wn2, s[mw], qn, smax, dtd, f, th, nz);
+            const bool need = live && nz && f[3] > 0.0f && th > 0.0;
+            const unsigned long long mask = __ballot(need);
+            if (need) {
+                const int pos = count + (int)__builtin_amdgcn_mbcnt_hi(
+                                            (unsigned)(mask >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                xs[pos][0] = to_bf16x4(f[0], f[1], f[2], f[3]);
+                xs[pos][1] = to_bf16x4(f[4], 0.0f, 0.0f, 0.0f);
+                qth[pos] = th;
+                qix[pos] = lane * MW + mw;
+            } else if (own) {
+                tab[e0 + mw] = (live && nz) ? mc_phi(th) : 0.0;
+            }
+            count += __popcll(mask);
+        }
+        if (count > 0) {
+            __syncthreads();  // queue entries written by other lanes
+            // slot of lane l is e(l) = e0(lane 0) + l * (MW or W*MW) along the line
+            const long step = ydir ? (long)W * MW : MW;
+            const long ebase = e0 - (long)lane * step;
+            for (int T = 0; 16 * T < count; T++) {
+                const int sidx = 16 * T + col;
+                const bool ok = sidx < count;
+                bf16x4 x = {0, 0, 0, 0};
+                if (grp < 2 && ok) x = xs[sidx][grp];
+                const float out = mlp_tile(F, x);
+                if (grp == 0 && ok) {
+                    const int e = qix[sidx];
+                    tab[ebase + (long)(e / MW) * step + e % MW] =
+                        clip_phi((double)out, qth[sidx], nullptr);
+                }
+            }
+            __syncthreads();  // before the queue is written again
         }
     }
 }
@@ -715,22 +823,25 @@ static int enqueue_fused(clawgpu_ctx* c, LearnedState* s, double dx, double dy, 
     a.phix = c->phix;
     a.phiy = c->phiy;
     a.ntab = s->ntab;
-    long nb = (2 * s->ntab + 255) / 256;
-    if (nb > 2048) nb = 2048;
+    const long ntask = (long)(c->my + 4) * ((c->mx + 4 + kLine - 1) / kLine) +
+                       (long)(c->mx + 4) * ((c->my + 4 + kLine - 1) / kLine);
+    if (ntask > 0x7fffffffL) return set_err(CLAWGPU_ERR_ARG, "grid too large");
+    // one wavefront per block, a multiple of 8 blocks (one share per XCD);
+    // a grid-stride loop over the lines beyond 16384 blocks (64 per CU)
+    const int nb = (int)(ntask < 16384 ? (ntask + 7) / 8 * 8 : 16384);
+    const dim3 g(nb), b(64);
+    const float* p = s->params;
+    const int nt = (int)ntask;
     if (c->solver == CLAWGPU_SOLVER_ACOUSTICS) {
         if (c->arith == ARITH_FAST)
-            hipLaunchKernelGGL(learned_fused_kernel<Acoustics2<FastMath>>, dim3((int)nb), dim3(256), 0,
-                               c->stream, a, (const float*)s->params);
+            hipLaunchKernelGGL(learned_line_kernel<Acoustics2<FastMath>>, g, b, 0, c->stream, a, p, nt);
         else
-            hipLaunchKernelGGL(learned_fused_kernel<Acoustics2<ExactMath>>, dim3((int)nb), dim3(256), 0,
-                               c->stream, a, (const float*)s->params);
+            hipLaunchKernelGGL(learned_line_kernel<Acoustics2<ExactMath>>, g, b, 0, c->stream, a, p, nt);
     } else {
         if (c->arith == ARITH_FAST)
-            hipLaunchKernelGGL(learned_fused_kernel<Euler2<FastMath>>, dim3((int)nb), dim3(256), 0,
-                               c->stream, a, (const float*)s->params);
+            hipLaunchKernelGGL(learned_line_kernel<Euler2<FastMath>>, g, b, 0, c->stream, a, p, nt);
         else
-            hipLaunchKernelGGL(learned_fused_kernel<Euler2<ExactMath>>, dim3((int)nb), dim3(256), 0,
-                               c->stream, a, (const float*)s->params);
+            hipLaunchKernelGGL(learned_line_kernel<Euler2<ExactMath>>, g, b, 0, c->stream, a, p, nt);
     }
     HIPCHK(c, hipGetLastError());
     return CLAWGPU_OK;

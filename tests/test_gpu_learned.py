@@ -233,6 +233,54 @@ def test_fused_prepass_equals_features_then_inference(arith):
         assert np.mean(close) > 0.999 and d.max() < 5e-2, (np.mean(close), d.max())
 
 
+@pytest.mark.parametrize("solver", ["euler", "acoustics"])
+@pytest.mark.parametrize("arith", [0, 1])
+def test_prepass_ragged_grids_equal_features_then_inference(solver, arith):
+    """The pre-pass takes lines of 61 table slots per wavefront (table rows for
+    x, table columns for y), one Riemann solve per interface, and queues only
+    the waves that need the network: on grids whose sides are not multiples
+    of 61 (130 x 45 Euler, 75 x 123 acoustics, both evolved) its tables equal
+    the separate feature pass (three solves per slot, every wave) followed by
+    the MFMA inference over all waves -- bit for bit, every entry written."""
+    from classic_amd import SOLVER_ACOUSTICS
+
+    if solver == "euler":
+        mx, my, sid, prm, lim = 130, 45, SOLVER_EULER, [1.4, 1.0], [4, 4, 4]
+        q, dx, dy = P.euler_quadrants(mx, my, seed=5)
+        bc = BC
+    else:
+        mx, my, sid, prm, lim = 75, 123, SOLVER_ACOUSTICS, [1.0, 4.0], [4, 4]
+        q, dx, dy = P.acoustics_pulse(mx, my)
+        bc = (3, 3, 3, 3)
+    with Lm.LearnedGrid(sid, mx, my, prm, arith=arith) as g:
+        g.set_q(q)
+        g.claw2(dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], 12, METHOD, lim, bc)
+        q = g.get_q()
+    dt = 0.2 * dx
+    with Lm.LearnedGrid(sid, mx, my, prm, arith=arith) as g:
+        g.set_params(Lm.init_params(11))
+        nw = g.nwaves()
+        g.write(1, np.full(nw, np.nan))
+        g.write(2, np.full(nw, np.nan))
+        g.set_q(q)
+        g.bc2(bc)
+        g.step2(dx, dy, dt, METHOD, [6] * g.mwaves)
+        march = np.concatenate([g.read(1), g.read(2)])
+        g.write(1, np.full(nw, np.nan))
+        g.write(2, np.full(nw, np.nan))
+        g.set_q(q)  # the step advanced the device q
+        g.bc2(bc)
+        g.features(dx, dy, dt)
+        g.phi(0)
+        feat = g.read(0)
+        theta = g.read(3)
+        twopass = np.concatenate([g.read(1), g.read(2)])
+    need = (feat[:, 3] > 0) & (theta > 0)
+    print(f"{solver}: {need.sum()} of {need.size} waves through the network", flush=True)
+    assert need.sum() > 100 and not np.isnan(march).any()
+    np.testing.assert_array_equal(march, twopass)
+
+
 def test_timed_passes_leave_the_inference_tables():
     """The measurement entry point runs the fused pre-pass, a feature pass
     and the MFMA inference; its phi tables equal one feature + inference
