@@ -101,6 +101,7 @@ _SIGS = {
     "clawgpu3_claw3_hooks": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
     "clawgpu3_set_segment": ([_V, _i], _i),
     "clawgpu3_set_tile": ([_V, _i, _i], _i),
+    "clawgpu3_set_arith": ([_V, _i], _i),
     "clawgpu3_get_tile": ([_V, _I, _I], _i),
     "clawgpu3_set_aux_table": ([_V, _i, _I], _i),
     "clawgpu3_time_steps": ([_V, _D, _d, _I, _I, _I, _i, _i, _F, _F, _D], _i),

@@ -12,6 +12,7 @@
 
 #include "ctx.hpp"
 #include "step3d.hpp"
+#include "step3g.hpp"
 
 using namespace clawgpu::s3;
 
@@ -20,7 +21,7 @@ struct clawgpu3_ctx {
     Layout3 L;
     double* dq[3];    // q (field planes) and two work buffers
     int cur;
-    double* daux;     // Z and c planes
+    double* daux;     // aux planes (Z, c, then the rest of aux)
     double* staging;  // reference-layout copy for uploads and downloads
     double cmax;      // max |c| over the cells whose interfaces the sweeps visit
     double* exy;      // piston: exp(-500 r^2) per (i, j) column, from the host's libm
@@ -35,6 +36,8 @@ struct clawgpu3_ctx {
     int use_table = 1;              // read the material table instead of aux when nmat > 0
     int last_hip_error;
     std::vector<double> haux;       // host copy of aux for the user routines
+    int exact = 0;                  // unsplit step with the reference's rounding and order
+    double* drec = nullptr;         // records of the exact / capacity unsplit step
 };
 
 namespace {
@@ -156,7 +159,8 @@ int check_args3(clawgpu3_ctx* c, const int* method, const int* mthlim)
     if (method[2] < 0 && method[2] != -1)
         return set_err(CLAWGPU_ERR_ARG, "method(3) < 0 must be -1 (dimsp3 is Godunov splitting only)");
     if (check_method3(method) != 0) return set_err(CLAWGPU_ERR_ARG, "invalid method (chkmth info=6)");
-    if (method[5] != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "capacity function is not supported in 3D");
+    if (method[5] < 0 || method[5] > c->maux)
+        return set_err(CLAWGPU_ERR_ARG, "method(6) must be 0 or an aux index (capacity)");
     for (int w = 0; w < c->mwaves; w++)
         if (mthlim[w] < 0 || mthlim[w] > 5) return set_err(CLAWGPU_ERR_ARG, "mthlim must be 0..5");
     return CLAWGPU_OK;
@@ -247,12 +251,72 @@ Args3 step_args(clawgpu3_ctx* c, const double* q, double* qn, double dx, double 
     return a;
 }
 
+const double* capa_plane(const clawgpu3_ctx* c, const int* method)
+{
+    return method[5] > 0 ? c->daux + (long)(method[5] - 1) * c->L.fs : nullptr;
+}
+
+// The unsplit step: with a capacity function or in exact arithmetic the
+// records path (step3g.hip, the reference's rounding and order), else the
+// z-marching / tile kernels.
 int enqueue_step3(clawgpu3_ctx* c, int res, double dx, double dy, double dz, double dt,
                   const int* method, const int* mthlim, int use_fwave)
 {
+    if (method[5] > 0 || c->exact) {
+        if (c->L.mx + 2 > 8190 || c->L.my + 2 > 8190 || c->L.mz + 2 > 8190)
+            return set_err(CLAWGPU_ERR_UNSUPPORTED, "exact 3D step: sides up to 8188");
+        if (!c->drec) HIPCHK(c, hipMalloc((void**)&c->drec, rec3_doubles(c->L) * sizeof(double)));
+        Args3g g;
+        g.q = c->dq[c->cur];
+        g.qn = c->dq[res];
+        g.aux = c->daux;
+        g.capa = capa_plane(c, method);
+        g.rec = c->drec;
+        g.L = c->L;
+        g.dt3[0] = dt / dx;
+        g.dt3[1] = dt / dy;
+        g.dt3[2] = dt / dz;
+        g.m2 = method[1];
+        g.m3 = method[2] / 10;
+        g.m4 = method[2] - 10 * g.m3;
+        g.fwave = use_fwave ? 1 : 0;
+        g.lim0 = mthlim[0];
+        g.lim1 = mthlim[1];
+        g.dir = 0;
+        if (launch_step3_records(g, c->stream)) return set_err(CLAWGPU_ERR_HIP, "step3 launch failed");
+        return CLAWGPU_OK;
+    }
     Args3 a = step_args(c, c->dq[c->cur], c->dq[res], dx, dy, dz, dt, method, mthlim, use_fwave);
     const int e = launch_step3_tile(a, c->tile, c->stream);
     if (e) return set_err(CLAWGPU_ERR_HIP, "step3 kernel launch failed");
+    return CLAWGPU_OK;
+}
+
+// The Courant numbers of the x, y and z sweeps at this dt: dt/d * max|c|,
+// or with a capacity function the max over the cells of (dt/d / kappa) * c
+// (flux3's dtdx1d(i) s), computed on the device.  A NaN sound speed makes
+// all three NaN.
+int courant3(clawgpu3_ctx* c, const int* method, double dx, double dy, double dz, double dt,
+             double* cf)
+{
+    if (method[5] <= 0) {
+        cf[0] = dt / dx * c->cmax;
+        cf[1] = dt / dy * c->cmax;
+        cf[2] = dt / dz * c->cmax;
+        return CLAWGPU_OK;
+    }
+    unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->staging);
+    HIPCHK(c, hipMemsetAsync(bits, 0, 4 * sizeof(unsigned long long), c->stream));
+    const double d3[3] = {dt / dx, dt / dy, dt / dz};
+    if (launch_capa_cfl(c->daux + c->L.fs, capa_plane(c, method), c->L, d3, bits, c->stream))
+        return set_err(CLAWGPU_ERR_HIP, "Courant kernel launch failed");
+    unsigned long long h[4] = {0, 0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(h, bits, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int r = 0; r < 3; r++) {
+        std::memcpy(&cf[r], &h[r], sizeof(double));
+        if (h[3]) cf[r] = NAN;
+    }
     return CLAWGPU_OK;
 }
 
@@ -263,6 +327,7 @@ int enqueue_sweep3(clawgpu3_ctx* c, int dir, const double* q, double* qn, double
     a.q = q;
     a.qn = qn;
     a.aux = c->daux;
+    a.capa = capa_plane(c, method);
     a.L = c->L;
     a.afs = c->L.fs;
     a.d = d;
@@ -285,7 +350,10 @@ int enqueue_sweep3(clawgpu3_ctx* c, int dir, const double* q, double* qn, double
 int enqueue_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz, double dt, const int* method,
                    const int* mthlim, const double* cflv, int use_fwave, int* res, double* cfl)
 {
-    const double cx = dt / dx * c->cmax, cy = dt / dy * c->cmax, cz = dt / dz * c->cmax;
+    double cf[3];
+    int rc0 = courant3(c, method, dx, dy, dz, dt, cf);
+    if (rc0) return rc0;
+    const double cx = cf[0], cy = cf[1], cz = cf[2];
     const int o1 = (c->cur + 1) % 3, o2 = (c->cur + 2) % 3;
     int rc = enqueue_sweep3(c, 0, c->dq[c->cur], c->dq[o1], dt / dx, method, mthlim, use_fwave);
     if (rc) return rc;
@@ -302,9 +370,16 @@ int enqueue_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz, double dt, 
     return CLAWGPU_OK;
 }
 
-double unsplit_cfl(const clawgpu3_ctx* c, double dx, double dy, double dz, double dt)
+// the Courant number of an unsplit step (the max over all slices); NaN
+// when a sound speed is NaN
+int unsplit_cfl(clawgpu3_ctx* c, const int* method, double dx, double dy, double dz, double dt,
+                double* cfl)
 {
-    return fmax(dt / dx * c->cmax, fmax(dt / dy * c->cmax, dt / dz * c->cmax));
+    double cf[3];
+    int rc = courant3(c, method, dx, dy, dz, dt, cf);
+    if (rc) return rc;
+    *cfl = (cf[0] != cf[0]) ? cf[0] : fmax(cf[0], fmax(cf[1], cf[2]));
+    return CLAWGPU_OK;
 }
 
 }  // namespace
@@ -344,12 +419,12 @@ extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int mau
     hipError_t e;
     for (int k = 0; k < 3 && rc == CLAWGPU_OK; k++)
         if ((e = hipMalloc((void**)&c->dq[k], 4 * planes)) != hipSuccess) fail(e);
-    if (rc == CLAWGPU_OK && (e = hipMalloc((void**)&c->daux, 2 * planes)) != hipSuccess) fail(e);
+    if (rc == CLAWGPU_OK && (e = hipMalloc((void**)&c->daux, (size_t)maux * planes)) != hipSuccess) fail(e);
     if (rc == CLAWGPU_OK &&
         (e = hipMalloc((void**)&c->staging, (size_t)(meqn > maux ? meqn : maux) * planes)) != hipSuccess)
         fail(e);
     if (rc == CLAWGPU_OK && (e = hipStreamCreate(&c->stream)) != hipSuccess) fail(e);
-    if (rc == CLAWGPU_OK && (e = hipMemsetAsync(c->daux, 0, 2 * planes, c->stream)) != hipSuccess) fail(e);
+    if (rc == CLAWGPU_OK && (e = hipMemsetAsync(c->daux, 0, (size_t)maux * planes, c->stream)) != hipSuccess) fail(e);
     if (rc == CLAWGPU_OK && (e = hipStreamSynchronize(c->stream)) != hipSuccess) fail(e);
     if (rc != CLAWGPU_OK) {
         for (int k = 0; k < 3; k++) (void)hipFree(c->dq[k]);
@@ -373,6 +448,7 @@ extern "C" int clawgpu3_destroy(clawgpu3_ctx* c)
     (void)hipFree(c->exy);
     if (c->dmat) (void)hipFree(c->dmat);
     if (c->dlut) (void)hipFree(c->dlut);
+    if (c->drec) (void)hipFree(c->drec);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return CLAWGPU_OK;
@@ -457,7 +533,7 @@ extern "C" int clawgpu3_set_aux(clawgpu3_ctx* c, const double* aux)
     HIPCHK(c, hipMemcpyAsync(c->staging, aux, (size_t)n * c->maux * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(aos_to_planes3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->staging, c->daux, n, c->maux, 2, n);
+                       c->staging, c->daux, n, c->maux, c->maux, n);
     HIPCHK(c, hipGetLastError());
     unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->staging);
     HIPCHK(c, hipMemsetAsync(bits, 0, 2 * sizeof(unsigned long long), c->stream));
@@ -503,8 +579,7 @@ extern "C" int clawgpu3_step3(clawgpu3_ctx* c, double dx, double dy, double dz, 
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->cur = res;
-    *cfl = unsplit_cfl(c, dx, dy, dz, dt);
-    return CLAWGPU_OK;
+    return unsplit_cfl(c, method, dx, dy, dz, dt, cfl);
 }
 
 extern "C" int clawgpu3_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz, double dt,
@@ -597,13 +672,16 @@ static int claw3_loop(clawgpu3_ctx* c, const double* geom, double tstart, double
                         hq.data(), c->maux, haux, told, dt / 2.0);
                 if ((rc = push())) return rc;
             }
-            if (!std::isfinite(c->cmax)) {
+            double cf[3];
+            if ((rc = courant3(c, method, dx, dy, dz, dt, cf))) return rc;
+            if (!std::isfinite(c->cmax) || !std::isfinite(cf[0]) || !std::isfinite(cf[1]) ||
+                !std::isfinite(cf[2])) {
                 *info_out = -5;
                 return set_err(CLAWGPU_ERR_NONFINITE, "non-finite sound speed (Courant number)");
             }
-            if (method[2] >= 0) cfl = unsplit_cfl(c, dx, dy, dz, dt);
+            if (method[2] >= 0) cfl = fmax(cf[0], fmax(cf[1], cf[2]));
             else {
-                const double cx = dt / dx * c->cmax, cy = dt / dy * c->cmax, cz = dt / dz * c->cmax;
+                const double cx = cf[0], cy = cf[1], cz = cf[2];
                 cfl = cx > cflv[0] ? cx : cy > cflv[0] ? cy : fmax(cx, fmax(cy, cz));
             }
             t = told + dt;
@@ -746,6 +824,13 @@ extern "C" int clawgpu3_time_steps(clawgpu3_ctx* c, const double* geom, double d
     if (step_kernel_ms) *step_kernel_ms = ksum;
     for (int k = 0; k < 2 * nsteps + 2; k++) (void)hipEventDestroy(ev[k]);
     delete[] ev;
-    if (cfl_out) *cfl_out = unsplit_cfl(c, dx, dy, dz, dt);
+    if (cfl_out) return unsplit_cfl(c, method, dx, dy, dz, dt, cfl_out);
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_set_arith(clawgpu3_ctx* c, int mode)
+{
+    if (!c || (mode != 0 && mode != 1)) return set_err(CLAWGPU_ERR_ARG, "mode must be 0 or 1");
+    c->exact = mode == 0 ? 1 : 0;
     return CLAWGPU_OK;
 }

@@ -663,6 +663,7 @@ struct SweepArgs3 {
     const double* __restrict__ q;
     double* __restrict__ qn;
     const double* __restrict__ aux;
+    const double* __restrict__ capa;  // capacity plane (method(6) > 0), or nullptr
     Layout3 L;
     long afs;
     double d;  // dt / (spacing along the sweep)
@@ -672,15 +673,24 @@ struct SweepArgs3 {
     int nsegs;
 };
 
-// update of one slice cell: (q + qadd) - d*(F(hi) - F(lo)), step3ds.f90
+// update of one slice cell: (q + qadd) - d*(F(hi) - F(lo)), step3ds.f90;
+// with capacity kappa the flux difference term is divided by it
 __device__ inline double ds_update(double q, double qadd, double Fhi, double Flo, double d)
 {
     return q + qadd - d * (Fhi - Flo);
+}
+__device__ inline double ds_update_capa(double q, double qadd, double Fhi, double Flo, double d,
+                                        double kappa)
+{
+    return q + qadd - d * (Fhi - Flo) / kappa;
 }
 
 // x sweep of step3ds (idir = 1): one wave per 60-cell strip of one x-slice
 // (j, k), over all slices including the ghost ones; lanes hold cells and get
 // their neighbours through DPP.  Cells outside the updated range are copied.
+// With capacity, the cell's dt/dx is d / kappa (step3ds.f90: dtdx1d), the
+// interface's correction uses the mean of its two cells' ratios and the flux
+// difference is divided by kappa.
 template <class M>
 __global__ __launch_bounds__(256) void sweep3_x_kernel(SweepArgs3 a)
 {
@@ -700,18 +710,26 @@ __global__ __launch_bounds__(256) void sweep3_x_kernel(SweepArgs3 a)
     const double p = a.q[c], u = a.q[fs + c];
     const Mat mm = {a.aux[c], a.aux[a.afs + c]};
     Iface f = solve<M>(lane_up(p), lane_up(u), mat_up(mm), p, u, mm);
+    const double kap = a.capa ? a.capa[c] : 1.0;
+    const double dc = a.capa ? a.d / kap : a.d;  // dtdx1d of this cell
     Wv cq = {0.0, 0.0};
     if (a.m2 == 2) {
         Wv w1 = limit<M>(f.w1, wv_up(f.w1), wv_dn(f.w1), f.s1, a.lim0);
         Wv w2 = limit<M>(f.w2, wv_up(f.w2), wv_dn(f.w2), f.s2, a.lim1);
-        cq = cqxx(w1, f.s1, w2, f.s2, a.d, a.fwave);
+        const double dave = a.capa ? 0.5 * (lane_up(dc) + dc) : a.d;
+        cq = cqxx(w1, f.s1, w2, f.s2, dave, a.fwave);
     }
     const Wv amhi = wv_dn(f.am), cqhi = wv_dn(cq);
     const bool own = (l >= 2 && l < 62) || (strip == 0 && l < 2);
     if (!own || i > L.mx + L.mbc) return;
     const bool upd = i >= 1 && i <= L.mx && j >= 0 && j <= L.my + 1 && k >= 0 && k <= L.mz + 1;
     const long o = L.off(i, j, k);
-    if (upd) {
+    if (upd && a.capa) {
+        const double qp = (0.0 - dc * f.ap.p) - dc * amhi.p;
+        const double qu = (0.0 - dc * f.ap.n) - dc * amhi.n;
+        a.qn[o] = ds_update_capa(p, qp, cqhi.p, cq.p, a.d, kap);
+        a.qn[fs + o] = ds_update_capa(u, qu, cqhi.n, cq.n, a.d, kap);
+    } else if (upd) {
         const double qp = (0.0 - a.d * f.ap.p) - a.d * amhi.p;
         const double qu = (0.0 - a.d * f.ap.n) - a.d * amhi.n;
         a.qn[o] = ds_update(p, qp, cqhi.p, cq.p, a.d);
@@ -753,36 +771,45 @@ __global__ __launch_bounds__(256) void sweep3_yz_kernel(SweepArgs3 a, int dir)
         s = s < 1 - L.mbc ? 1 - L.mbc : (s > n + L.mbc ? n + L.mbc : s);
         return dir == 1 ? L.off(i, s, o2) : L.off(i, o2, s);
     };
-    auto cell = [&](int s, double& pv, double& nv, Mat& mv) {
+    // cell s: p, normal velocity, material and (with capacity) dt/d / kappa
+    auto cell = [&](int s, double& pv, double& nv, Mat& mv, double& dv) {
         const long c = idx(s);
         pv = a.q[c];
         nv = a.q[comp * fs + c];
         mv = {a.aux[c], a.aux[a.afs + c]};
+        dv = a.capa ? a.d / a.capa[c] : a.d;
     };
-    auto lim_cq = [&](const Iface& f, const Iface& fl, const Iface& fr) -> Wv {
+    // limited cqxx of interface f between cells with ratios dl, dr
+    auto lim_cq = [&](const Iface& f, const Iface& fl, const Iface& fr, double dl, double dr) -> Wv {
         if (a.m2 != 2) return {0.0, 0.0};
         Wv w1 = limit<M>(f.w1, fl.w1, fr.w1, f.s1, a.lim0);
         Wv w2 = limit<M>(f.w2, fl.w2, fr.w2, f.s2, a.lim1);
-        return cqxx(w1, f.s1, w2, f.s2, a.d, a.fwave);
+        return cqxx(w1, f.s1, w2, f.s2, a.capa ? 0.5 * (dl + dr) : a.d, a.fwave);
     };
-    double pm2, nm2, pm1, nm1, p0, n0, p1, n1;
+    double pm2, nm2, pm1, nm1, p0, n0, p1, n1, dm2, dm1, d0, d1;
     Mat mm2, mm1, m0, m1;
-    cell(s0 - 2, pm2, nm2, mm2);
-    cell(s0 - 1, pm1, nm1, mm1);
-    cell(s0, p0, n0, m0);
-    cell(s0 + 1, p1, n1, m1);
+    cell(s0 - 2, pm2, nm2, mm2, dm2);
+    cell(s0 - 1, pm1, nm1, mm1, dm1);
+    cell(s0, p0, n0, m0, d0);
+    cell(s0 + 1, p1, n1, m1, d1);
     Iface fprev = solve<M>(pm2, nm2, mm2, pm1, nm1, mm1);  // interface s0-1
     Iface fcur = solve<M>(pm1, nm1, mm1, p0, n0, m0);      // interface s0
     Iface fnext = solve<M>(p0, n0, m0, p1, n1, m1);        // interface s0+1
-    Wv cqcur = lim_cq(fcur, fprev, fnext);
+    Wv cqcur = lim_cq(fcur, fprev, fnext, dm1, d0);
     for (int s = s0; s <= s1; s++) {
-        double p2, n2v;
+        double p2, n2v, d2;
         Mat m2;
-        cell(s + 2, p2, n2v, m2);
+        cell(s + 2, p2, n2v, m2, d2);
         const Iface fnn = solve<M>(p1, n1, m1, p2, n2v, m2);  // interface s+2
-        const Wv cqnext = lim_cq(fnext, fcur, fnn);
+        const Wv cqnext = lim_cq(fnext, fcur, fnn, d0, d1);
         const long o = idx(s);
-        if (slice_ok && s >= 1 && s <= n) {
+        if (slice_ok && s >= 1 && s <= n && a.capa) {
+            const double kap = a.capa[o];
+            const double qp = (0.0 - d0 * fcur.ap.p) - d0 * fnext.am.p;
+            const double qv = (0.0 - d0 * fcur.ap.n) - d0 * fnext.am.n;
+            a.qn[o] = ds_update_capa(p0, qp, cqnext.p, cqcur.p, a.d, kap);
+            a.qn[comp * fs + o] = ds_update_capa(n0, qv, cqnext.n, cqcur.n, a.d, kap);
+        } else if (slice_ok && s >= 1 && s <= n) {
             const double qp = (0.0 - a.d * fcur.ap.p) - a.d * fnext.am.p;
             const double qv = (0.0 - a.d * fcur.ap.n) - a.d * fnext.am.n;
             a.qn[o] = ds_update(p0, qp, cqnext.p, cqcur.p, a.d);
@@ -797,8 +824,8 @@ __global__ __launch_bounds__(256) void sweep3_yz_kernel(SweepArgs3 a, int dir)
         fcur = fnext;
         fnext = fnn;
         cqcur = cqnext;
-        p0 = p1; n0 = n1; m0 = m1;
-        p1 = p2; n1 = n2v; m1 = m2;
+        p0 = p1; n0 = n1; m0 = m1; d0 = d1;
+        p1 = p2; n1 = n2v; m1 = m2; d1 = d2;
     }
 }
 

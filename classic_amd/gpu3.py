@@ -188,6 +188,12 @@ class Grid3D:
             out.append((tend, self.get_q()))
         return out
 
+    def set_arith(self, mode):
+        """Unsplit step arithmetic: 1 fast (default), 0 exact (the reference's
+        rounding and order, bit for bit; steps with a capacity function
+        always run exact)."""
+        check(self.lib.clawgpu3_set_arith(self.h, int(mode)))
+
     def set_segment(self, cells):
         check(self.lib.clawgpu3_set_segment(self.h, cells))
 

@@ -334,15 +334,21 @@ int clawgpu3_bc3(clawgpu3_ctx* ctx, const double* geom, double t, const int* mth
  * replaces it (ghost cells of the result are unspecified; the reference
  * leaves partial updates there); *cfl is the Courant number.  method(2) in
  * {1, 2}, method(3) in {0, 10, 11, 20, 21, 22} (src/3d/chkmth.f:3-72),
- * method(6) = 0, mthlim(1:2) in 0..5.  Summation order differs from the
- * reference's slice order: results agree to round-off (tested to 1e-12). */
+ * mthlim(1:2) in 0..5.  method(6) > 0 names the aux component holding the
+ * capacity function (step3.f90:119-202, 337-424; dtdx1d = dtdx / kappa).
+ * Fast arithmetic (the default, method(6) = 0): summation order differs
+ * from the reference's slice order, results agree to round-off (tested to
+ * 1e-12).  With a capacity function, or in exact arithmetic
+ * (clawgpu3_set_arith), the step follows the reference's rounding and the
+ * order of its passes over blocks of slices: bit for bit. */
 int clawgpu3_step3(clawgpu3_ctx* ctx, double dx, double dy, double dz, double dt,
                    const int* method, const int* mthlim, int use_fwave, double* cfl);
 
 /* One Godunov-split step, the contract of dimsp3 (src/3d/dimsp3.f:5-85,
  * step3ds src/3d/step3ds.f90:3-408) with method(3) = -1, including its early
- * return after a sweep whose Courant number exceeds cflv(1).  Bit for bit
- * with the reference, ghost cells included. */
+ * return after a sweep whose Courant number exceeds cflv(1); method(6) > 0:
+ * capacity function (step3ds.f90:125-173, 217-228).  Bit for bit with the
+ * reference, ghost cells included. */
 int clawgpu3_dimsp3(clawgpu3_ctx* ctx, double dx, double dy, double dz, double dt,
                     const int* method, const int* mthlim, const double* cflv, int use_fwave,
                     double* cfl);
@@ -383,6 +389,13 @@ int clawgpu3_claw3_hooks(clawgpu3_ctx* ctx, const double* geom, double tstart, d
 
 /* Cells per segment of the y/z split sweeps (tuning; default 64). */
 int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
+
+/* Arithmetic of the unsplit 3D step, as clawgpu_set_arith: mode 1 (the
+ * default) fast (fused kernels, FMA contraction, within 1e-12 of the
+ * reference), mode 0 exact (one kernel per slice position and direction
+ * plus one per cell: the reference's rounding and order, bit for bit with
+ * step3; slower).  Steps with a capacity function always run exact. */
+int clawgpu3_set_arith(clawgpu3_ctx* ctx, int mode);
 
 /* Tile shape of the unsplit kernel (tuning): 0 = 60x4x4, 1 = 60x4x2,
  * 2 = 60x2x2, 3 = 60x8x2, 4 = 60x2x4 cells per wavefront;

@@ -1,9 +1,11 @@
 """3D path on the GPU against the oracle (oracle/claw3d_oracle.c): bc3 and
 the split step (dimsp3) bit for bit, the unsplit step (step3, every
-method(3) family) to 1e-12 relative per field (the tile kernel sums each
-cell's increments in its own order), the claw3 loop (dt, Courant numbers,
-step counts exactly; q to 1e-12), and the heterogeneous-acoustics example's
-regression frame sums."""
+method(3) family) to 1e-12 relative per field in fast arithmetic (the tile
+and z-marching kernels sum each cell's increments in their own order) and
+bit for bit in exact arithmetic and with a capacity function (the records
+path), the claw3 loop (dt, Courant numbers, step counts exactly; q to
+1e-12, or bit for bit with a capacity function), and the
+heterogeneous-acoustics example's regression frame sums."""
 import os
 
 import numpy as np
@@ -255,7 +257,7 @@ def test_unsupported_options_fail_loudly():
     from classic_amd._lib import ClawGpuError
     with Grid3D(8, 8, 8) as g:
         with pytest.raises(ClawGpuError):
-            g.step3(0.01, method=(1, 2, 22, 0, 0, 1, 2))   # capacity function
+            g.step3(0.01, method=(1, 2, 22, 0, 0, 3, 2))   # capacity index beyond maux
         with pytest.raises(ClawGpuError):
             g.step3(0.01, method=(1, 2, 12, 0, 0, 0, 2))   # not a chkmth method
         with pytest.raises(ClawGpuError):
@@ -403,3 +405,125 @@ def test_config3_full_size_zmarch_vs_tile_kernel_100_steps():
     print(f"z-marching vs tile kernel 512^3 x 100: max rel err {err:.2e}, cell rel err "
           f"{cerr:.2e}, dt rel err {dterr:.2e}", flush=True)
     assert err <= TOL and dterr <= TOL
+
+
+# ------------------------------------------- exact mode and capacity function
+
+def _gold(name):
+    return np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_exact_and_capacity_steps_equal_the_reference_build(case):
+    """The golden single steps written by the reference Fortran
+    (tests/golden/make_golden3.py: every method(3) family, f-waves, the
+    split step, and the capacity function in aux(3) for the unsplit step
+    (case 6) and the split step (case 9)): the exact unsplit step (the
+    records path) and the split step equal the reference's q bit for bit,
+    ghost cells excluded for the unsplit step (the reference leaves partial
+    updates there), and the Courant number exactly."""
+    d = _gold(f"step3_case{case}")
+    method = tuple(int(x) for x in d["method"])
+    lim = tuple(int(x) for x in d["mthlim"])
+    fw = bool(int(d["use_fwave"]))
+    maux = d["aux"].shape[-1]
+    p = O3.Problem3(6, 5, 4, maux=maux, mthlim=lim, method=method, use_fwave=fw)
+    dt = 0.02
+    with _grid(p, d["aux"], d["q_in"]) as g:
+        g.set_arith(0)
+        if method[2] >= 0:
+            cfl = g.step3(dt, method=method, mthlim=lim, use_fwave=fw)
+        else:
+            cfl = g.dimsp3(dt, method=method, mthlim=lim, use_fwave=fw)
+        out = g.get_q()
+    print(f"case {case}: method {method[:3]} capa {method[5]} cfl {cfl!r}", flush=True)
+    assert cfl == float(d["cfl"])
+    if method[2] >= 0:
+        np.testing.assert_array_equal(out[2:-2, 2:-2, 2:-2], d["q_out"][2:-2, 2:-2, 2:-2])
+    else:
+        np.testing.assert_array_equal(out, d["q_out"])
+
+
+def _capa_state(p, rng):
+    q, aux2 = random_layered(p, rng)
+    aux = np.empty(p.shape(3))
+    aux[..., :2] = aux2
+    aux[..., 2] = rng.uniform(0.5, 1.5, aux.shape[:3])
+    return q, aux
+
+
+@pytest.mark.parametrize("m3,lim,fw", [(22, (3, 3), False), (22, (4, 2), False), (21, (1, 4), False),
+                                       (20, (4, 4), False), (11, (0, 0), False), (10, (0, 0), False),
+                                       (0, (3, 3), False), (22, (3, 3), True)])
+def test_step3_capacity_bitwise_vs_oracle(m3, lim, fw):
+    """Unsplit step with the capacity function in aux(3) on a grid of
+    ragged sides (70 x 9 x 6; blocks of slices with odd counts) against the
+    oracle (pinned to the reference build by the golden vectors): q and the
+    Courant number bit for bit, every method(3) family."""
+    rng = np.random.default_rng(80 + m3)
+    order = 1 if lim == (0, 0) else 2
+    p = O3.Problem3(MX, MY, MZ, maux=3, mthlim=lim, method=(1, order, m3, 0, 0, 3, 0),
+                    use_fwave=fw)
+    q, aux = _capa_state(p, rng)
+    dt = 0.25 * p.geom.dx / 2.0
+    ref, cfl_ref = O3.step3(p, q, aux, dt)
+    with _grid(p, aux, q) as g:
+        cfl = g.step3(dt, method=p.method, mthlim=lim, use_fwave=fw)
+        out = g.get_q()
+    assert cfl == cfl_ref
+    np.testing.assert_array_equal(out[2:-2, 2:-2, 2:-2], ref[2:-2, 2:-2, 2:-2])
+
+
+@pytest.mark.parametrize("order,lim", [(2, (3, 3)), (2, (4, 1)), (1, (0, 0))])
+def test_dimsp3_capacity_bitwise(order, lim):
+    """Split step with the capacity function: bit for bit with the oracle,
+    ghost cells included, and the Courant number."""
+    rng = np.random.default_rng(90 + order)
+    p = O3.Problem3(MX, MY, MZ, maux=3, mthlim=lim, method=(1, order, -1, 0, 0, 3, 0))
+    q, aux = _capa_state(p, rng)
+    dt = 0.3 * p.geom.dx / 2.0
+    ref, cfl_ref = O3.dimsp3(p, q, aux, dt)
+    with _grid(p, aux, q) as g:
+        cfl = g.dimsp3(dt, method=p.method, mthlim=lim)
+        out = g.get_q()
+    assert cfl == cfl_ref
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_exact_mode_equals_oracle_without_capacity():
+    """Without a capacity function the exact mode (set_arith(0)) is the same
+    records path: bit for bit with the oracle on a layered medium, where the
+    fast kernels agree to 1e-12 only."""
+    rng = np.random.default_rng(99)
+    p = O3.Problem3(MX, MY + 4, MZ + 3, mthlim=(2, 2), method=(1, 2, 22, 0, 0, 0, 0))
+    q, aux = random_layered(p, rng)
+    dt = 0.3 * p.geom.dx / 2.0
+    ref, cfl_ref = O3.step3(p, q, aux, dt)
+    with _grid(p, aux, q) as g:
+        g.set_arith(0)
+        cfl = g.step3(dt, method=p.method, mthlim=(2, 2))
+        out = g.get_q()
+    assert cfl == cfl_ref
+    np.testing.assert_array_equal(out[2:-2, 2:-2, 2:-2], ref[2:-2, 2:-2, 2:-2])
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_claw3_with_capacity_matches_oracle(split):
+    """The claw3 loop with a capacity function (variable time steps from its
+    Courant numbers, retakes on a too-large first dt) against the oracle's
+    claw3: dt, Courant numbers and step counts exactly, q bit for bit."""
+    rng = np.random.default_rng(100 + int(split))
+    m3 = -1 if split else 22
+    p = O3.Problem3(24, 14, 10, maux=3, mthlim=(3, 4), method=(1, 2, m3, 0, 0, 3, 0),
+                    mthbc=(1, 3, 2, 2, 3, 1))
+    q, aux = _capa_state(p, rng)
+    dtv, cflv = (0.5, 1.0), (1.0, 0.9)
+    r = O3.claw3(p, q, aux, 0.0, 0.3, dtv, cflv, 40)
+    with _grid(p, aux, q) as g:
+        out = g.claw3(0.0, 0.3, dtv, cflv, 40, method=p.method, mthlim=(3, 4), mthbc=p.mthbc)
+        qg = g.get_q()
+    print(f"split {split}: steps {out.nv[1]} info {out.info} dtv {out.dtv}", flush=True)
+    assert out.info == r.info and out.nv[1] == r.nv[1] and out.nv[1] > 3
+    np.testing.assert_array_equal(out.dtv, r.dtv)
+    np.testing.assert_array_equal(out.cflv, r.cflv)
+    np.testing.assert_array_equal(qg[2:-2, 2:-2, 2:-2], r.q[2:-2, 2:-2, 2:-2])
