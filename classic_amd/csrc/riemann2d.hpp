@@ -61,6 +61,18 @@ struct FastMath {
         r = fma(r, fma(-b, r, 1.0), r);
         return a * r;
     }
+    // 1/sqrt(a) for a > 0: the hardware estimate refined by two Newton steps
+    // y <- y + y (1 - a y^2) / 2 (about 1 ulp); sqrt(a) = a y and 1/a = y^2
+    // then cost one multiplication each instead of a square root and a
+    // reciprocal
+    __device__ static inline double rsqrt(double a)
+    {
+        double y = __builtin_amdgcn_rsq(a);
+        double e = fma(-a * y, y, 1.0);
+        y = fma(0.5 * y, e, y);
+        e = fma(-a * y, y, 1.0);
+        return fma(0.5 * y, e, y);
+    }
     // hardware reciprocal square root, one coupled Newton step for sqrt and
     // 1/(2 sqrt), then a residual correction (a > 0; about 1 ulp)
     __device__ static inline double sqrt_(double a)
@@ -196,11 +208,12 @@ struct Euler2 {
 #pragma unroll
         for (int m = 0; m < M; m++) c.q[m] = q[m];
         const double g1 = P.gamma1;
-        c.sr = Mth::sqrt_(q[0]);
         if constexpr (Mth::fast) {
-            double ri = Mth::rcp(q[0]);
+            // one reciprocal square root gives sqrt(rho), 1/rho and 1/sqrt(rho)
+            const double isr = Mth::rsqrt(q[0]);
+            c.sr = q[0] * isr;
+            const double ri = isr * isr;
             double p = g1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) * ri);
-            double isr = c.sr * ri;
             c.rw[0] = q[1] * isr;
             c.rw[1] = q[2] * isr;
             c.hw = (q[3] + p) * isr;
@@ -208,6 +221,7 @@ struct Euler2 {
             c.vel[0] = q[1] * ri;
             c.vel[1] = q[2] * ri;
         } else {
+            c.sr = Mth::sqrt_(q[0]);
             double p = g1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
             c.rw[0] = q[1] / c.sr;
             c.rw[1] = q[2] / c.sr;
@@ -241,12 +255,14 @@ struct Euler2 {
         }
         r.u2v2 = r.u * r.u + r.v * r.v;
         double a2 = g1 * (r.enth - 0.5 * r.u2v2);
-        r.a = Mth::sqrt_(a2);
         if constexpr (Mth::fast) {
-            double ia2 = Mth::rcp(a2);
-            r.g1a2 = g1 * ia2;
-            r.h2a = 0.5 * r.a * ia2;
+            // a = a2 / sqrt(a2), 1/a2 = (1/sqrt(a2))^2, 1/(2a) = 0.5/sqrt(a2)
+            const double ia = Mth::rsqrt(a2);
+            r.a = a2 * ia;
+            r.g1a2 = g1 * (ia * ia);
+            r.h2a = 0.5 * ia;
         } else {
+            r.a = Mth::sqrt_(a2);
             r.g1a2 = g1 / a2;
             r.h2a = 0.0;
         }
