@@ -325,7 +325,7 @@ __device__ __forceinline__ void step2_wave_fast(const StepArgs& a, const int str
     const double dtdx = a.dtdx, dtdy = a.dtdy;
     const double kt = -0.5 * dtdx * dtdy;  // transverse terms of either sweep
     const ColumnReader<BC> rd(a, i);
-    double* __restrict__ qn = a.qnew + L.off(i, 0);
+    double* __restrict__ qn_base = a.qnew + L.off(i, 0);
     const long pitch = L.pitch, fs = L.fs;
     const int rowmax = my + mbc;
     const SolverParams P = a.P;
@@ -349,8 +349,8 @@ __device__ __forceinline__ void step2_wave_fast(const StepArgs& a, const int str
 #pragma unroll
         for (int m = 0; m < M; m++) ysA.am[m] = ysA.ap[m] = 0.0;
     }
-    double qnx[M];
-    rd.row(r0 - 1, qnx);
+    double qX[M], qY[M];  // q of the current row and the next (loaded ahead)
+    rd.row(r0 - 1, qX);
     double vA[M], vB[M], b2prev[M];
 #pragma unroll
     for (int m = 0; m < M; m++) vA[m] = vB[m] = b2prev[m] = 0.0;
@@ -373,14 +373,92 @@ __device__ __forceinline__ void step2_wave_fast(const StepArgs& a, const int str
         cfl_sum = cfl_sum + t;
     };
 
-    auto row = [&](YS& yp, YS& yc, const int J) {
-        double qc[M];
-#pragma unroll
-        for (int m = 0; m < M; m++) qc[m] = qnx[m];
-        rd.row(min(J + 1, rowmax), qnx);
+    // One row of the sweep.  yp holds interface J-1 (and cell J-1), yc
+    // receives interface J (and cell J); qc holds q of row J and qn receives
+    // row J+1; va holds cell J-2, which the y-interface J-1 completes, and
+    // then receives the new cell J of the x-row; vb holds cell J-1.  The
+    // caller alternates the two sets of each pair, so nothing is copied
+    // between iterations.
+    auto row = [&](YS& yp, YS& yc, const int J, const double (&qc)[M], double (&qn)[M],
+                   double (&va)[M], double (&vb)[M]) {
+        rd.row(min(J + 1, rowmax), qn);
         yc.cell = S::cell(P, qc);
         const Cell& cc = yc.cell;
-        double vC[M];
+
+        double vout[M];  // cell (i, J-2) when it is complete and this lane owns it
+        bool store = false;
+#pragma unroll
+        for (int m = 0; m < M; m++) vout[m] = 0.0;
+
+        // ------------------------------ y-interface J (flux2, ixy=2) on column i
+        S::template rpn<2, 1>(P, yp.cell, cc, yc.w, yc.s, yc.am, yc.ap, yc.cache);
+        if (J >= 1 && J <= my + 1) courant(yc.s, dtdy, ycfl_lane);
+#pragma unroll
+        for (int mw = 0; mw < MW; mw++) {
+            double dr = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; m++) dr = dr + yp.w[mw][m] * yc.w[mw][m];
+            yc.dotl[mw] = dr;  // dotr of interface J-1 = dotl of interface J
+        }
+        if (J - 1 >= r0) {
+            // interface J-1 (held in yp), limited with its neighbours J-2 and J
+            double cq[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) cq[m] = 0.0;
+#pragma unroll
+            for (int mw = 0; mw < MW; mw++) {
+                double nn = 0.0;
+#pragma unroll
+                for (int m = 0; m < M; m++) nn = nn + yp.w[mw][m] * yp.w[mw][m];
+                const double c =
+                    coef(yp.s[mw], yp.s[mw] > 0.0 ? yp.dotl[mw] : yc.dotl[mw], nn, dtdy);
+#pragma unroll
+                for (int m = 0; m < M; m++) cq[m] = cq[m] + c * yp.w[mw][m];
+            }
+            double amt[M], apt[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                amt[m] = yp.am[m] + cq[m];
+                apt[m] = yp.ap[m] - cq[m];
+            }
+            double bma[M], bpa[M], bmp[M], bpp[M];
+            S::template rpt<2, 1>(P, yp.cache, amt, bma, bpa);
+            S::template rpt<2, 1>(P, yp.cache, apt, bmp, bpp);
+            // B^- parts of the columns to the right, B^+ parts of those to the
+            // left, two components at a time
+            if (J - 2 >= r0) store = out_lane && J - 2 < r1;  // cell (i, J-2) completes
+#pragma unroll
+            for (int m = 0; m < M; m += 2) {
+                double bmar[2], bmpr[2], bpal[2], bppl[2];
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    bmar[e] = lane_dn(bma[m + e]);
+                    bmpr[e] = lane_dn(bmp[m + e]);
+                    bpal[e] = lane_up(bpa[m + e]);
+                    bppl[e] = lane_up(bpp[m + e]);
+                }
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int c = m + e;
+                    if (J - 2 >= r0) {
+                        // cell (i, J-2): its upper face
+                        const double t = (bpal[e] - bmar[e]) - (bpa[c] - bma[c]);
+                        const double v2 = va[c] - dtdy * (yp.am[c] + 0.5 * cq[c]);
+                        vout[c] = v2 + kt * t;
+                    }
+                    // cell (i, J-1): its lower face
+                    const double t = (bppl[e] - bmpr[e]) - (bpp[c] - bmp[c]);
+                    const double v2 = vb[c] - dtdy * (yp.ap[c] - 0.5 * cq[c]);
+                    vb[c] = v2 + kt * t;
+                }
+            }
+        }
+        {
+            double* dst = store ? qn_base + (long)(J - 2) * pitch : g_store_sink + lane;
+            const long stride = store ? fs : 64;
+#pragma unroll
+            for (int m = 0; m < M; m++) dst[m * stride] = vout[m];
+        }
 
         // ------------------------------------------------ x-row J (flux2, ixy=1)
         if (J <= r1) {
@@ -453,94 +531,14 @@ __device__ __forceinline__ void step2_wave_fast(const StepArgs& a, const int str
                     double v2 = qc[c] + kt * b2prev[c];
                     v2 = v2 - dtdx * apx[c];
                     v2 = v2 - dtdx * lfr[e];
-                    vC[c] = v2 - kt * (b2 - b1);
-                    vB[c] = vB[c] - kt * b1;
+                    va[c] = v2 - kt * (b2 - b1);
+                    vb[c] = vb[c] - kt * b1;
                     b2prev[c] = b2;
                 }
             }
         } else {
 #pragma unroll
-            for (int m = 0; m < M; m++) vC[m] = 0.0;
-        }
-
-        double vout[M];  // cell (i, J-2) when it is complete and this lane owns it
-        bool store = false;
-#pragma unroll
-        for (int m = 0; m < M; m++) vout[m] = 0.0;
-
-        // ------------------------------ y-interface J (flux2, ixy=2) on column i
-        S::template rpn<2, 1>(P, yp.cell, cc, yc.w, yc.s, yc.am, yc.ap, yc.cache);
-        if (J >= 1 && J <= my + 1) courant(yc.s, dtdy, ycfl_lane);
-#pragma unroll
-        for (int mw = 0; mw < MW; mw++) {
-            double dr = 0.0;
-#pragma unroll
-            for (int m = 0; m < M; m++) dr = dr + yp.w[mw][m] * yc.w[mw][m];
-            yc.dotl[mw] = dr;  // dotr of interface J-1 = dotl of interface J
-        }
-        if (J - 1 >= r0) {
-            // interface J-1 (held in yp), limited with its neighbours J-2 and J
-            double cq[M];
-#pragma unroll
-            for (int m = 0; m < M; m++) cq[m] = 0.0;
-#pragma unroll
-            for (int mw = 0; mw < MW; mw++) {
-                double nn = 0.0;
-#pragma unroll
-                for (int m = 0; m < M; m++) nn = nn + yp.w[mw][m] * yp.w[mw][m];
-                const double c =
-                    coef(yp.s[mw], yp.s[mw] > 0.0 ? yp.dotl[mw] : yc.dotl[mw], nn, dtdy);
-#pragma unroll
-                for (int m = 0; m < M; m++) cq[m] = cq[m] + c * yp.w[mw][m];
-            }
-            double amt[M], apt[M];
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                amt[m] = yp.am[m] + cq[m];
-                apt[m] = yp.ap[m] - cq[m];
-            }
-            double bma[M], bpa[M], bmp[M], bpp[M];
-            S::template rpt<2, 1>(P, yp.cache, amt, bma, bpa);
-            S::template rpt<2, 1>(P, yp.cache, apt, bmp, bpp);
-            // B^- parts of the columns to the right, B^+ parts of those to the
-            // left, two components at a time
-            if (J - 2 >= r0) store = out_lane && J - 2 < r1;  // cell (i, J-2) completes
-#pragma unroll
-            for (int m = 0; m < M; m += 2) {
-                double bmar[2], bmpr[2], bpal[2], bppl[2];
-#pragma unroll
-                for (int e = 0; e < 2; e++) {
-                    bmar[e] = lane_dn(bma[m + e]);
-                    bmpr[e] = lane_dn(bmp[m + e]);
-                    bpal[e] = lane_up(bpa[m + e]);
-                    bppl[e] = lane_up(bpp[m + e]);
-                }
-#pragma unroll
-                for (int e = 0; e < 2; e++) {
-                    const int c = m + e;
-                    if (J - 2 >= r0) {
-                        // cell (i, J-2): its upper face
-                        const double t = (bpal[e] - bmar[e]) - (bpa[c] - bma[c]);
-                        const double v2 = vA[c] - dtdy * (yp.am[c] + 0.5 * cq[c]);
-                        vout[c] = v2 + kt * t;
-                    }
-                    // cell (i, J-1): its lower face
-                    const double t = (bppl[e] - bmpr[e]) - (bpp[c] - bmp[c]);
-                    const double v2 = vB[c] - dtdy * (yp.ap[c] - 0.5 * cq[c]);
-                    vB[c] = v2 + kt * t;
-                }
-            }
-        }
-        {
-            double* dst = store ? qn + (long)(J - 2) * pitch : g_store_sink + lane;
-            const long stride = store ? fs : 64;
-#pragma unroll
-            for (int m = 0; m < M; m++) dst[m * stride] = vout[m];
-        }
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-            vA[m] = vB[m];
-            vB[m] = vC[m];
+            for (int m = 0; m < M; m++) va[m] = 0.0;
         }
     };
 
@@ -549,8 +547,8 @@ __device__ __forceinline__ void step2_wave_fast(const StepArgs& a, const int str
 #pragma unroll
     for (int m = 0; m < M; m++) g_store_sink[m * 64 + lane] = 0.0;
     for (int J = r0 - 1; J < jend; J += 2) {
-        row(ysA, ysB, J);
-        row(ysB, ysA, J + 1);
+        row(ysA, ysB, J, qX, qY, vA, vB);
+        row(ysB, ysA, J + 1, qY, qX, vB, vA);
     }
     publish_cfl(cfl_sum == cfl_sum ? cfl : __builtin_huge_val(), a.cfl_bits);
 }
