@@ -51,6 +51,12 @@ _SIGS = {
     "clawgpu_dimsp2": ([_V, _d, _d, _d, _I, _I, _D, _i, _D], _i),
     "clawgpu_claw2": ([_V, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
     "clawgpu_claw2_hooks": ([_V, _d, _d, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
+    "clawgpu_claw2_dev_hooks": ([_V, _d, _d, _d, _d, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I],
+                                _i),
+    "clawgpu_dev_hooks_load": ([C.POINTER(_V), C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                _i], _i),
+    "clawgpu_dev_hooks_destroy": ([_V], _i),
+    "clawgpu_dev_hooks_user": ([_V, _D, _D, _i], _i),
     "clawgpu_set_segment_rows": ([_V, _i], _i),
     "clawgpu_get_segment_rows": ([_V, _I], _i),
     "clawgpu_set_arith": ([_V, _i], _i),

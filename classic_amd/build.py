@@ -58,6 +58,17 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     return LIB
 
 
+def build_code_object(src, out, verbose=False):
+    """Compile a HIP source of device-side user routines (include/clawgpu.h,
+    clawgpu_dev_grid) into a gfx950 code object for clawgpu_dev_hooks_load,
+    with the library's arithmetic flags (no contraction)."""
+    cmd = [HIPCC, "--genco", *[f for f in FLAGS if f not in ("-fPIC",)], "-o", out, src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
 def _wait(proc, others):
     if proc.wait() != 0:
         for p in others:

@@ -59,6 +59,7 @@ int launch_loop2_finish(const LoopArgs& l, hipStream_t s)
 using namespace clawgpu;
 
 #include "ctx.hpp"
+#include "devhooks.hpp"
 #include "step2g.hpp"
 
 thread_local char g_errbuf[256];
@@ -200,6 +201,7 @@ extern "C" int clawgpu_destroy(clawgpu_ctx* c)
     if (c->grec) (void)hipFree(c->grec);
     if (c->dloop) (void)hipFree(c->dloop);
     if (c->dcflb) (void)hipFree(c->dcflb);
+    if (c->dsave) (void)hipFree(c->dsave);
     if (c->hloop) (void)hipHostFree(c->hloop);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -647,18 +649,26 @@ static int claw2_device(clawgpu_ctx* c, double dx, double dy, double tstart, dou
 // host -> accept (swap buffers) or reject (keep the old buffer, which is the
 // reference's copyq2 restore, and retake with a smaller dt).
 // The claw2 loop (claw2.f:596-784).  With hooks, the user's bc2, b4step2
-// and src2 are called on a host copy of q at the reference's points:
-// bc2 and b4step2 before every attempt (retakes included), src2 over dt/2
-// before the step and dt/2 after it (method(5) = 2, the pre-source state
-// kept for a retake) or over dt after it (method(5) = 1).
+// and src2 are called at the reference's points: bc2 and b4step2 before
+// every attempt (retakes included), src2 over dt/2 before the step and dt/2
+// after it (method(5) = 2, the pre-source state kept for a retake) or over
+// dt after it (method(5) = 1).  Host routines (h) get a host copy of q;
+// device routines (dh) run on the device q.
 static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, double ylower,
                       double tstart, double* tend, double* dtv, double* cflv, int* nv,
                       const int* method, const int* mthlim, const int* mthbc, int use_fwave,
-                      const clawgpu_hooks* h, int* info_out)
+                      const clawgpu_hooks* h, const clawgpu_dev_hooks* dh, int* info_out)
 {
     if (!c || !tend || !dtv || !cflv || !nv || !method || !mthlim || !mthbc || !info_out)
         return set_err(CLAWGPU_ERR_ARG, "null argument");
-    const bool user_bc = h && h->bc2;
+    const bool dev_bc = dh && dh->bc2;
+    const bool user_bc = (h && h->bc2) || dev_bc;
+    const bool hooks = h || dh;
+    const bool src = (h && h->src2) || (dh && dh->src2);
+    auto dev_call = [&](hipFunction_t f, double tt, double dtt) -> int {
+        return launch_dev_hook(c, f, dh, c->dq[c->cur], xlower, ylower, dx, dy, tt, dtt, mthbc);
+    };
+    const size_t qbytes = (size_t)c->L.fs * c->meqn * sizeof(double);
     const bool split = method[2] < 0;
     int rc = split ? CLAWGPU_OK : check_method(c, method, mthlim);
     if (rc) return rc;
@@ -687,7 +697,7 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
     // right behind the Courant-number read-back (into the buffer that becomes
     // q if the step is accepted), so it runs while the host decides; a
     // rejected step refills the unchanged q as usual.
-    const bool overlap_bc = !h && !split;
+    const bool overlap_bc = !hooks && !split;
     bool bc_ready = false;
     struct Ev {
         hipEvent_t e = nullptr;
@@ -714,7 +724,7 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
     if (method[5] > method[6]) { info = 5; goto done; }
     if (method[1] < method[2]) { info = 6; goto done; }
     if (maxn == 0) goto done;
-    if (!h && !split && !uses_table(c, mthlim) && !general_path(c)) {
+    if (!hooks && !split && !uses_table(c, mthlim) && !general_path(c)) {
         LoopState f;
         rc = claw2_device(c, dx, dy, tstart, *tend, dtv, cflv, maxn, method, mthlim, mthbc,
                           use_fwave, &f);
@@ -739,6 +749,9 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
         for (;;) {
             if (bc_ready) {
                 bc_ready = false;  // filled while the host waited for the last Courant number
+            } else if (dev_bc) {
+                if ((rc = dev_call(dh->bc2, told, dt))) return rc;
+                HIPCHK(c, hipMemsetAsync(c->dcfl, 0, 4 * sizeof(unsigned long long), c->stream));
             } else if (user_bc) {
                 if ((rc = pull())) return rc;
                 h->bc2(h->user, c->meqn, c->mbc, c->mx, c->my, xlower, ylower, dx, dy, hq.data(),
@@ -749,15 +762,22 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
                 rc = enqueue_bc(c, c->dq[c->cur], mthbc, true);
                 if (rc) return rc;
             }
-            if (h && h->b4step2) {
+            if (dh && dh->b4step2) {
+                if ((rc = dev_call(dh->b4step2, told, dt))) return rc;
+            } else if (h && h->b4step2) {
                 if ((rc = pull())) return rc;
                 h->b4step2(h->user, c->mbc, c->mx, c->my, c->meqn, hq.data(), xlower, ylower, dx, dy,
                            told, dt, c->maux, haux);
                 if ((rc = push())) return rc;
                 if (haux && (rc = clawgpu_set_aux(c, c->haux.data()))) return rc;
             }
-            const bool strang_src = method[4] == 2 && h && h->src2;
-            if (strang_src) {
+            const bool strang_src = method[4] == 2 && src;
+            if (strang_src && dh) {
+                if (!c->dsave) HIPCHK(c, hipMalloc((void**)&c->dsave, qbytes));
+                HIPCHK(c, hipMemcpyAsync(c->dsave, c->dq[c->cur], qbytes, hipMemcpyDeviceToDevice,
+                                         c->stream));
+                if ((rc = dev_call(dh->src2, told, dt / 2.0))) return rc;
+            } else if (strang_src) {
                 if ((rc = pull())) return rc;
                 saved = hq;
                 h->src2(h->user, c->meqn, c->mbc, c->mx, c->my, xlower, ylower, dx, dy, hq.data(),
@@ -806,7 +826,10 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
             }
             t = told;  // reject: dq[cur] still holds the step's initial data
             bc_ready = false;
-            if (strang_src) {  // back to the state before the source half step
+            if (strang_src && dh) {  // back to the state before the source half step
+                HIPCHK(c, hipMemcpyAsync(c->dq[c->cur], c->dsave, qbytes, hipMemcpyDeviceToDevice,
+                                         c->stream));
+            } else if (strang_src) {
                 hq = saved;
                 if ((rc = push())) return rc;
             }
@@ -817,7 +840,9 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
             cflmax = fmax(cfl, cflmax);
             goto done;
         }
-        if (h && h->src2 && (method[4] == 1 || method[4] == 2)) {
+        if (dh && dh->src2 && (method[4] == 1 || method[4] == 2)) {
+            if ((rc = dev_call(dh->src2, t, method[4] == 2 ? dt / 2.0 : dt))) return rc;
+        } else if (h && h->src2 && (method[4] == 1 || method[4] == 2)) {
             if ((rc = pull())) return rc;
             h->src2(h->user, c->meqn, c->mbc, c->mx, c->my, xlower, ylower, dx, dy, hq.data(),
                     c->maux, haux, t, method[4] == 2 ? dt / 2.0 : dt);
@@ -850,7 +875,7 @@ extern "C" int clawgpu_claw2(clawgpu_ctx* c, double dx, double dy, double tstart
                              const int* mthlim, const int* mthbc, int use_fwave, int* info_out)
 {
     return claw2_loop(c, dx, dy, 0.0, 0.0, tstart, tend, dtv, cflv, nv, method, mthlim, mthbc,
-                      use_fwave, nullptr, info_out);
+                      use_fwave, nullptr, nullptr, info_out);
 }
 
 extern "C" int clawgpu_claw2_hooks(clawgpu_ctx* c, double dx, double dy, double xlower,
@@ -863,7 +888,18 @@ extern "C" int clawgpu_claw2_hooks(clawgpu_ctx* c, double dx, double dy, double 
     if (c && c->maux > 0 && c->haux.empty() && (hooks->b4step2 || hooks->src2 || hooks->bc2))
         return set_err(CLAWGPU_ERR_ARG, "aux has not been set (clawgpu_set_aux)");
     return claw2_loop(c, dx, dy, xlower, ylower, tstart, tend, dtv, cflv, nv, method, mthlim,
-                      mthbc, use_fwave, hooks, info_out);
+                      mthbc, use_fwave, hooks, nullptr, info_out);
+}
+
+extern "C" int clawgpu_claw2_dev_hooks(clawgpu_ctx* c, double dx, double dy, double xlower,
+                                       double ylower, double tstart, double* tend, double* dtv,
+                                       double* cflv, int* nv, const int* method,
+                                       const int* mthlim, const int* mthbc, int use_fwave,
+                                       const clawgpu_dev_hooks* hooks, int* info_out)
+{
+    if (!hooks) return set_err(CLAWGPU_ERR_ARG, "null hooks (use clawgpu_claw2)");
+    return claw2_loop(c, dx, dy, xlower, ylower, tstart, tend, dtv, cflv, nv, method, mthlim,
+                      mthbc, use_fwave, nullptr, hooks, info_out);
 }
 
 namespace {

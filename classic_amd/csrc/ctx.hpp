@@ -39,6 +39,7 @@ struct clawgpu_ctx {
     LoopState* dloop = nullptr;
     unsigned long long* dcflb = nullptr;
     LoopState* hloop = nullptr;  // 3 entries
+    double* dsave = nullptr;     // q before a Strang source half step (device user routines)
     int last_hip_error;
 };
 

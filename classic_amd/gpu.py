@@ -109,6 +109,41 @@ def make_hooks(bc2=None, b4step2=None, src2=None):
     return h, keep, errors
 
 
+class DeviceHooks:
+    """Device-side user routines: the kernels named bc2 / b4step2 / src2 in
+    the gfx950 code object at path (see include/clawgpu.h,
+    clawgpu_dev_grid), with nuser doubles of device memory passed to them."""
+
+    def __init__(self, path, bc2=None, b4step2=None, src2=None, nuser=0):
+        self.lib = _lib.load()
+        self.nuser = nuser
+        h = C.c_void_p()
+        enc = (lambda x: None if x is None else x.encode())
+        check(self.lib.clawgpu_dev_hooks_load(C.byref(h), str(path).encode(), enc(bc2),
+                                              enc(b4step2), enc(src2), int(nuser)))
+        self.h = h
+
+    def write_user(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(self.lib.clawgpu_dev_hooks_user(self.h, _dp(v), None, v.size))
+
+    def read_user(self, n=None):
+        out = np.empty(self.nuser if n is None else n)
+        check(self.lib.clawgpu_dev_hooks_user(self.h, None, _dp(out), out.size))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clawgpu_dev_hooks_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 class Grid2D:
     """Device-resident single grid (one clawgpu context)."""
 
@@ -185,9 +220,15 @@ class Grid2D:
         return cfl.value
 
     def claw2(self, dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc,
-              use_fwave=False, bc2=None, b4step2=None, src2=None, xlower=0.0, ylower=0.0):
+              use_fwave=False, bc2=None, b4step2=None, src2=None, xlower=0.0, ylower=0.0,
+              dev_hooks=None):
         """claw2 time loop on the device q; returns dict of the outputs.
-        bc2 / b4step2 / src2: optional user routines (see make_hooks)."""
+        bc2 / b4step2 / src2: optional user routines (see make_hooks);
+        dev_hooks: a DeviceHooks of device routines instead."""
+        if dev_hooks is not None:
+            return self._claw2_hooks(dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim,
+                                     mthbc, use_fwave, None, None, None, xlower, ylower,
+                                     dev_hooks)
         if bc2 is not None or b4step2 is not None or src2 is not None:
             return self._claw2_hooks(dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim,
                                      mthbc, use_fwave, bc2, b4step2, src2, xlower, ylower)
@@ -207,7 +248,7 @@ class Grid2D:
         return dict(info=info.value, tend=te.value, dtv=dtv_a, cflv=cflv_a, nv=nv)
 
     def _claw2_hooks(self, dx, dy, tstart, tend, dtv, cflv, nmax, method, mthlim, mthbc,
-                     use_fwave, bc2, b4step2, src2, xlower, ylower):
+                     use_fwave, bc2, b4step2, src2, xlower, ylower, dev_hooks=None):
         te = C.c_double(tend)
         dtv_a = np.zeros(5)
         dtv_a[: len(dtv)] = dtv
@@ -217,6 +258,13 @@ class Grid2D:
         info = C.c_int(0)
         m = _ivec(method, 7).copy()
         m[6] = self.maux
+        if dev_hooks is not None:
+            rc = self.lib.clawgpu_claw2_dev_hooks(
+                self.h, dx, dy, float(xlower), float(ylower), tstart, C.byref(te), _dp(dtv_a),
+                _dp(cflv_a), _ip(nv), _ip(m), _ip(_ivec(mthlim, self.mwaves)),
+                _ip(_ivec(mthbc, 4)), int(bool(use_fwave)), dev_hooks.h, C.byref(info))
+            check(rc)
+            return dict(info=info.value, tend=te.value, dtv=dtv_a, cflv=cflv_a, nv=nv)
         hooks, keep, errors = make_hooks(bc2, b4step2, src2)
         rc = self.lib.clawgpu_claw2_hooks(self.h, dx, dy, float(xlower), float(ylower), tstart,
                                           C.byref(te), _dp(dtv_a), _dp(cflv_a), _ip(nv), _ip(m),

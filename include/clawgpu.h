@@ -168,6 +168,52 @@ int clawgpu_claw2_hooks(clawgpu_ctx* ctx, double dx, double dy, double xlower, d
                         const int* method, const int* mthlim, const int* mthbc, int use_fwave,
                         const clawgpu_hooks* hooks, int* info);
 
+/* Device-side user routines: the same plug-in contract (bc2, b4step2, src2
+ * of claw2.f:620-737) as HIP kernels in a gfx950 code object, launched by
+ * the library on the device q at the points clawgpu_claw2_hooks calls the
+ * host routines, so q never leaves the GPU.  Each kernel takes one
+ * clawgpu_dev_grid by value and is launched with one thread per cell,
+ * ghost cells included: thread t of the 1D grid (blocks of 256 threads)
+ * handles cell (i, j) with t = (j + mbc - 1) * (mx + 2 mbc) + (i + mbc - 1),
+ * threads with t beyond the grid must return (include/clawgpu_device.h has
+ * the index helpers).  Build a code object with
+ *   hipcc --genco --offload-arch=gfx950 -O3 -I include hooks.hip -o hooks.co
+ * (add -ffp-contract=off for the reference's rounding). */
+typedef struct clawgpu_dev_grid {
+    double* q;       /* q(m, i, j) at q[m * fs + (j + mbc - 1) * pitch + (i + mbc - 1)] */
+    double* aux;     /* aux(m, i, j) likewise; NULL when maux = 0 */
+    double* user;    /* the hooks' user array (clawgpu_dev_hooks_user), on the device */
+    long fs;         /* doubles per field plane */
+    int pitch;       /* doubles per row */
+    int meqn, maux, mbc, mx, my;
+    int mthbc[4];
+    double xlower, ylower, dx, dy, t, dt;
+} clawgpu_dev_grid;
+
+typedef struct clawgpu_dev_hooks clawgpu_dev_hooks;
+
+/* Load the code object at path (no reference counterpart: device code
+ * cannot be linked into the Fortran loop) and look up the named kernels;
+ * a NULL name leaves that routine out (built-in bc2 for NULL bc2).  nuser
+ * doubles of device memory, zeroed, are passed to the kernels as
+ * clawgpu_dev_grid.user.  Load on the device the contexts run on. */
+int clawgpu_dev_hooks_load(clawgpu_dev_hooks** hooks, const char* path, const char* bc2,
+                           const char* b4step2, const char* src2, int nuser);
+int clawgpu_dev_hooks_destroy(clawgpu_dev_hooks* hooks);
+/* Copy n doubles of the user array from the host (in, if not NULL), then
+ * back to the host (out, if not NULL), after the context's pending work. */
+int clawgpu_dev_hooks_user(clawgpu_dev_hooks* hooks, const double* in, double* out, int n);
+
+/* clawgpu_claw2_hooks with device routines: bc2 then b4step2 on the device
+ * q before every step attempt (again on a retake), src2 over dt/2 before
+ * and after the step (method(5) = 2; the state before the first half step
+ * is kept on the device for a retake) or over dt after it (method(5) = 1). */
+int clawgpu_claw2_dev_hooks(clawgpu_ctx* ctx, double dx, double dy, double xlower,
+                            double ylower, double tstart, double* tend, double* dtv,
+                            double* cflv, int* nv, const int* method, const int* mthlim,
+                            const int* mthbc, int use_fwave, const clawgpu_dev_hooks* hooks,
+                            int* info);
+
 /* Tuning and measurement helpers (no reference counterpart): rows per
  * wavefront segment of the step kernel (0, the default: chosen per grid so
  * that the kernel's waves fill whole rounds of the chip; get_segment_rows

@@ -177,3 +177,105 @@ def test_user_bc1_and_bc3_equal_builtin():
             res.append((g.get_q()[2:-2, 2:-2, 2:-2], out.nv[1], out.tend))
     np.testing.assert_array_equal(res[0][0], res[1][0])
     assert res[0][1:] == res[1][1:]
+
+
+# ------------------------------------------------------ device-side routines
+
+DEV_CO = os.path.join(os.path.dirname(os.path.abspath(__file__)), "device_hooks.co")
+
+
+def _dev_hooks(k, ncalls, **names):
+    h = gpu.DeviceHooks(DEV_CO, nuser=3 + 3 * ncalls, **names)
+    h.write_user([k, 0.0, float(ncalls)])
+    return h
+
+
+def _dev_log(h):
+    u = h.read_user()
+    n = int(u[1])
+    assert n <= int(u[2]), "more calls than the log holds"
+    return u[3:3 + 3 * n].reshape(n, 3)
+
+
+@pytest.mark.parametrize("name", list(H.CASES))
+def test_device_hooks_match_reference_claw2(name):
+    """The test routines as device kernels in a code object
+    (tests/device_hooks.hip; clawgpu_claw2_dev_hooks): the same calls with
+    the same (t, dt) as the reference claw2 calling the host routines --
+    retakes and Strang half steps included, the pre-source state restored on
+    the device for a retake -- and the same final state, bit for bit."""
+    c = H.CASES[name]
+    g0 = np.load(os.path.join(GOLD, f"claw2_hooks_{name}.npz"))
+    q0, dx, dy = H.initial(name)
+    euler = c["solver"] == "euler"
+    sid, params, mthlim = ((SOLVER_EULER, [1.4, 1.0], [4, 4, 4]) if euler
+                           else (SOLVER_ACOUSTICS, [1.0, 4.0], [4, 4]))
+    with _dev_hooks(c["k"], len(g0["log"]) + 16, b4step2="test_b4step2", src2="test_src2") as dh, \
+            gpu.Grid2D(sid, c["mx"], c["my"], params) as g:
+        g.set_q(q0)
+        out = g.claw2(dx, dy, 0.0, c["tend"], list(c["dtv"]), [1.0, 0.9], 500, c["method"], mthlim,
+                      c["mthbc"], dev_hooks=dh)
+        q = g.get_q()
+        log = _dev_log(dh)
+    print(name, "steps", out["nv"][1], "calls", len(log), flush=True)
+    np.testing.assert_array_equal(log, g0["log"])
+    assert out["info"] == int(g0["info"]) and out["tend"] == float(g0["tend"])
+    assert out["nv"][1] == g0["nv"][1]
+    np.testing.assert_array_equal(out["dtv"][2:5], g0["dtv"][2:5])
+    np.testing.assert_array_equal(out["cflv"][2:4], g0["cflv"][2:4])
+    np.testing.assert_array_equal(interior(q), interior(g0["q"]))
+
+
+def test_device_bc2_equals_builtin():
+    """A device user boundary routine (extrapolation) gives the built-in
+    mthbc = 1 result bit for bit, retakes included."""
+    q0, dx, dy = H.initial("euler_src_strang")
+    method = (1, 2, 2, 0, 0, 0, 0)
+    res = []
+    for user in (False, True):
+        with _dev_hooks(0.0, 1, bc2="test_bc2_extrap") as dh, \
+                gpu.Grid2D(SOLVER_EULER, q0.shape[1] - 4, q0.shape[0] - 4, [1.4, 1.0]) as g:
+            g.set_q(q0)
+            kw = dict(dev_hooks=dh) if user else {}
+            out = g.claw2(dx, dy, 0.0, 0.1, [0.02, 1e99], [1.0, 0.9], 500, method, [4, 4, 4],
+                          (1, 1, 1, 1) if not user else (0, 0, 0, 0), **kw)
+            res.append((interior(g.get_q()), out))
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    assert res[0][1]["nv"][1] == res[1][1]["nv"][1] and res[0][1]["tend"] == res[1][1]["tend"]
+
+
+def test_device_hooks_keep_q_on_the_device():
+    """At 1024^2 (Euler, src2 every step) the device routines run the loop
+    without moving q: bit for bit with the host routines, and faster (each
+    host call copies q to the host and back)."""
+    import time
+
+    n, k, steps = 1024, 0.9, 12
+    from classic_amd import problems as P
+    q0, dx, dy = P.euler_quadrants(n, n, seed=5)
+    method, mthlim = (1, 2, 2, 0, 1, 0, 0), [4, 4, 4]
+    b4, src, log = H.python_hooks(k)
+    times, qs = [], []
+    for dev in (False, True):
+        with _dev_hooks(k, 4 * steps, b4step2="test_b4step2", src2="test_src2") as dh, \
+                gpu.Grid2D(SOLVER_EULER, n, n, [1.4, 1.0]) as g:
+            g.set_q(q0)
+            kw = dict(dev_hooks=dh) if dev else dict(b4step2=b4, src2=src)
+            t0 = time.perf_counter()
+            out = g.claw2(dx, dy, 0.0, 1e9, [1e-4, 1e99], [1.0, 0.9], steps, method, mthlim,
+                          (1, 1, 1, 1), **kw)
+            times.append(time.perf_counter() - t0)
+            qs.append(g.get_q())
+            assert out["nv"][1] == steps
+    print(f"1024^2, {steps} steps with src2: host routines {times[0] * 1e3:.1f} ms, "
+          f"device routines {times[1] * 1e3:.1f} ms", flush=True)
+    np.testing.assert_array_equal(interior(qs[0]), interior(qs[1]))
+    assert times[1] < times[0]
+
+
+def test_device_hooks_load_errors():
+    from classic_amd._lib import ClawGpuError
+    with pytest.raises(ClawGpuError):
+        gpu.DeviceHooks(DEV_CO, src2="no_such_kernel")
+    with pytest.raises(ClawGpuError):
+        gpu.DeviceHooks(DEV_CO + ".missing", src2="test_src2")
