@@ -19,6 +19,7 @@ extern "C" int clawgpu_dev_hooks_load(clawgpu_dev_hooks** out, const char* path,
     if (!h) return set_err(CLAWGPU_ERR_ARG, "out of host memory");
     hipError_t e = hipModuleLoad(&h->mod, path);
     if (e != hipSuccess) {
+        (void)hipGetLastError();  // reported here: later launch checks must not see it
         snprintf(g_errbuf, sizeof g_errbuf, "cannot load code object %s: %s", path,
                  hipGetErrorString(e));
         delete h;
@@ -30,6 +31,7 @@ extern "C" int clawgpu_dev_hooks_load(clawgpu_dev_hooks** out, const char* path,
         if (!names[k]) continue;
         e = hipModuleGetFunction(fns[k], h->mod, names[k]);
         if (e != hipSuccess) {
+            (void)hipGetLastError();
             snprintf(g_errbuf, sizeof g_errbuf, "kernel %s not found in %s", names[k], path);
             clawgpu_dev_hooks_destroy(h);
             return CLAWGPU_ERR_ARG;
@@ -40,6 +42,7 @@ extern "C" int clawgpu_dev_hooks_load(clawgpu_dev_hooks** out, const char* path,
         e = hipMalloc((void**)&h->user, (size_t)nuser * sizeof(double));
         if (e == hipSuccess) e = hipMemset(h->user, 0, (size_t)nuser * sizeof(double));
         if (e != hipSuccess) {
+            (void)hipGetLastError();
             clawgpu_dev_hooks_destroy(h);
             return set_err(CLAWGPU_ERR_HIP, "user array allocation failed");
         }

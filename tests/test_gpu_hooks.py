@@ -274,8 +274,16 @@ def test_device_hooks_keep_q_on_the_device():
 
 
 def test_device_hooks_load_errors():
+    """A missing code object or kernel is an error of the load call alone:
+    the next step on the device runs normally."""
     from classic_amd._lib import ClawGpuError
     with pytest.raises(ClawGpuError):
         gpu.DeviceHooks(DEV_CO, src2="no_such_kernel")
     with pytest.raises(ClawGpuError):
         gpu.DeviceHooks(DEV_CO + ".missing", src2="test_src2")
+    q0, dx, dy = H.initial("euler_src_strang")
+    with gpu.Grid2D(SOLVER_EULER, q0.shape[1] - 4, q0.shape[0] - 4, [1.4, 1.0]) as g:
+        g.set_q(q0)
+        out = g.claw2(dx, dy, 0.0, 0.02, [0.02, 1e99], [1.0, 0.9], 10, (1, 2, 2, 0, 0, 0, 0),
+                      [4, 4, 4], (1, 1, 1, 1))
+        assert out["tend"] == pytest.approx(0.02) and np.isfinite(g.get_q()).all()
