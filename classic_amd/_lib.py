@@ -24,6 +24,7 @@ SOLVER_EULER = 2
 SOLVER_VC_ACOUSTICS = 3
 SOLVER_ADVECTION = 4
 SOLVER_VC_ADVECTION = 5
+SOLVER_USER = 100
 
 ARITH_EXACT = 0
 ARITH_FAST = 1
@@ -56,6 +57,10 @@ _SIGS = {
     "clawgpu_dev_hooks_load": ([C.POINTER(_V), C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                 _i], _i),
     "clawgpu_dev_hooks_destroy": ([_V], _i),
+    "clawgpu_solver2_load": ([C.POINTER(_V), C.c_char_p, C.c_char_p], _i),
+    "clawgpu_solver2_destroy": ([_V], _i),
+    "clawgpu_solver2_shape": ([_V, _I, _I, _I], _i),
+    "clawgpu_create_user": ([C.POINTER(_V), _V, _i, _i, _i, _D, _i], _i),
     "clawgpu_dev_hooks_user": ([_V, _D, _D, _i], _i),
     "clawgpu_set_segment_rows": ([_V, _i], _i),
     "clawgpu_get_segment_rows": ([_V, _I], _i),

@@ -58,11 +58,15 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     return LIB
 
 
-def build_code_object(src, out, verbose=False):
+def build_code_object(src, out, verbose=False, extra=()):
     """Compile a HIP source of device-side user routines (include/clawgpu.h,
-    clawgpu_dev_grid) into a gfx950 code object for clawgpu_dev_hooks_load,
-    with the library's arithmetic flags (no contraction)."""
-    cmd = [HIPCC, "--genco", *[f for f in FLAGS if f not in ("-fPIC",)], "-o", out, src]
+    clawgpu_dev_grid) or user solvers (include/clawgpu_solver.h, which needs
+    the kernel headers of csrc/) into a gfx950 code object for
+    clawgpu_dev_hooks_load / clawgpu_solver2_load, with the library's
+    arithmetic flags (no contraction unless the source asks for it) and any
+    extra flags."""
+    cmd = [HIPCC, "--genco", *[f for f in FLAGS if f not in ("-fPIC",)],
+           "-I", os.path.join(HERE, "csrc"), *extra, "-o", out, src]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

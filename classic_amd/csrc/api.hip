@@ -61,6 +61,7 @@ using namespace clawgpu;
 #include "ctx.hpp"
 #include "devhooks.hpp"
 #include "step2g.hpp"
+#include "usolver.hpp"
 
 thread_local char g_errbuf[256];
 
@@ -112,6 +113,9 @@ static int solver_shape(int solver, int* meqn, int* mwaves)
     }
 }
 
+static int create_ctx(clawgpu_ctx** out, int meqn, int mwaves, int maux, int mbc, int mx, int my,
+                      int solver_id, const SolverParams& P, const clawgpu_solver2* us);
+
 extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux, int mbc, int mx,
                               int my, int solver_id, const double* params, int nparams)
 {
@@ -131,6 +135,14 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     SolverParams P{};
     const int prc = solver_params(solver_id, params, nparams, &P);
     if (prc) return prc;
+    return create_ctx(out, meqn, mwaves, maux, mbc, mx, my, solver_id, P, nullptr);
+}
+
+// The context of a validated grid and solver: buffers, stream, layout.
+static int create_ctx(clawgpu_ctx** out, int meqn, int mwaves, int maux, int mbc, int mx, int my,
+                      int solver_id, const SolverParams& P, const clawgpu_solver2* us)
+{
+    const bool general = solver_id == CLAWGPU_SOLVER_VC_ADVECTION;
     clawgpu_ctx* c = new (std::nothrow) clawgpu_ctx();
     if (!c) return set_err(CLAWGPU_ERR_ARG, "out of host memory");
     c->meqn = meqn;
@@ -140,6 +152,7 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     c->mx = mx;
     c->my = my;
     c->solver = solver_id;
+    c->us = us;
     c->P = P;
     c->L.mx = mx;
     c->L.my = my;
@@ -148,7 +161,7 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     c->L.fs = ((long)c->L.pitch * (my + 2 * mbc) + 31) / 32 * 32;
     c->cur = 0;
     c->seglen = 0;
-    c->arith = ARITH_EXACT;
+    c->arith = us && us->fast ? ARITH_FAST : ARITH_EXACT;
     {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -184,6 +197,22 @@ extern "C" int clawgpu_create(clawgpu_ctx** out, int meqn, int mwaves, int maux,
     }
     *out = c;
     return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_create_user(clawgpu_ctx** out, const clawgpu_solver2* us, int mbc, int mx,
+                                   int my, const double* params, int nparams)
+{
+    if (!out) return set_err(CLAWGPU_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!us) return set_err(CLAWGPU_ERR_ARG, "null solver");
+    if (nparams < 0 || nparams > kUserParams || (nparams > 0 && !params))
+        return set_err(CLAWGPU_ERR_ARG, "a user solver takes 0..16 parameters");
+    if (mbc < 2) return set_err(CLAWGPU_ERR_ARG, "mbc must be >= 2 (claw2 info=1)");
+    if (mx < 1 || my < 1) return set_err(CLAWGPU_ERR_ARG, "grid must have at least one cell");
+    SolverParams P{};
+    P.nuser = nparams;
+    for (int k = 0; k < nparams; k++) P.user[k] = params[k];
+    return create_ctx(out, us->meqn, us->mwaves, 0, mbc, mx, my, CLAWGPU_SOLVER_USER, P, us);
 }
 
 extern "C" int clawgpu_destroy(clawgpu_ctx* c)
@@ -266,6 +295,16 @@ static int check_bc(const int* mthbc)
     return 0;
 }
 
+// Walls negate components 2 and 3 (1-based), as bc2 does: a user solver
+// with fewer equations has no such components.
+static int check_walls(const clawgpu_ctx* c, const int* mthbc)
+{
+    if (c->us && c->meqn < 3)
+        for (int k = 0; k < 4; k++)
+            if (mthbc[k] == 3) return set_err(CLAWGPU_ERR_ARG, "wall boundaries (mthbc = 3) need meqn >= 3");
+    return CLAWGPU_OK;
+}
+
 static int enqueue_bc(clawgpu_ctx* c, double* q, const int* mthbc, bool clear_cfl)
 {
     BcArgs b;
@@ -285,6 +324,7 @@ extern "C" int clawgpu_bc2(clawgpu_ctx* c, const int* mthbc)
     if (!c || !mthbc) return set_err(CLAWGPU_ERR_ARG, "null argument");
     if (check_bc(mthbc) != 0)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides)");
+    if (const int wrc = check_walls(c, mthbc)) return wrc;
     int rc = enqueue_bc(c, c->dq[c->cur], mthbc, false);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -353,8 +393,9 @@ static int step_rows(clawgpu_ctx* c, const StepArgs& a)
     for (int k = 0; k < c->mwaves; k++) mc = mc && a.lim[k] == 4;
     if (c->auto_key[0] == c->arith && c->auto_key[1] == (int)mc && c->auto_key[2] == c->my)
         return c->auto_rows;
-    const int wpc = c->arith == ARITH_FAST ? step2_waves_per_cu_fast(c->solver, a)
-                                           : step2_waves_per_cu_exact(c->solver, a);
+    const int wpc = c->us                    ? usolver_waves_per_cu(c->us)
+                    : c->arith == ARITH_FAST ? step2_waves_per_cu_fast(c->solver, a)
+                                             : step2_waves_per_cu_exact(c->solver, a);
     const int best = segment_rows_for(c->ncu, wpc, a.nstrips, c->my);
     c->auto_key[0] = c->arith;
     c->auto_key[1] = (int)mc;
@@ -415,8 +456,9 @@ static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const i
     a.cfl_bits = c->dcfl;
     long waves = (long)a.nstrips * a.nsegs;
     int nb = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    int e = c->arith == ARITH_FAST ? launch_step2_fast(c->solver, a, nb, c->stream)
-                                   : launch_step2_exact(c->solver, a, nb, c->stream);
+    int e = c->us                    ? usolver_step(c->us, a, nb, c->stream)
+            : c->arith == ARITH_FAST ? launch_step2_fast(c->solver, a, nb, c->stream)
+                                     : launch_step2_exact(c->solver, a, nb, c->stream);
     HIPCHK(c, (hipError_t)e);
     return CLAWGPU_OK;
 }
@@ -494,8 +536,9 @@ static int enqueue_sweep(clawgpu_ctx* c, int dir, int from, int to, double dtd, 
     a.P = c->P;
     long waves = (long)a.nstrips * a.nsegs;
     int nb = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    int e = c->arith == ARITH_FAST ? launch_sweep_fast(c->solver, dir, a, nb, c->stream)
-                                   : launch_sweep_exact(c->solver, dir, a, nb, c->stream);
+    int e = c->us                    ? usolver_sweep(c->us, dir, a, nb, c->stream)
+            : c->arith == ARITH_FAST ? launch_sweep_fast(c->solver, dir, a, nb, c->stream)
+                                     : launch_sweep_exact(c->solver, dir, a, nb, c->stream);
     HIPCHK(c, (hipError_t)e);
     return CLAWGPU_OK;
 }
@@ -615,8 +658,9 @@ static int claw2_device(clawgpu_ctx* c, double dx, double dy, double tstart, dou
             const int m = est > 0 ? (int)(est < 32 ? est : 32) : 1;
             for (int j = 0; j < m; j++) {
                 l.k = k++;
-                HIPCHK(c, (hipError_t)(fast ? launch_loop2_fast(c->solver, a, l, nb, c->stream)
-                                            : launch_loop2_exact(c->solver, a, l, nb, c->stream)));
+                HIPCHK(c, (hipError_t)(c->us ? usolver_loop(c->us, a, l, nb, c->stream)
+                                       : fast  ? launch_loop2_fast(c->solver, a, l, nb, c->stream)
+                                               : launch_loop2_exact(c->solver, a, l, nb, c->stream)));
             }
             const int slot = (qh + qn) % 2;
             snap_k[slot] = k;
@@ -685,6 +729,9 @@ static int claw2_loop(clawgpu_ctx* c, double dx, double dy, double xlower, doubl
             if (mthlim[mw] < 0 || mthlim[mw] > 5) return set_err(CLAWGPU_ERR_ARG, "invalid limiter method");
     if (!user_bc && check_bc(mthbc) != 0)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides)");
+    if (!user_bc) {
+        if (const int wrc = check_walls(c, mthbc)) return wrc;
+    }
     std::vector<double> hq, saved;  // host copies of q for the hooks
     double* haux = c->maux > 0 && !c->haux.empty() ? c->haux.data() : nullptr;
     auto pull = [&]() -> int {
@@ -1089,6 +1136,8 @@ extern "C" int clawgpu_set_arith(clawgpu_ctx* c, int mode)
 {
     if (!c || (mode != CLAWGPU_ARITH_EXACT && mode != CLAWGPU_ARITH_FAST))
         return set_err(CLAWGPU_ERR_ARG, "mode must be CLAWGPU_ARITH_EXACT or CLAWGPU_ARITH_FAST");
+    if (c->us && mode != c->arith)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "a user solver runs in the arithmetic its code object was built with");
     c->arith = mode;
     return CLAWGPU_OK;
 }
@@ -1107,6 +1156,7 @@ extern "C" int clawgpu_time_steps(clawgpu_ctx* c, double dx, double dy, double d
     int rc = check_method(c, method, mthlim);
     if (rc) return rc;
     if (check_bc(mthbc) != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "bad mthbc");
+    if (const int wrc = check_walls(c, mthbc)) return wrc;
     hipEvent_t* ev = new (std::nothrow) hipEvent_t[2 * nsteps + 2];
     if (!ev) return set_err(CLAWGPU_ERR_ARG, "out of host memory");
     for (int k = 0; k < 2 * nsteps + 2; k++) HIPCHK(c, hipEventCreate(&ev[k]));
@@ -1168,7 +1218,8 @@ extern "C" int clawgpu_solver_probe(clawgpu_ctx* c, int ixy, int n, const double
         (e = hipMemcpy(das, asdq, nq * 8, hipMemcpyHostToDevice)) != hipSuccess) {
         rc = set_err(CLAWGPU_ERR_HIP, hipGetErrorString(e));
     } else {
-        int le = c->arith == ARITH_FAST
+        int le = c->us ? usolver_probe(c->us, n, ixy, c->P, dql, dqr, das, dw, ds, dam, dap, dbm, dbp, c->stream)
+                 : c->arith == ARITH_FAST
                      ? launch_probe_fast(c->solver, n, ixy, c->P, dql, dqr, das, dw, ds, dam, dap, dbm, dbp, c->stream)
                      : launch_probe_exact(c->solver, n, ixy, c->P, dql, dqr, das, dw, ds, dam, dap, dbm, dbp, c->stream);
         if (le != 0 || (e = hipStreamSynchronize(c->stream)) != hipSuccess ||

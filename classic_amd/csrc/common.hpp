@@ -9,10 +9,14 @@ constexpr int kMaxWaves = 8;
 constexpr int kWavesPerBlock = 4;
 constexpr int kStripOut = 60;  // output columns per 64-lane strip (lanes 2..61)
 
+constexpr int kUserParams = 16;  // parameters of a user solver
+
 struct SolverParams {
     double rho, bulk, cc, zz;  // acoustics: density, bulk modulus, c, impedance
     double gamma, gamma1;      // Euler: gamma and gamma - 1
     int efix;                  // Euler: Harten-Hyman entropy fix
+    int nuser;                 // user solver: parameters given
+    double user[kUserParams];  // user solver: its parameters, zero beyond nuser
 };
 
 // Field-plane layout of a grid with mbc ghost cells: element (m, i, j) at

@@ -13,6 +13,8 @@
 
 using namespace clawgpu;
 
+struct clawgpu_solver2;
+
 struct clawgpu_ctx {
     int meqn, mwaves, maux, mbc, mx, my, solver;
     SolverParams P;
@@ -40,6 +42,7 @@ struct clawgpu_ctx {
     unsigned long long* dcflb = nullptr;
     LoopState* hloop = nullptr;  // 3 entries
     double* dsave = nullptr;     // q before a Strang source half step (device user routines)
+    const clawgpu_solver2* us = nullptr;  // user solver (CLAWGPU_SOLVER_USER), not owned
     int last_hip_error;
 };
 

@@ -721,6 +721,17 @@ static LearnedState* learned_state(clawgpu_ctx* c)
     return s;
 }
 
+// The context's learned-limiter state, or an error: the features and the
+// network are defined for the built-in acoustics and Euler solvers only.
+static int learned_get(clawgpu_ctx* c, LearnedState** out)
+{
+    if (c->solver != CLAWGPU_SOLVER_ACOUSTICS && c->solver != CLAWGPU_SOLVER_EULER)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "the learned limiter runs with the built-in solvers only");
+    *out = learned_state(c);
+    if (!*out) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    return CLAWGPU_OK;
+}
+
 void learned_release(clawgpu_ctx* c)
 {
     LearnedState* s = (LearnedState*)c->learned;
@@ -849,16 +860,16 @@ static int enqueue_fused(clawgpu_ctx* c, LearnedState* s, double dx, double dy, 
 
 int learned_prepare(clawgpu_ctx* c, double dx, double dy, double dt)
 {
-    LearnedState* s = learned_state(c);
-    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    LearnedState* s = nullptr;
+    if (const int rc_ = learned_get(c, &s)) return rc_;
     return enqueue_fused(c, s, dx, dy, dt);
 }
 
 extern "C" int clawgpu_learned_set_params(clawgpu_ctx* c, const float* params, int n)
 {
     if (!c || !params || n != kNParams) return set_err(CLAWGPU_ERR_ARG, "expected 1281 parameters");
-    LearnedState* s = learned_state(c);
-    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    LearnedState* s = nullptr;
+    if (const int rc_ = learned_get(c, &s)) return rc_;
     HIPCHK(c, hipMemcpy(s->params, params, kNParams * sizeof(float), hipMemcpyHostToDevice));
     s->have_params = true;
     return CLAWGPU_OK;
@@ -867,8 +878,8 @@ extern "C" int clawgpu_learned_set_params(clawgpu_ctx* c, const float* params, i
 extern "C" int clawgpu_learned_features(clawgpu_ctx* c, double dx, double dy, double dt)
 {
     if (!c) return set_err(CLAWGPU_ERR_ARG, "null context");
-    LearnedState* s = learned_state(c);
-    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    LearnedState* s = nullptr;
+    if (const int rc_ = learned_get(c, &s)) return rc_;
     int rc = enqueue_features(c, s, dx, dy, dt);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -893,8 +904,8 @@ extern "C" int clawgpu_learned_time(clawgpu_ctx* c, double dx, double dy, double
                                     float* fused_ms, float* infer_ms)
 {
     if (!c || reps < 1 || !fused_ms || !infer_ms) return set_err(CLAWGPU_ERR_ARG, "bad argument");
-    LearnedState* s = learned_state(c);
-    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    LearnedState* s = nullptr;
+    if (const int rc_ = learned_get(c, &s)) return rc_;
     if (!s->have_params) return set_err(CLAWGPU_ERR_ARG, "learned limiter parameters not set");
     hipEvent_t ev[4];
     for (int k = 0; k < 4; k++) HIPCHK(c, hipEventCreate(&ev[k]));
@@ -944,8 +955,8 @@ extern "C" int clawgpu_learned_write(clawgpu_ctx* c, int what, const void* in, l
     // host: for the gradient checks of the training kernels and for tests
     // that a pass writes every entry
     if (!c || !in) return set_err(CLAWGPU_ERR_ARG, "bad argument");
-    LearnedState* s = learned_state(c);
-    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    LearnedState* s = nullptr;
+    if (const int rc_ = learned_get(c, &s)) return rc_;
     void* dst = nullptr;
     size_t bytes = 0;
     switch (what) {
@@ -971,8 +982,8 @@ extern "C" int clawgpu_learned_table_size(clawgpu_ctx* c, long* waves_per_direct
 extern "C" int clawgpu_learned_set_target(clawgpu_ctx* c, const double* q)
 {
     if (!c || !q) return set_err(CLAWGPU_ERR_ARG, "bad argument");
-    LearnedState* s = learned_state(c);
-    if (!s) return set_err(CLAWGPU_ERR_HIP, "learned limiter allocation failed");
+    LearnedState* s = nullptr;
+    if (const int rc_ = learned_get(c, &s)) return rc_;
     // reuse the upload path: stage into the current buffer's spare copy
     int keep = c->cur;
     c->cur = (keep + 2) % 3;
@@ -1009,8 +1020,9 @@ extern "C" int clawgpu_learned_grad(clawgpu_ctx* c, double dx, double dy, double
 {
     if (!c || !grad || !loss || mode < 0 || mode > 3 || !method || !mthbc)
         return set_err(CLAWGPU_ERR_ARG, "bad argument");
-    LearnedState* s = learned_state(c);
-    if (!s || !s->have_params) return set_err(CLAWGPU_ERR_ARG, "parameters not set");
+    LearnedState* s = nullptr;
+    if (const int rc_ = learned_get(c, &s)) return rc_;
+    if (!s->have_params) return set_err(CLAWGPU_ERR_ARG, "parameters not set");
     HIPCHK(c, hipMemsetAsync(s->grad, 0, kNParams * sizeof(float), c->stream));
     HIPCHK(c, hipMemsetAsync(s->lossacc, 0, sizeof(double), c->stream));
     int rc;

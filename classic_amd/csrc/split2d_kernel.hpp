@@ -39,8 +39,10 @@ struct SweepArgs {
     SolverParams P;
 };
 
+// Kernel bodies (shared with the kernels of a user solver's code object,
+// include/clawgpu_solver.h); the kernels follow launch_sweep_impl's callers.
 template <class S>
-__global__ __launch_bounds__(256) void sweep_x_kernel(SweepArgs a)
+__device__ __forceinline__ void sweep_x_body(const SweepArgs& a)
 {
     using Mth = typename S::Math;
     using Cell = typename S::Cell;
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(SweepArgs a)
 }
 
 template <class S>
-__global__ __launch_bounds__(256) void sweep_y_kernel(SweepArgs a)
+__device__ __forceinline__ void sweep_y_body(const SweepArgs& a)
 {
     using Mth = typename S::Math;
     using Cell = typename S::Cell;
@@ -276,6 +278,18 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(SweepArgs a)
         cprev = cc;
     }
     publish_cfl(cfl, a.cfl_bits);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void sweep_x_kernel(SweepArgs a)
+{
+    sweep_x_body<S>(a);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void sweep_y_kernel(SweepArgs a)
+{
+    sweep_y_body<S>(a);
 }
 
 template <class Mth>

@@ -877,13 +877,22 @@ __device__ __forceinline__ void step2_wave(const StepArgs& a, const int strip, c
     publish_cfl(cfl, a.cfl_bits);
 }
 
+// Kernel bodies: the library's kernels below and the kernels a user solver's
+// code object exports (include/clawgpu_solver.h) are these functions with
+// the same launch bounds.
 template <class S, class Cfg>
-__global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(StepArgs a)
+__device__ __forceinline__ void step2_unsplit_body(const StepArgs& a)
 {
     const int lane = threadIdx.x & 63;
     const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (wid >= a.nstrips * a.nsegs) return;  // uniform per wave
     step2_wave<S, Cfg>(a, wid % a.nstrips, wid / a.nstrips, lane);
+}
+
+template <class S, class Cfg>
+__global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step2_unsplit_kernel(StepArgs a)
+{
+    step2_unsplit_body<S, Cfg>(a);
 }
 
 // A value equal in all lanes, moved to scalar registers.
@@ -903,7 +912,7 @@ __device__ __forceinline__ double uniform_double(double x)
 // (FAST arithmetic: held to two waves per SIMD, the unsplit kernel's
 // occupancy, which the loop's few extra live values would otherwise cost.)
 template <class S, class Cfg, int BC>
-__global__ __launch_bounds__(256, S::Math::fast ? 2 : CLAWGPU_MIN_WAVES) void step2_loop_kernel(StepArgs a, LoopArgs l)
+__device__ __forceinline__ void step2_loop_body(StepArgs a, const LoopArgs& l)
 {
     LoopState s;
     if (l.k == 0)
@@ -930,15 +939,22 @@ __global__ __launch_bounds__(256, S::Math::fast ? 2 : CLAWGPU_MIN_WAVES) void st
     step2_wave<S, Cfg, BC>(a, wid % a.nstrips, wid / a.nstrips, lane);
 }
 
+template <class S, class Cfg, int BC>
+__global__ __launch_bounds__(256, S::Math::fast ? 2 : CLAWGPU_MIN_WAVES) void step2_loop_kernel(StepArgs a, LoopArgs l)
+{
+    step2_loop_body<S, Cfg, BC>(a, l);
+}
+
 // Solver-level evaluation for tests: interface k between states ql[k]
 // (left, reference layout q(meqn)) and qr[k]; outputs wave(meqn, mwaves),
 // s(mwaves), amdq, apdq, and the transverse split of asdq[k] (imp = 1 uses
 // amdq-like data, the split does not depend on imp) into bmasdq, bpasdq.
 template <class S>
-__global__ void solver_probe_kernel(int n, int ixy, SolverParams P, const double* ql,
-                                    const double* qr, const double* asdq, double* wave,
-                                    double* s, double* amdq, double* apdq, double* bm,
-                                    double* bp)
+__device__ __forceinline__ void solver_probe_body(int n, int ixy, const SolverParams& P,
+                                                  const double* ql, const double* qr,
+                                                  const double* asdq, double* wave, double* s,
+                                                  double* amdq, double* apdq, double* bm,
+                                                  double* bp)
 {
     constexpr int M = S::M, MW = S::MW;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -963,6 +979,15 @@ __global__ void solver_probe_kernel(int n, int ixy, SolverParams P, const double
         bm[k * M + m] = b0[m];
         bp[k * M + m] = b1[m];
     }
+}
+
+template <class S>
+__global__ void solver_probe_kernel(int n, int ixy, SolverParams P, const double* ql,
+                                    const double* qr, const double* asdq, double* wave,
+                                    double* s, double* amdq, double* apdq, double* bm,
+                                    double* bp)
+{
+    solver_probe_body<S>(n, ixy, P, ql, qr, asdq, wave, s, amdq, apdq, bm, bp);
 }
 
 template <class Mth>

@@ -144,18 +144,52 @@ class DeviceHooks:
         self.close()
 
 
+class UserSolver:
+    """A user Riemann solver: solver `name` of the gfx950 code object at path,
+    built from include/clawgpu_solver.h.  Pass it to Grid2D in place of a
+    solver id; keep it open while its grids exist."""
+
+    def __init__(self, path, name):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        check(self.lib.clawgpu_solver2_load(C.byref(h), str(path).encode(), name.encode()))
+        self.h = h
+        m, w, a = C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.clawgpu_solver2_shape(h, C.byref(m), C.byref(w), C.byref(a)))
+        self.meqn, self.mwaves, self.arith = m.value, w.value, a.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clawgpu_solver2_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 class Grid2D:
-    """Device-resident single grid (one clawgpu context)."""
+    """Device-resident single grid (one clawgpu context).  solver_id is a
+    built-in solver id or a UserSolver (whose params reach it as
+    SolverParams.user)."""
 
     def __init__(self, solver_id, mx, my, params, mbc=2, maux=0, arith=0):
         self.lib = _lib.load()
-        meqn, mwaves = SOLVER_SHAPE[solver_id]
-        self.solver_id, self.mx, self.my, self.mbc = solver_id, mx, my, mbc
-        self.meqn, self.mwaves, self.maux = meqn, mwaves, maux
         p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
         h = C.c_void_p()
-        check(self.lib.clawgpu_create(C.byref(h), meqn, mwaves, maux, mbc, mx, my, solver_id,
-                                      _dp(p), p.size))
+        if isinstance(solver_id, UserSolver):
+            meqn, mwaves = solver_id.meqn, solver_id.mwaves
+            check(self.lib.clawgpu_create_user(C.byref(h), solver_id.h, mbc, mx, my, _dp(p), p.size))
+            self.solver = solver_id
+            solver_id = _lib.SOLVER_USER
+        else:
+            meqn, mwaves = SOLVER_SHAPE[solver_id]
+            check(self.lib.clawgpu_create(C.byref(h), meqn, mwaves, maux, mbc, mx, my, solver_id,
+                                          _dp(p), p.size))
+        self.solver_id, self.mx, self.my, self.mbc = solver_id, mx, my, mbc
+        self.meqn, self.mwaves, self.maux = meqn, mwaves, maux
         self.h = h
         if arith:
             self.set_arith(arith)

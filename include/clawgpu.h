@@ -51,7 +51,8 @@ enum {
     CLAWGPU_SOLVER_EULER = 2,
     CLAWGPU_SOLVER_VC_ACOUSTICS = 3,
     CLAWGPU_SOLVER_ADVECTION = 4,
-    CLAWGPU_SOLVER_VC_ADVECTION = 5
+    CLAWGPU_SOLVER_VC_ADVECTION = 5,
+    CLAWGPU_SOLVER_USER = 100     /* a context of clawgpu_create_user */
 };
 
 /* Arithmetic of the step kernel.  EXACT (the default) performs the
@@ -213,6 +214,27 @@ int clawgpu_claw2_dev_hooks(clawgpu_ctx* ctx, double dx, double dy, double xlowe
                             double* cflv, int* nv, const int* method, const int* mthlim,
                             const int* mthbc, int use_fwave, const clawgpu_dev_hooks* hooks,
                             int* info);
+
+/* User Riemann solvers: the rpn2/rpt2 a caller links into claw2
+ * (claw2.f:312-402, called from flux2.f90:121-235 and step2ds).  Here a
+ * solver is a policy struct compiled into the library's kernel templates by
+ * include/clawgpu_solver.h, built into a gfx950 code object that exports
+ * solver `name`.  Loading checks the kernel-argument layout the object was
+ * built against.  A context of clawgpu_create_user runs every 2D path of a
+ * built-in solver (clawgpu_step2, clawgpu_dimsp2, clawgpu_claw2 with its
+ * device-side loop, the hook variants, clawgpu_solver_probe) with the
+ * user's kernels, in the arithmetic the object was built with
+ * (clawgpu_solver2_shape); it has no aux arrays, capacity function or
+ * learned limiter.  Keep the solver loaded while its contexts exist. */
+typedef struct clawgpu_solver2 clawgpu_solver2;
+int clawgpu_solver2_load(clawgpu_solver2** solver, const char* path, const char* name);
+int clawgpu_solver2_destroy(clawgpu_solver2* solver);
+/* meqn, mwaves and arith (CLAWGPU_ARITH_EXACT or _FAST) of a loaded solver */
+int clawgpu_solver2_shape(const clawgpu_solver2* solver, int* meqn, int* mwaves, int* arith);
+/* clawgpu_create for a user solver: params[0..nparams-1] (nparams <= 16)
+ * reach the solver as SolverParams.user. */
+int clawgpu_create_user(clawgpu_ctx** ctx, const clawgpu_solver2* solver, int mbc, int mx,
+                        int my, const double* params, int nparams);
 
 /* Tuning and measurement helpers (no reference counterpart): rows per
  * wavefront segment of the step kernel (0, the default: chosen per grid so
