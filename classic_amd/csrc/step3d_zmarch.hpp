@@ -53,6 +53,18 @@ __device__ inline Iface from_raw(const Raw& w, const Mat& lo, const Mat& hi)
     return f;
 }
 
+// An interface carried as its wave strengths (a1, a2): the waves are
+// a1 (-Z_lo, 1) and a2 (Z_hi, 1), rebuilt with the products solve() forms,
+// so they are the same doubles.
+struct Alpha {
+    double a1, a2;
+};
+
+__device__ inline Iface from_alpha(const Alpha& w, const Mat& lo, const Mat& hi)
+{
+    return from_raw(Raw{{-w.a1 * lo.Z, w.a1}, {w.a2 * hi.Z, w.a2}}, lo, hi);
+}
+
 __device__ inline Iface waves_only(const Raw& w)
 {
     Iface f{};
@@ -192,16 +204,21 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
     auto mat_of = [&](int id) -> Mat { return {lut[2 * id], lut[2 * id + 1], id}; };
     auto id_at = [&](int jj, int kk) -> int { return a.mat[L.offc(i, jj, kk)]; };
 
-    // material ids of rows j-1..j+1 (first index) at planes k-1..k+2
-    int R[3][4];
+    // material ids (< kMaxMat <= 256) of rows j-1..j+1 at planes k-1..k+2,
+    // one byte per plane: Rp[s] byte p is the id of row j+s-1, plane k-1+p
+    static_assert(kMaxMat <= 256, "material ids are kept as bytes");
+    unsigned Rp[3];
 #pragma unroll
-    for (int s = 0; s < 3; s++)
+    for (int s = 0; s < 3; s++) {
+        Rp[s] = 0u;
 #pragma unroll
-        for (int p = 0; p < 4; p++) R[s][p] = id_at(j + s - 1, kb - 2 + p);
+        for (int p = 0; p < 4; p++) Rp[s] |= (unsigned)id_at(j + s - 1, kb - 2 + p) << (8 * p);
+    }
+    auto R = [&](int s, int p) -> int { return (int)((Rp[s] >> (8 * p)) & 0xffu); };
 
-    // z pipeline of the own column, ready for plane kb: raw interfaces
-    // kb-1/2 (Fa) and kb+1/2 (Fb), cqxx of kb-1/2, q(p, w) of cell kb+1
-    Raw Fa, Fb;
+    // z pipeline of the own column, ready for plane kb: interfaces kb-1/2
+    // (Fa) and kb+1/2 (Fb), cqxx of kb-1/2, q(p, w) of cell kb+1
+    Alpha Fa, Fb;
     Wv cqa;
     double pz1, wz1;
     {
@@ -218,8 +235,8 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
         const Iface f1 = solve<M>(pz[1], wz[1], mat_of(mz_[1]), pz[2], wz[2], mat_of(mz_[2]));
         const Iface f2 = solve<M>(pz[2], wz[2], mat_of(mz_[2]), pz[3], wz[3], mat_of(mz_[3]));
         cqa = limited_cq<M, Cfg>(a, f1, f0, f2, dz_);
-        Fa = {f1.w1, f1.w2};
-        Fb = {f2.w1, f2.w2};
+        Fa = {f1.w1.n, f1.w2.n};
+        Fb = {f2.w1.n, f2.w2.n};
         pz1 = pz[3];
         wz1 = wz[3];
     }
@@ -268,17 +285,20 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
     };
 
     for (int k = kb - 1; k <= ke + 1; k++) {
-        const Mat me = mat_of(R[1][1]);
+        const Mat me = mat_of(R(1, 1));
         // one material over the whole wave's 3x3 neighbourhoods: rows j-1..j+1
         // and planes k-1..k+1 (x), row j (y), plane k (z)
-        const int m0 = __builtin_amdgcn_readfirstlane(R[1][1]);
-        bool sy = true, sz = true, sx;
-#pragma unroll
-        for (int e = 0; e < 3; e++) {
-            sy = sy && R[1][e] == m0;
-            sz = sz && R[e][1] == m0;
-        }
-        sx = sy && sz && R[0][0] == m0 && R[0][2] == m0 && R[2][0] == m0 && R[2][2] == m0;
+        const int m0 = __builtin_amdgcn_readfirstlane(R(1, 1));
+        // one material over the x-, y- and z-neighbourhoods: planes k-1..k+1
+        // (bytes 0..2) of each row, or plane k of each row, equal to m0
+        const unsigned m4 = (unsigned)m0 * 0x01010101u;
+        const bool r0 = ((Rp[0] ^ m4) & 0x00ffffffu) == 0u;
+        const bool r1 = ((Rp[1] ^ m4) & 0x00ffffffu) == 0u;
+        const bool r2 = ((Rp[2] ^ m4) & 0x00ffffffu) == 0u;
+        const bool sy = r1;
+        const bool sz = ((Rp[0] ^ m4) & 0xff00u) == 0u && ((Rp[1] ^ m4) & 0xff00u) == 0u &&
+                        ((Rp[2] ^ m4) & 0xff00u) == 0u;  // plane k (byte 1) of each row
+        const bool sx = r0 && r1 && r2;
         const bool ux = __all(sx), uy = __all(sy), uz = __all(sz);
         Kc K0{};
         if (ux || uy || uz) K0 = hg_of(m0);
@@ -289,7 +309,7 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
 #pragma unroll
             for (int s = 0; s < 3; s++)
 #pragma unroll
-                for (int p = 0; p < 3; p++) N[s][p] = mat_of(R[s][p]);
+                for (int p = 0; p < 3; p++) N[s][p] = mat_of(R(s, p));
             const Iface f = solve<M>(lane_up(p0), lane_up(u0), mat_up(me), p0, u0, me);
             Iface fl, fr;
             fl.w1 = wv_up(f.w1);
@@ -335,15 +355,15 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
 
         // ---- z-slice position (j, k): t1 = x (lanes), t2 = y (rows)
         if (k >= kb && k <= ke) {
-            const Mat mk1 = mat_of(R[1][2]), mk2 = mat_of(R[1][3]);
+            const Mat mk1 = mat_of(R(1, 2)), mk2 = mat_of(R(1, 3));
             const Iface fc = solve<M>(pz1, wz1, mk1, pz2, wz2, mk2);  // interface k+3/2
-            const Iface fa = from_raw(Fa, mat_of(R[1][0]), me);        // k-1/2
-            const Iface fb = from_raw(Fb, me, mk1);                     // k+1/2
+            const Iface fa = from_alpha(Fa, mat_of(R(1, 0)), me);      // k-1/2
+            const Iface fb = from_alpha(Fb, me, mk1);                   // k+1/2
             const Wv cqb = limited_cq<M, Cfg>(a, fb, fa, fc, dz_);
             Mat N[3][3];
 #pragma unroll
             for (int s = 0; s < 3; s++) {
-                const Mat c = mat_of(R[s][1]);
+                const Mat c = mat_of(R(s, 1));
                 N[1][s] = c;
                 N[0][s] = mat_up(c);
                 N[2][s] = mat_dn(c);
@@ -372,7 +392,7 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
                 acc_z(T);
             }
             Fa = Fb;
-            Fb = {fc.w1, fc.w2};
+            Fb = {fc.w1.n, fc.w2.n};
             cqa = cqb;
             pz1 = pz2;
             wz1 = wz2;
@@ -383,13 +403,13 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
 
         // ---- y interface j+1/2 (and j-1/2 for wave 0) at plane k
         {
-            const Iface fy = solve<M>(p0, v0, me, pN, vN, mat_of(R[2][1]));
+            const Iface fy = solve<M>(p0, v0, me, pN, vN, mat_of(R(2, 1)));
             ybw[0] = fy.w1.p;
             ybw[64] = fy.w1.n;
             ybw[128] = fy.w2.p;
             ybw[192] = fy.w2.n;
             if (r == 0) {
-                const Iface fs_ = solve<M>(pS, vS, mat_of(R[0][1]), p0, v0, me);
+                const Iface fs_ = solve<M>(pS, vS, mat_of(R(0, 1)), p0, v0, me);
                 double* yb0 = ybuf + l;
                 yb0[0] = fs_.w1.p;
                 yb0[64] = fs_.w1.n;
@@ -414,14 +434,14 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
                 return {{y[0], y[64]}, {y[128], y[192]}};
             };
             const Raw y0 = raw_at(r - 1), y1 = raw_at(r), y2 = raw_at(r + 1), y3 = raw_at(r + 2);
-            const Iface flo = from_raw(y1, mat_of(R[0][1]), me);  // j-1/2
-            const Iface fhi = from_raw(y2, me, mat_of(R[2][1]));  // j+1/2
+            const Iface flo = from_raw(y1, mat_of(R(0, 1)), me);  // j-1/2
+            const Iface fhi = from_raw(y2, me, mat_of(R(2, 1)));  // j+1/2
             const Wv cqlo = limited_cq<M, Cfg>(a, flo, waves_only(y0), waves_only(y2), dy_);
             const Wv cqhi = limited_cq<M, Cfg>(a, fhi, waves_only(y1), waves_only(y3), dy_);
             Mat N[3][3];
 #pragma unroll
             for (int s = 0; s < 3; s++) {
-                const Mat c = mat_of(R[1][s]);
+                const Mat c = mat_of(R(1, s));
                 N[s][1] = c;
                 N[s][0] = mat_up(c);
                 N[s][2] = mat_dn(c);
@@ -469,12 +489,7 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
             A[2][m] = 0.0;
         }
 #pragma unroll
-        for (int s = 0; s < 3; s++) {
-            R[s][0] = R[s][1];
-            R[s][1] = R[s][2];
-            R[s][2] = R[s][3];
-            R[s][3] = Rn[s];
-        }
+        for (int s = 0; s < 3; s++) Rp[s] = (Rp[s] >> 8) | ((unsigned)Rn[s] << 24);
         p0 = np0; u0 = nu0; v0 = nv0; pN = npN; vN = nvN; pS = npS; vS = nvS;
         pz2 = npz2; wz2 = nwz2;
         __syncthreads();
