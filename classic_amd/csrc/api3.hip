@@ -71,12 +71,43 @@ struct Bc3 {
 // negates its normal velocity.  Code 0 on z-hi is the piston wall of the
 // heterogeneous-acoustics example (examples/acoustics_3d_heterogeneous/bc3.f
 // labels 600-606): w = 2 s(x, y, t) - w(mirror).
+// One thread per ghost cell (0-based padded indices ii, jj, kk): the ghost
+// planes of the z faces (whole planes), then the ghost rows of the y faces
+// in the interior planes, then the ghost cells of the x faces in the
+// interior rows, each run along its contiguous index.  Every source cell is
+// an interior cell, so the order of the writes does not matter.
+__host__ __device__ inline long bc3_ghosts(const Layout3& L)
+{
+    const long m2 = 2L * L.mbc;
+    return m2 * L.nx * L.ny + m2 * L.nx * L.mz + m2 * L.my * L.mz;
+}
+
 __global__ void bc3_kernel(double* __restrict__ q, Layout3 L, Bc3 b)
 {
-    const long n = (long)L.nx * L.ny * L.nz;
+    const int mb = L.mbc;
+    const long nzf = 2L * mb * L.nx * L.ny, nyf = 2L * mb * L.nx * L.mz;
     long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const int ii = (int)(e % L.nx), jj = (int)((e / L.nx) % L.ny), kk = (int)(e / ((long)L.nx * L.ny));
+    int ii, jj, kk;
+    if (e < nzf) {
+        ii = (int)(e % L.nx);
+        jj = (int)((e / L.nx) % L.ny);
+        const int r = (int)(e / ((long)L.nx * L.ny));  // 0 .. 2mbc-1
+        kk = r < mb ? r : L.nz - 2 * mb + r;
+    } else if ((e -= nzf) < nyf) {
+        ii = (int)(e % L.nx);
+        const long t = e / L.nx;
+        const int r = (int)(t % (2 * mb));
+        kk = mb + (int)(t / (2 * mb));
+        jj = r < mb ? r : L.ny - 2 * mb + r;
+    } else if ((e -= nyf) < 2L * mb * L.my * L.mz) {
+        const int r = (int)(e % (2 * mb));
+        const long t = e / (2 * mb);
+        jj = mb + (int)(t % L.my);
+        kk = mb + (int)(t / L.my);
+        ii = r < mb ? r : L.nx - 2 * mb + r;
+    } else {
+        return;
+    }
     const int g[3] = {ii + 1 - L.mbc, jj + 1 - L.mbc, kk + 1 - L.mbc};
     const int nn[3] = {L.mx, L.my, L.mz};
     int s[3];
@@ -95,7 +126,7 @@ __global__ void bc3_kernel(double* __restrict__ q, Layout3 L, Bc3 b)
             else piston = true;  // code 0, z-hi only (checked on the host)
         }
     }
-    if (!ghost) return;
+    if (!ghost) return;  // not reached: every enumerated cell is a ghost cell
     const long src = L.off(s[0], s[1], s[2]);
     double v[4];
     for (int m = 0; m < 4; m++) v[m] = q[m * L.fs + src];
@@ -111,7 +142,8 @@ __global__ void bc3_kernel(double* __restrict__ q, Layout3 L, Bc3 b)
         }
         v[3] = 2.0 * sp - v[3];
     }
-    for (int m = 0; m < 4; m++) q[m * L.fs + e] = v[m];
+    const long dst = L.off(g[0], g[1], g[2]);
+    for (int m = 0; m < 4; m++) q[m * L.fs + dst] = v[m];
 }
 
 // max |c| over cells with all indices in [0, n+1] (the cells adjacent to
@@ -217,7 +249,7 @@ int enqueue_bc3(clawgpu3_ctx* c, double* q, const double* geom, double t, const 
         b.exy = c->exy;
         b.et = std::exp(-(500.0 * (t * t)));
     }
-    const long n = (long)c->L.nx * c->L.ny * c->L.nz;
+    const long n = bc3_ghosts(c->L);
     hipLaunchKernelGGL(bc3_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, q,
                        c->L, b);
     HIPCHK(c, hipGetLastError());
