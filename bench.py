@@ -484,7 +484,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="accepted steps to time (default 200 for euler2d, 100 for acoustics3d)")
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed warmup steps (default 100 for euler2d/acoustics2d, run as a "
+                         "claw2 run on a second grid right before the timed run; 5 otherwise)")
     ap.add_argument("--n", type=int, default=None,
                     help="grid side (default 4096; 1024 for learned2d)")
     ap.add_argument("--seed", type=int, default=0)
@@ -508,6 +510,8 @@ def main():
 
     if args.steps is None:
         args.steps = 100 if args.workload == "acoustics3d" else 200
+    if args.warmup is None:
+        args.warmup = 100 if args.workload in ("euler2d", "acoustics2d") else 5
     if args.workload == "acoustics3d":
         print(json.dumps(bench3(args)), flush=True)
         return
@@ -545,14 +549,24 @@ def main():
         data = "synthetic (seeded four-quadrant Euler initial data)"
     g.set_segment_rows(args.seg_rows)
     g.set_q(q0)
-    # warmup: untimed fixed-dt steps on a copy of the initial data
+    # warmup: an untimed claw2 run of the same problem on a second grid,
+    # started after q0 is resident in the timed grid, so that the timed run
+    # follows it with no idle gap (this FP64 load takes the GPU about 50 ms
+    # of continuous work to reach its steady clock; after an idle gap the
+    # first steps run up to 30 % slower)
+    gw = None
     if args.warmup > 0:
-        g.time_steps(dx, dy, 0.25 * dx, method, mthlim, mthbc, args.warmup)
-    g.set_q(q0)
+        gw = gpu.Grid2D(SOLVER_ACOUSTICS if acoustics else SOLVER_EULER, n, n,
+                        [1.0, 4.0] if acoustics else [1.4, 1.0], arith=arith)
+        gw.set_segment_rows(args.seg_rows)
+        gw.set_q(q0)
+        gw.claw2(dx, dy, 0.0, 1e30, [dt0, 1e99], [1.0, 0.9], args.warmup, method, mthlim, mthbc)
 
     t0 = time.perf_counter()
     out = g.claw2(dx, dy, 0.0, 1e30, [dt0, 1e99], [1.0, 0.9], args.steps, method, mthlim, mthbc)
     wall = time.perf_counter() - t0
+    if gw is not None:
+        gw.close()
     steps = int(out["nv"][1])
     if steps != args.steps:
         raise SystemExit(f"claw2 took {steps} steps, expected {args.steps} (info={out['info']})")
