@@ -128,6 +128,7 @@ def cpu_baseline(mx, my, seed, nsteps=None, solver="euler"):
 
 
 WORKLOAD3 = "acoustics3d_512cubed_vc_layered_superbee_method22_walls_100steps"
+WORKLOAD3_SPLIT = "acoustics3d_512cubed_vc_layered_superbee_dimsplit_walls_100steps"
 BYTES_PER_CELL_UPDATE3 = 80  # (2*meqn + maux) * 8 with meqn = 4, maux = 2
 
 
@@ -173,7 +174,7 @@ def bench3(args):
 
     n = args.n3
     q0, aux, _ = P.acoustics3d_pulse(n, seed=args.seed)
-    method, mthlim, mthbc = (1, 2, 22, 0, 0, 0, 2), (2, 2), (3, 3, 3, 3, 3, 3)
+    method, mthlim, mthbc = (1, 2, args.m3, 0, 0, 0, 2), (2, 2), (3, 3, 3, 3, 3, 3)
     g = Grid3D(n, n, n)
     if args.tile >= 0 or args.tile_order >= 0:
         tile, order = g.get_tile()
@@ -222,7 +223,8 @@ def bench3(args):
         "steps": steps, "warmup": args.warmup, "ms_per_step": wall / steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded Gaussian pulse in a two-layer medium)",
-        "config": {"workload": WORKLOAD3 if n == 512 else f"acoustics3d_{n}cubed", "mx": n,
+        "config": {"workload": (WORKLOAD3 if args.m3 == 22 else WORKLOAD3_SPLIT) if n == 512
+                   else f"acoustics3d_{n}cubed_method{args.m3}", "mx": n,
                    "my": n, "mz": n, "seed": args.seed, "method": list(method),
                    "mthlim": list(mthlim), "mthbc": list(mthbc), "cfl_desired": 0.9,
                    "cfl_max": 1.0},
@@ -232,7 +234,8 @@ def bench3(args):
         "step_overhead_ms_avg": (total_ms - kern_ms) / ksteps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profile_traffic("acoustics3d512") if n == 512 else None},
+                     "traffic": (profile_traffic("acoustics3d512")
+                                 if n == 512 and args.m3 == 22 else None)},
         "measured_ceilings": ceil,
         "frac_of_measured_copy": achieved / ceil["copy_GB_s"],
         "cpu_baseline": cpu,
@@ -487,6 +490,9 @@ def main():
     ap.add_argument("--tile", type=int, default=-1,
                     help="acoustics3d: tile shape of the unsplit kernel (0..4 tile kernels, "
                          "5, 6 z-marching kernel; -1: the library's default)")
+    ap.add_argument("--m3", type=int, default=22, choices=[22, -1],
+                    help="acoustics3d: method(3), 22 (unsplit, both corrections; the headline) "
+                         "or -1 (dimensional splitting, dimsp3)")
     ap.add_argument("--tile-order", type=int, default=-1,
                     help="acoustics3d: tile order of the unsplit kernel (bit 0 XCD-contiguous, "
                          "bit 1 z fastest; -1: the library's default)")

@@ -352,13 +352,20 @@ int courant3(clawgpu3_ctx* c, const int* method, double dx, double dy, double dz
     return CLAWGPU_OK;
 }
 
-int enqueue_sweep3(clawgpu3_ctx* c, int dir, const double* q, double* qn, double d,
+// One sweep reading field m from buffer from[m] and writing it to buffer
+// to[m] (-1: not written; see SweepArgs3).
+int enqueue_sweep3(clawgpu3_ctx* c, int dir, const int from[4], const int to[4], double d,
                    const int* method, const int* mthlim, int use_fwave)
 {
     SweepArgs3 a;
-    a.q = q;
-    a.qn = qn;
+    for (int m = 0; m < 4; m++) {
+        a.qf[m] = c->dq[from[m]] + (size_t)m * c->L.fs;
+        a.qo[m] = to[m] >= 0 ? c->dq[to[m]] + (size_t)m * c->L.fs : nullptr;
+    }
     a.aux = c->daux;
+    const bool table = c->use_table && c->nmat > 0;
+    a.mat = table ? c->dmat : nullptr;
+    a.lut = table ? c->dlut : nullptr;
     a.capa = capa_plane(c, method);
     a.L = c->L;
     a.afs = c->L.fs;
@@ -376,9 +383,15 @@ int enqueue_sweep3(clawgpu3_ctx* c, int dir, const double* q, double* qn, double
     return CLAWGPU_OK;
 }
 
-// dimsp3.f:5-85 on the device: x, y, z sweeps out of place, cur -> o1 -> o2
-// -> o1.  The Courant numbers are known before the sweeps (dt/dx * max|c|),
-// so the early returns of dimsp3 become a count of sweeps to run.
+// dimsp3.f:5-85 on the device: x, y, z sweeps out of place.  The Courant
+// numbers are known before the sweeps (dt/dx * max|c|), so the early returns
+// of dimsp3 become a count of sweeps to run.  A full step (buffers A = cur,
+// B, C) moves each field only where a sweep changes it: x writes (p, u) to
+// B; y reads p from B and v from A and writes (p, v) to C; z reads p and v
+// from C and w from A and writes (p, w) to B with v copied there, so B holds
+// the new state (u from the x sweep).  A step that stops after its first or
+// second sweep writes every field in each sweep, so its last buffer is
+// complete.
 int enqueue_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz, double dt, const int* method,
                    const int* mthlim, const double* cflv, int use_fwave, int* res, double* cfl)
 {
@@ -386,18 +399,28 @@ int enqueue_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz, double dt, 
     int rc0 = courant3(c, method, dx, dy, dz, dt, cf);
     if (rc0) return rc0;
     const double cx = cf[0], cy = cf[1], cz = cf[2];
-    const int o1 = (c->cur + 1) % 3, o2 = (c->cur + 2) % 3;
-    int rc = enqueue_sweep3(c, 0, c->dq[c->cur], c->dq[o1], dt / dx, method, mthlim, use_fwave);
+    const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
+    if (cx > cflv[0] || cy > cflv[0]) {
+        const int fa[4] = {A, A, A, A}, tb[4] = {B, B, B, B};
+        int rc = enqueue_sweep3(c, 0, fa, tb, dt / dx, method, mthlim, use_fwave);
+        if (rc) return rc;
+        *res = B;
+        if (cx > cflv[0]) { *cfl = cx; return CLAWGPU_OK; }
+        const int tc[4] = {C, C, C, C};
+        rc = enqueue_sweep3(c, 1, tb, tc, dt / dy, method, mthlim, use_fwave);
+        if (rc) return rc;
+        *res = C;
+        *cfl = cy;
+        return CLAWGPU_OK;
+    }
+    const int fx[4] = {A, A, A, A}, tx[4] = {B, B, -1, -1};
+    const int fy[4] = {B, B, A, A}, ty[4] = {C, -1, C, -1};
+    const int fz[4] = {C, B, C, A}, tz[4] = {B, -1, B, B};
+    int rc = enqueue_sweep3(c, 0, fx, tx, dt / dx, method, mthlim, use_fwave);
+    if (!rc) rc = enqueue_sweep3(c, 1, fy, ty, dt / dy, method, mthlim, use_fwave);
+    if (!rc) rc = enqueue_sweep3(c, 2, fz, tz, dt / dz, method, mthlim, use_fwave);
     if (rc) return rc;
-    *res = o1;
-    if (cx > cflv[0]) { *cfl = cx; return CLAWGPU_OK; }
-    rc = enqueue_sweep3(c, 1, c->dq[o1], c->dq[o2], dt / dy, method, mthlim, use_fwave);
-    if (rc) return rc;
-    *res = o2;
-    if (cy > cflv[0]) { *cfl = cy; return CLAWGPU_OK; }
-    rc = enqueue_sweep3(c, 2, c->dq[o2], c->dq[o1], dt / dz, method, mthlim, use_fwave);
-    if (rc) return rc;
-    *res = o1;
+    *res = B;
     *cfl = fmax(cx, fmax(cy, cz));
     return CLAWGPU_OK;
 }
@@ -819,13 +842,14 @@ extern "C" int clawgpu3_time_steps(clawgpu3_ctx* c, const double* geom, double d
                                    int use_fwave, int nsteps, float* total_ms,
                                    float* step_kernel_ms, double* cfl_out)
 {
-    // Fixed-dt unsplit stepping timed with HIP events on the library's
-    // stream, no host synchronisation between steps: total_ms spans all
-    // launches (ghost fill + step kernel per step), step_kernel_ms sums the
-    // event pairs around each step-kernel launch.  Used by the benchmark.
+    // Fixed-dt stepping (unsplit, or dimensionally split with method(3) =
+    // -1, all three sweeps) timed with HIP events on the library's stream,
+    // no host synchronisation between steps: total_ms spans all launches
+    // (ghost fill + step kernels per step), step_kernel_ms sums the event
+    // pairs around each step's kernels.  Used by the benchmark.
     if (!c || !geom || !method || !mthlim || !mthbc || !total_ms || nsteps < 1)
         return set_err(CLAWGPU_ERR_ARG, "bad argument");
-    if (method[2] < 0) return set_err(CLAWGPU_ERR_ARG, "time_steps runs the unsplit step");
+    if (method[2] < -1) return set_err(CLAWGPU_ERR_ARG, "method(3) must be -1 or >= 0");
     int rc = check_args3(c, method, mthlim);
     if (rc) return rc;
     if (check_bc3(mthbc) != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "bad mthbc");
@@ -838,8 +862,14 @@ extern "C" int clawgpu3_time_steps(clawgpu3_ctx* c, const double* geom, double d
         rc = enqueue_bc3(c, c->dq[c->cur], geom, n * dt, mthbc);
         if (rc) return rc;
         HIPCHK(c, hipEventRecord(ev[2 + 2 * n], c->stream));
-        const int res = (c->cur + 1) % 3;
-        rc = enqueue_step3(c, res, dx, dy, dz, dt, method, mthlim, use_fwave);
+        int res = (c->cur + 1) % 3;
+        if (method[2] == -1) {
+            const double all[2] = {1e300, 1e300};  // every sweep runs
+            double cf;
+            rc = enqueue_dimsp3(c, dx, dy, dz, dt, method, mthlim, all, use_fwave, &res, &cf);
+        } else {
+            rc = enqueue_step3(c, res, dx, dy, dz, dt, method, mthlim, use_fwave);
+        }
         if (rc) return rc;
         HIPCHK(c, hipEventRecord(ev[3 + 2 * n], c->stream));
         c->cur = res;

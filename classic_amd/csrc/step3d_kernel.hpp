@@ -659,10 +659,18 @@ __global__ __launch_bounds__(64) void step3_tile_kernel(Args3 a)
 
 // ------------------------------------------------------------ split sweeps
 
+// A sweep reads field m from plane qf[m] and writes it to plane qo[m]: the
+// pressure and its own velocity component are updated, another field is
+// copied when qo[m] is set and otherwise not touched (a split step then
+// passes that field's plane on to the next sweep).  Materials come from the
+// byte index mat and the table lut (the same doubles as aux) when mat is
+// set.
 struct SweepArgs3 {
-    const double* __restrict__ q;
-    double* __restrict__ qn;
+    const double* qf[4];
+    double* qo[4];
     const double* __restrict__ aux;
+    const unsigned char* __restrict__ mat;
+    const double* __restrict__ lut;
     const double* __restrict__ capa;  // capacity plane (method(6) > 0), or nullptr
     Layout3 L;
     long afs;
@@ -672,6 +680,15 @@ struct SweepArgs3 {
     int seg;      // y/z: cells per segment along the sweep
     int nsegs;
 };
+
+__device__ inline Mat sweep_mat(const SweepArgs3& a, long c)
+{
+    if (a.mat) {
+        const int id = a.mat[c];
+        return {a.lut[2 * id], a.lut[2 * id + 1], id};
+    }
+    return {a.aux[c], a.aux[a.afs + c], 0};
+}
 
 // update of one slice cell: (q + qadd) - d*(F(hi) - F(lo)), step3ds.f90;
 // with capacity kappa the flux difference term is divided by it
@@ -705,10 +722,9 @@ __global__ __launch_bounds__(256) void sweep3_x_kernel(SweepArgs3 a)
     const int j = (int)(sl % nyt) + 1 - L.mbc;
     const int k = (int)(sl / nyt) + 1 - L.mbc;
     const int i = kOut * strip + l - 1;
-    const long fs = L.fs;
     const long c = L.offc(i, j, k);
-    const double p = a.q[c], u = a.q[fs + c];
-    const Mat mm = {a.aux[c], a.aux[a.afs + c]};
+    const double p = a.qf[0][c], u = a.qf[1][c];
+    const Mat mm = sweep_mat(a, c);
     Iface f = solve<M>(lane_up(p), lane_up(u), mat_up(mm), p, u, mm);
     const double kap = a.capa ? a.capa[c] : 1.0;
     const double dc = a.capa ? a.d / kap : a.d;  // dtdx1d of this cell
@@ -727,19 +743,19 @@ __global__ __launch_bounds__(256) void sweep3_x_kernel(SweepArgs3 a)
     if (upd && a.capa) {
         const double qp = (0.0 - dc * f.ap.p) - dc * amhi.p;
         const double qu = (0.0 - dc * f.ap.n) - dc * amhi.n;
-        a.qn[o] = ds_update_capa(p, qp, cqhi.p, cq.p, a.d, kap);
-        a.qn[fs + o] = ds_update_capa(u, qu, cqhi.n, cq.n, a.d, kap);
+        a.qo[0][o] = ds_update_capa(p, qp, cqhi.p, cq.p, a.d, kap);
+        a.qo[1][o] = ds_update_capa(u, qu, cqhi.n, cq.n, a.d, kap);
     } else if (upd) {
         const double qp = (0.0 - a.d * f.ap.p) - a.d * amhi.p;
         const double qu = (0.0 - a.d * f.ap.n) - a.d * amhi.n;
-        a.qn[o] = ds_update(p, qp, cqhi.p, cq.p, a.d);
-        a.qn[fs + o] = ds_update(u, qu, cqhi.n, cq.n, a.d);
+        a.qo[0][o] = ds_update(p, qp, cqhi.p, cq.p, a.d);
+        a.qo[1][o] = ds_update(u, qu, cqhi.n, cq.n, a.d);
     } else {
-        a.qn[o] = p;
-        a.qn[fs + o] = u;
+        a.qo[0][o] = p;
+        a.qo[1][o] = u;
     }
-    a.qn[2 * fs + o] = a.q[2 * fs + o];
-    a.qn[3 * fs + o] = a.q[3 * fs + o];
+    if (a.qo[2]) a.qo[2][o] = a.qf[2][o];
+    if (a.qo[3]) a.qo[3][o] = a.qf[3][o];
 }
 
 // y (dir = 1) or z (dir = 2) sweep of step3ds: lanes hold 64 consecutive
@@ -762,7 +778,6 @@ __global__ __launch_bounds__(256) void sweep3_yz_kernel(SweepArgs3 a, int dir)
     if (i > L.mx + L.mbc) return;
     const int n = dir == 1 ? L.my : L.mz;
     const int n2 = dir == 1 ? L.mz : L.my;
-    const long fs = L.fs;
     const int comp = dir == 1 ? 2 : 3;
     const bool slice_ok = i >= 0 && i <= L.mx + 1 && o2 >= 0 && o2 <= n2 + 1;
     const int s0 = seg == 0 ? 1 - L.mbc : 1 + seg * a.seg;
@@ -774,9 +789,9 @@ __global__ __launch_bounds__(256) void sweep3_yz_kernel(SweepArgs3 a, int dir)
     // cell s: p, normal velocity, material and (with capacity) dt/d / kappa
     auto cell = [&](int s, double& pv, double& nv, Mat& mv, double& dv) {
         const long c = idx(s);
-        pv = a.q[c];
-        nv = a.q[comp * fs + c];
-        mv = {a.aux[c], a.aux[a.afs + c]};
+        pv = a.qf[0][c];
+        nv = a.qf[comp][c];
+        mv = sweep_mat(a, c);
         dv = a.capa ? a.d / a.capa[c] : a.d;
     };
     // limited cqxx of interface f between cells with ratios dl, dr
@@ -807,20 +822,20 @@ __global__ __launch_bounds__(256) void sweep3_yz_kernel(SweepArgs3 a, int dir)
             const double kap = a.capa[o];
             const double qp = (0.0 - d0 * fcur.ap.p) - d0 * fnext.am.p;
             const double qv = (0.0 - d0 * fcur.ap.n) - d0 * fnext.am.n;
-            a.qn[o] = ds_update_capa(p0, qp, cqnext.p, cqcur.p, a.d, kap);
-            a.qn[comp * fs + o] = ds_update_capa(n0, qv, cqnext.n, cqcur.n, a.d, kap);
+            a.qo[0][o] = ds_update_capa(p0, qp, cqnext.p, cqcur.p, a.d, kap);
+            a.qo[comp][o] = ds_update_capa(n0, qv, cqnext.n, cqcur.n, a.d, kap);
         } else if (slice_ok && s >= 1 && s <= n) {
             const double qp = (0.0 - a.d * fcur.ap.p) - a.d * fnext.am.p;
             const double qv = (0.0 - a.d * fcur.ap.n) - a.d * fnext.am.n;
-            a.qn[o] = ds_update(p0, qp, cqnext.p, cqcur.p, a.d);
-            a.qn[comp * fs + o] = ds_update(n0, qv, cqnext.n, cqcur.n, a.d);
+            a.qo[0][o] = ds_update(p0, qp, cqnext.p, cqcur.p, a.d);
+            a.qo[comp][o] = ds_update(n0, qv, cqnext.n, cqcur.n, a.d);
         } else {
-            a.qn[o] = p0;
-            a.qn[comp * fs + o] = n0;
+            a.qo[0][o] = p0;
+            a.qo[comp][o] = n0;
         }
 #pragma unroll
         for (int m = 1; m < 4; m++)
-            if (m != comp) a.qn[m * fs + o] = a.q[m * fs + o];
+            if (m != comp && a.qo[m]) a.qo[m][o] = a.qf[m][o];
         fcur = fnext;
         fnext = fnn;
         cqcur = cqnext;
