@@ -38,10 +38,9 @@ struct clawgpu_batch {
     batch::Prob* dprob = nullptr;
     double* pool = nullptr;
     double* staging = nullptr;
-    long* ghost0 = nullptr;
     int* ndone = nullptr;
     int* hndone = nullptr;  // pinned
-    long nghost = 0;
+    int walls[batch::kGroups] = {0, 0};  // some problem of the group has a wall side
     int ncu = 256;
     hipStream_t stream = nullptr;
     Schedule sched;
@@ -72,16 +71,6 @@ __global__ void prepare_kernel(Params b)
         p.told = p.t;
         if (p.told + p.dt > p.tend && p.tstart < p.tend) p.dt = p.tend - p.told;
     }
-}
-
-__global__ void bc_kernel(Params b)
-{
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.nghost) return;
-    const int k = find_range(b.ghost0, b.nprob, t);
-    const Prob& p = b.prob[k];
-    if (p.done) return;
-    bc2_cell(p.q[p.cur], p.L, p.meqn, p.mthbc, t - b.ghost0[k]);
 }
 
 // The tiles of group G are dealt to the waves of the grid in turn (wave w
@@ -222,8 +211,7 @@ static batch::Params params_of(const clawgpu_batch* b, int dry)
         a.ntiles[g] = b->plan.ntiles[g];
         off += b->plan.nrank[g];
     }
-    a.ghost0 = b->ghost0;
-    a.nghost = b->nghost;
+    for (int g = 0; g < batch::kGroups; g++) a.walls[g] = b->walls[g];
     a.ndone = b->ndone;
     a.seglen = b->sched.seglen;
     a.dry = dry;
@@ -237,8 +225,6 @@ static hipError_t enqueue_iteration(clawgpu_batch* b, int dry, hipEvent_t* ev = 
     const batch::Params a = params_of(b, dry);
     const int pb = (b->nprob + 127) / 128;
     hipLaunchKernelGGL(batch::prepare_kernel, dim3(pb), dim3(128), 0, b->stream, a);
-    hipLaunchKernelGGL(batch::bc_kernel, dim3((unsigned)((b->nghost + 255) / 256)), dim3(256), 0,
-                       b->stream, a);
     if (ev) (void)hipEventRecord(ev[0], b->stream);
     const int nblk = b->sched.nblocks > 0 ? b->sched.nblocks : 2 * b->ncu;
     const bool w3 = b->sched.w3 != 0;
@@ -272,7 +258,6 @@ extern "C" int clawgpu_batch_destroy(clawgpu_batch* b)
     if (b->dprob) (void)hipFree(b->dprob);
     if (b->pool) (void)hipFree(b->pool);
     if (b->staging) (void)hipFree(b->staging);
-    if (b->ghost0) (void)hipFree(b->ghost0);
     if (b->ndone) (void)hipFree(b->ndone);
     if (b->hndone) (void)hipHostFree(b->hndone);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -295,8 +280,6 @@ extern "C" int clawgpu_batch_create(clawgpu_batch** out, int nprob, int mbc, con
     b->mbc = mbc;
     b->hp.resize(nprob);
     size_t pool = 0, stage = 0;
-    std::vector<long> g0(nprob);
-    long ghosts = 0;
     for (int k = 0; k < nprob; k++) {
         batch::Prob& p = b->hp[k];
         p = batch::Prob{};
@@ -338,10 +321,9 @@ extern "C" int clawgpu_batch_create(clawgpu_batch** out, int nprob, int mbc, con
         p.done = 1;  // until clawgpu_batch_start
         pool += 3 * (size_t)p.L.fs * p.meqn;
         stage = std::max(stage, (size_t)(mx[k] + 2 * mbc) * (my[k] + 2 * mbc) * p.meqn);
-        g0[k] = ghosts;
-        ghosts += 2L * mbc * (mx[k] + 2 * mbc) + 2L * mbc * my[k];
+        for (int s = 0; s < 4; s++)
+            if (p.mthbc[s] == 3) b->walls[batch::group_of(p.solver)] = 1;
     }
-    b->nghost = ghosts;
     int dev = 0;
     hipDeviceProp_t prop;
     hipError_t e = hipGetDevice(&dev);
@@ -351,7 +333,6 @@ extern "C" int clawgpu_batch_create(clawgpu_batch** out, int nprob, int mbc, con
     if (e == hipSuccess) e = hipMalloc((void**)&b->pool, pool * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&b->staging, stage * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&b->dprob, nprob * sizeof(batch::Prob));
-    if (e == hipSuccess) e = hipMalloc((void**)&b->ghost0, nprob * sizeof(long));
     if (e == hipSuccess) e = hipMalloc((void**)&b->ndone, sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void**)&b->hndone, sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -369,7 +350,6 @@ extern "C" int clawgpu_batch_create(clawgpu_batch** out, int nprob, int mbc, con
     }
     e = hipMemsetAsync(b->pool, 0, pool * sizeof(double), b->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b->ndone, 0, sizeof(int), b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(b->ghost0, g0.data(), nprob * sizeof(long), hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) {
         snprintf(g_errbuf, sizeof g_errbuf, "init failed: %s", hipGetErrorString(e));

@@ -7,12 +7,12 @@
 //   prepare  per problem: at the start of a step (not a retake) clamp dt so
 //            the step ends at tend (claw2.f:601), pick the output buffer,
 //            clear the Courant accumulator;
-//   bc       one thread per ghost cell of every unfinished problem (bc2);
 //   step     a persistent kernel per solver: the (problem, 60-column strip,
 //            row segment) tiles of all problems are dealt to its wavefronts
 //            in turn and each runs the fused step of its tiles (step2_wave,
 //            the code of the single-grid kernel), so grids of any size share
-//            the GPU without a launch per grid;
+//            the GPU without a launch per grid; ghost cells are read through
+//            each problem's bc2 index maps, so no ghost-fill pass runs;
 //   accept   per problem: accept iff cfl <= cflv(1) (claw2.f:695), else keep
 //            the old buffer; with variable dt the next dt is
 //            min(dtv(2), dt*cflv(2)/cfl) after either outcome
@@ -22,7 +22,7 @@
 // and each problem goes through exactly the dt, Courant numbers and states
 // of a single-grid claw2 run of it.
 //
-// A "dry" iteration (autotuning) runs bc and step but changes no state: the
+// A "dry" iteration (autotuning) runs the step but changes no state: the
 // step writes the spare buffer and prepare/accept leave t, dt and the
 // buffer choice alone.
 #pragma once
@@ -67,8 +67,7 @@ struct Params {
     const long* rank_tile0[kGroups];  // first tile of rank r (ascending in r)
     int nrank[kGroups];               // problems in the group
     long ntiles[kGroups];             // tiles of the group
-    const long* ghost0;               // first ghost cell of problem k (ascending in k)
-    long nghost;
+    int walls[kGroups];               // some problem of the group has a wall side
     int* ndone;  // finished problems
     int order, trans, fwave, seglen;
     int lim[kMaxWaves];
@@ -88,7 +87,10 @@ __device__ inline int find_range(const long* start, int n, long t)
     return lo;
 }
 
-template <class S, class Cfg, int G>
+// The tiles of group G; ghost cells are read through each problem's
+// boundary maps (BC = 1: extrapolation and periodic sides; BC = 2: walls as
+// well), so no ghost-fill pass runs between iterations.
+template <class S, class Cfg, int G, int BC>
 __device__ __forceinline__ void step_loop(const Params& b)
 {
     const int lane = threadIdx.x & 63;
@@ -117,31 +119,31 @@ __device__ __forceinline__ void step_loop(const Params& b)
         a.cfl_bits = &b.prob[k].cfl_bits;
         a.P = p.P;
         a.phix = a.phiy = nullptr;
-        a.bcmap = 0;  // the batch fills ghost cells in its own pass
-        for (int e = 0; e < 4; e++) a.mthbc[e] = 0;
-        step2_wave<S, Cfg>(a, (int)(local % p.nstrips), (int)(local / p.nstrips), lane);
+        a.bcmap = 1;
+        for (int e = 0; e < 4; e++) a.mthbc[e] = p.mthbc[e];
+        step2_wave<S, Cfg, BC>(a, (int)(local % p.nstrips), (int)(local / p.nstrips), lane);
     }
 }
 
 // The compiler's register budget; step_kernel_w2 holds the Euler kernel
 // to two waves per SIMD (256 registers) as the single-grid kernel reaches
 // with fast arithmetic, step_kernel_w3 the acoustics kernel to three.
-template <class S, class Cfg, int G>
+template <class S, class Cfg, int G, int BC>
 __global__ __launch_bounds__(256, CLAWGPU_MIN_WAVES) void step_kernel(Params b)
 {
-    step_loop<S, Cfg, G>(b);
+    step_loop<S, Cfg, G, BC>(b);
 }
 
-template <class S, class Cfg, int G>
+template <class S, class Cfg, int G, int BC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void step_kernel_w2(Params b)
 {
-    step_loop<S, Cfg, G>(b);
+    step_loop<S, Cfg, G, BC>(b);
 }
 
-template <class S, class Cfg, int G>
+template <class S, class Cfg, int G, int BC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void step_kernel_w3(Params b)
 {
-    step_loop<S, Cfg, G>(b);
+    step_loop<S, Cfg, G, BC>(b);
 }
 
 inline bool mc_config(const Params& a)
@@ -151,23 +153,30 @@ inline bool mc_config(const Params& a)
     return mc;
 }
 
-template <class S, int G>
-inline void launch_group(const Params& a, bool mc, int nblk, bool w3, hipStream_t s)
+template <class S, int G, int BC>
+inline void launch_group_bc(const Params& a, bool mc, int nblk, bool w3, hipStream_t s)
 {
     if constexpr (G == 1) {
         if (w3) {
-            if (mc) hipLaunchKernelGGL((step_kernel_w3<S, CfgFix<2, 2, 4>, G>), dim3(nblk), dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((step_kernel_w3<S, CfgAny, G>), dim3(nblk), dim3(256), 0, s, a);
+            if (mc) hipLaunchKernelGGL((step_kernel_w3<S, CfgFix<2, 2, 4>, G, BC>), dim3(nblk), dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((step_kernel_w3<S, CfgAny, G, BC>), dim3(nblk), dim3(256), 0, s, a);
             return;
         }
     }
     if constexpr (G == 0 && S::Math::fast) {
-        if (mc) hipLaunchKernelGGL((step_kernel_w2<S, CfgFix<2, 2, 4>, G>), dim3(nblk), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((step_kernel_w2<S, CfgAny, G>), dim3(nblk), dim3(256), 0, s, a);
+        if (mc) hipLaunchKernelGGL((step_kernel_w2<S, CfgFix<2, 2, 4>, G, BC>), dim3(nblk), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((step_kernel_w2<S, CfgAny, G, BC>), dim3(nblk), dim3(256), 0, s, a);
         return;
     }
-    if (mc) hipLaunchKernelGGL((step_kernel<S, CfgFix<2, 2, 4>, G>), dim3(nblk), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((step_kernel<S, CfgAny, G>), dim3(nblk), dim3(256), 0, s, a);
+    if (mc) hipLaunchKernelGGL((step_kernel<S, CfgFix<2, 2, 4>, G, BC>), dim3(nblk), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((step_kernel<S, CfgAny, G, BC>), dim3(nblk), dim3(256), 0, s, a);
+}
+
+template <class S, int G>
+inline void launch_group(const Params& a, bool mc, int nblk, bool w3, hipStream_t s)
+{
+    if (a.walls[G]) launch_group_bc<S, G, 2>(a, mc, nblk, w3, s);
+    else launch_group_bc<S, G, 1>(a, mc, nblk, w3, s);
 }
 
 // the step kernels of both solver groups (nblk blocks each)
