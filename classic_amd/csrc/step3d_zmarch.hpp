@@ -21,8 +21,9 @@
 // lanes l-1..l+1) are added to register accumulators of planes k-1, k, k+1;
 // those in rows j-1 and j+1 are summed per cell and written to a per-wave
 // LDS mailbox that the neighbouring wave adds after the barrier.  When plane
-// k is done, plane k-1 of the wave's row is complete and is stored as
-// qold + accumulated change.  Sums per cell run in another order than the
+// k is done, plane k-1 of the wave's row is complete and is stored (each
+// plane's accumulators start from its qold, added when the plane is the
+// current one, so the store needs no second read of q).  Sums per cell run in another order than the
 // reference's slice-by-slice updates (as in the tile kernel): agreement to
 // round-off, not bit for bit.
 #pragma once
@@ -241,14 +242,15 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
         wz1 = wz[3];
     }
 
-    // q of the current plane: own cell (p, u, v), the row above (p, v) and,
-    // for wave 0, the row below (p, v); cell k+2 of the own column (p, w)
-    auto load_plane = [&](int k, double& p0, double& u0, double& v0, double& pN, double& vN,
-                          double& pS, double& vS, double& pz2, double& wz2) {
+    // q of the current plane: own cell (p, u, v, w), the row above (p, v)
+    // and, for wave 0, the row below (p, v); cell k+2 of the own column (p, w)
+    auto load_plane = [&](int k, double& p0, double& u0, double& v0, double& w0, double& pN,
+                          double& vN, double& pS, double& vS, double& pz2, double& wz2) {
         const long c = L.offc(i, j, k);
         p0 = a.q[c];
         u0 = a.q[fs + c];
         v0 = a.q[2 * fs + c];
+        w0 = a.q[3 * fs + c];
         const long cn = L.offc(i, j + 1, k);
         pN = a.q[cn];
         vN = a.q[2 * fs + cn];
@@ -261,8 +263,8 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
         pz2 = a.q[cz];
         wz2 = a.q[3 * fs + cz];
     };
-    double p0, u0, v0, pN, vN, pS = 0.0, vS = 0.0, pz2, wz2;
-    load_plane(kb - 1, p0, u0, v0, pN, vN, pS, vS, pz2, wz2);
+    double p0, u0, v0, w0, pN, vN, pS = 0.0, vS = 0.0, pz2, wz2;
+    load_plane(kb - 1, p0, u0, v0, w0, pN, vN, pS, vS, pz2, wz2);
 
     double A[3][4];  // own-row change of planes k-1, k, k+1
 #pragma unroll
@@ -285,6 +287,10 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
     };
 
     for (int k = kb - 1; k <= ke + 1; k++) {
+        A[1][P] += p0;  // qold of plane k
+        A[1][U] += u0;
+        A[1][V] += v0;
+        A[1][W] += w0;
         const Mat me = mat_of(R(1, 1));
         // one material over the whole wave's 3x3 neighbourhoods: rows j-1..j+1
         // and planes k-1..k+1 (x), row j (y), plane k (z)
@@ -420,9 +426,10 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
         __syncthreads();
 
         // next plane's q, in flight during the rest of this one
-        double np0 = 0.0, nu0 = 0.0, nv0 = 0.0, npN = 0.0, nvN = 0.0, npS = 0.0, nvS = 0.0,
-               npz2 = 0.0, nwz2 = 0.0;
-        if (k + 1 <= ke + 1) load_plane(k + 1, np0, nu0, nv0, npN, nvN, npS, nvS, npz2, nwz2);
+        double np0 = 0.0, nu0 = 0.0, nv0 = 0.0, nw0 = 0.0, npN = 0.0, nvN = 0.0, npS = 0.0,
+               nvS = 0.0, npz2 = 0.0, nwz2 = 0.0;
+        if (k + 1 <= ke + 1)
+            load_plane(k + 1, np0, nu0, nv0, nw0, npN, nvN, npS, nvS, npz2, nwz2);
         int Rn[3];
 #pragma unroll
         for (int s = 0; s < 3; s++) Rn[s] = id_at(j + s - 1, k + 3);
@@ -479,7 +486,7 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
             if (store_row && out_lane && k - 1 >= kb) {
                 const long c = L.off(i, j, k - 1);
 #pragma unroll
-                for (int m = 0; m < 4; m++) a.qn[m * fs + c] = a.q[m * fs + c] + A[0][m];
+                for (int m = 0; m < 4; m++) a.qn[m * fs + c] = A[0][m];
             }
         }
 #pragma unroll
@@ -490,7 +497,7 @@ __global__ __launch_bounds__(64 * (TY + 2)) void step3_zmarch_kernel(Args3 a, in
         }
 #pragma unroll
         for (int s = 0; s < 3; s++) Rp[s] = (Rp[s] >> 8) | ((unsigned)Rn[s] << 24);
-        p0 = np0; u0 = nu0; v0 = nv0; pN = npN; vN = nvN; pS = npS; vS = nvS;
+        p0 = np0; u0 = nu0; v0 = nv0; w0 = nw0; pN = npN; vN = nvN; pS = npS; vS = nvS;
         pz2 = npz2; wz2 = nwz2;
         __syncthreads();
     }
