@@ -110,6 +110,7 @@ _SIGS = {
     "clawgpu3_dimsp3": ([_V, _d, _d, _d, _d, _I, _I, _D, _i, _D], _i),
     "clawgpu3_claw3": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
     "clawgpu3_claw3_hooks": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
+    "clawgpu3_claw3_dev_hooks": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
     "clawgpu3_set_segment": ([_V, _i], _i),
     "clawgpu3_set_tile": ([_V, _i, _i], _i),
     "clawgpu3_set_arith": ([_V, _i], _i),

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "devhooks.hpp"
 #include "step3d.hpp"
 #include "step3g.hpp"
 
@@ -38,6 +39,7 @@ struct clawgpu3_ctx {
     std::vector<double> haux;       // host copy of aux for the user routines
     int exact = 0;                  // unsplit step with the reference's rounding and order
     double* drec = nullptr;         // records of the exact / capacity unsplit step
+    double* dsave = nullptr;        // q before a Strang source half step (device user routines)
 };
 
 namespace {
@@ -171,6 +173,22 @@ __global__ void cmax_kernel(const double* __restrict__ c, Layout3 L, unsigned lo
         atomicMax(&out[0], (unsigned long long)__double_as_longlong(m));
         if (nan) atomicOr(&out[1], 1ull);
     }
+}
+
+// Whether every cell's (Z, c) still equals its material's table entry, bit
+// for bit (out[0] set when some cell differs).
+__global__ void table_check_kernel(const double* __restrict__ Z, const double* __restrict__ cc,
+                                   const unsigned char* __restrict__ mat,
+                                   const double* __restrict__ lut, long n, unsigned long long* out)
+{
+    bool diff = false;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x) {
+        const int id = mat[e];
+        diff = diff || __double_as_longlong(Z[e]) != __double_as_longlong(lut[2 * id]) ||
+               __double_as_longlong(cc[e]) != __double_as_longlong(lut[2 * id + 1]);
+    }
+    if (__any(diff) && (threadIdx.x & 63) == 0) atomicOr(&out[0], 1ull);
 }
 
 int check_method3(const int* method)
@@ -504,6 +522,7 @@ extern "C" int clawgpu3_destroy(clawgpu3_ctx* c)
     if (c->dmat) (void)hipFree(c->dmat);
     if (c->dlut) (void)hipFree(c->dlut);
     if (c->drec) (void)hipFree(c->drec);
+    if (c->dsave) (void)hipFree(c->dsave);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return CLAWGPU_OK;
@@ -653,6 +672,64 @@ extern "C" int clawgpu3_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz,
     return CLAWGPU_OK;
 }
 
+// After aux changed on the device (a device b4step3): the sound-speed
+// maximum again, and the material table kept only if every cell still
+// matches it (otherwise the steps read aux).
+static int refresh_aux_on_device(clawgpu3_ctx* c)
+{
+    const long n = c->L.fs;
+    unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->staging);
+    HIPCHK(c, hipMemsetAsync(bits, 0, 3 * sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(cmax_kernel, dim3(1024), dim3(256), 0, c->stream, c->daux + n, c->L, bits);
+    if (c->nmat > 0)
+        hipLaunchKernelGGL(table_check_kernel, dim3(1024), dim3(256), 0, c->stream, c->daux,
+                           c->daux + n, c->dmat, c->dlut, n, bits + 2);
+    HIPCHK(c, hipGetLastError());
+    unsigned long long h[3] = {0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(h, bits, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double m;
+    std::memcpy(&m, &h[0], sizeof m);
+    c->cmax = h[1] ? NAN : m;
+    if (h[2]) c->nmat = 0;
+    return CLAWGPU_OK;
+}
+
+// One launch of device routine f on q (one thread per cell, ghost cells
+// included, blocks of 256).
+static int launch_dev_hook3(clawgpu3_ctx* c, hipFunction_t f, const clawgpu_dev_hooks* h, double* q,
+                            const double* geom, double t, double dt, const int* mthbc)
+{
+    clawgpu_dev_grid3 g;
+    g.q = q;
+    g.aux = c->maux > 0 ? c->daux : nullptr;
+    g.user = h->user;
+    g.fs = c->L.fs;
+    g.meqn = c->meqn;
+    g.maux = c->maux;
+    g.mbc = c->mbc;
+    g.mx = c->mx;
+    g.my = c->my;
+    g.mz = c->mz;
+    for (int k = 0; k < 6; k++) g.mthbc[k] = mthbc[k];
+    g.xlower = geom[0];
+    g.ylower = geom[1];
+    g.zlower = geom[2];
+    g.dx = geom[3];
+    g.dy = geom[4];
+    g.dz = geom[5];
+    g.t = t;
+    g.dt = dt;
+    void* args[] = {&g};
+    const hipError_t e = hipModuleLaunchKernel(f, (unsigned)((c->L.fs + 255) / 256), 1, 1, 256, 1, 1,
+                                               0, c->stream, args, nullptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(CLAWGPU_ERR_HIP, "user routine launch failed");
+    }
+    return CLAWGPU_OK;
+}
+
 // The claw3 loop (src/3d/claw3.f:735-904).  The Courant number of this
 // solver is dt/d * max|c|, known before the step, so an attempt that would be
 // rejected is not run (the reference runs it, then discards the result and
@@ -660,14 +737,22 @@ extern "C" int clawgpu3_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz,
 // host copy of q at the reference's points (claw3.f:766-885): bc3 and b4step3
 // before every attempt, src3 over dt/2 before the step (its input kept for a
 // retake) and over dt/2 after it (method(5) = 2), or over dt after it
-// (method(5) = 1).
+// (method(5) = 1).  Device routines (dh) run on the device q at the same
+// points; the state before a Strang half step is kept on the device.
 static int claw3_loop(clawgpu3_ctx* c, const double* geom, double tstart, double* tend,
                       double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
-                      const int* mthbc, int use_fwave, const clawgpu3_hooks* h, int* info_out)
+                      const int* mthbc, int use_fwave, const clawgpu3_hooks* h,
+                      const clawgpu_dev_hooks* dh, int* info_out)
 {
     if (!c || !geom || !tend || !dtv || !cflv || !nv || !method || !mthlim || !mthbc || !info_out)
         return set_err(CLAWGPU_ERR_ARG, "null argument");
-    const bool user_bc = h && h->bc3;
+    const bool dev_bc = dh && dh->bc2;
+    const bool user_bc = (h && h->bc3) || dev_bc;
+    const bool dev_src = dh && dh->src2;
+    const size_t qbytes = (size_t)c->L.fs * c->meqn * sizeof(double);
+    auto dev_call = [&](hipFunction_t f, double tt, double dtt) -> int {
+        return launch_dev_hook3(c, f, dh, c->dq[c->cur], geom, tt, dtt, mthbc);
+    };
     if (!user_bc && check_bc3(mthbc) != 0)
         return set_err(CLAWGPU_ERR_UNSUPPORTED, "mthbc must be 1, 2 or 3 (periodic on both sides), or 0 on z-hi (piston)");
     const double xl = geom[0], yl = geom[1], zl = geom[2];
@@ -703,7 +788,9 @@ static int claw3_loop(clawgpu3_ctx* c, const double* geom, double tstart, double
         double told = t;
         if (told + dt > *tend && tstart < *tend) dt = *tend - told;
         for (;;) {
-            if (user_bc) {
+            if (dev_bc) {
+                if ((rc = dev_call(dh->bc2, told, dt))) return rc;
+            } else if (user_bc) {
                 if ((rc = pull())) return rc;
                 h->bc3(h->user, c->meqn, c->mbc, c->mx, c->my, c->mz, xl, yl, zl, dx, dy, dz,
                        hq.data(), c->maux, haux, told, dt, mthbc);
@@ -718,8 +805,18 @@ static int claw3_loop(clawgpu3_ctx* c, const double* geom, double tstart, double
                            dx, dy, dz, told, dt, c->maux, haux);
                 if ((rc = push())) return rc;
                 if (haux && (rc = clawgpu3_set_aux(c, c->haux.data()))) return rc;
+            } else if (dh && dh->b4step2) {
+                if ((rc = dev_call(dh->b4step2, told, dt))) return rc;
+                if (c->maux > 0 && (rc = refresh_aux_on_device(c))) return rc;
             }
             const bool strang_src = method[4] == 2 && h && h->src3;
+            const bool dev_strang = method[4] == 2 && dev_src;
+            if (dev_strang) {
+                if (!c->dsave) HIPCHK(c, hipMalloc((void**)&c->dsave, qbytes));
+                HIPCHK(c, hipMemcpyAsync(c->dsave, c->dq[c->cur], qbytes, hipMemcpyDeviceToDevice,
+                                         c->stream));
+                if ((rc = dev_call(dh->src2, told, dt / 2.0))) return rc;
+            }
             if (strang_src) {
                 if ((rc = pull())) return rc;
                 saved = hq;
@@ -757,6 +854,9 @@ static int claw3_loop(clawgpu3_ctx* c, const double* geom, double tstart, double
                 hq = saved;
                 if ((rc = push())) return rc;
             }
+            if (dev_strang)
+                HIPCHK(c, hipMemcpyAsync(c->dq[c->cur], c->dsave, qbytes, hipMemcpyDeviceToDevice,
+                                         c->stream));
             if (method[0] == 1) {
                 dt = fmin(dtv[1], dt * cflv[1] / cfl);
                 continue;
@@ -769,6 +869,8 @@ static int claw3_loop(clawgpu3_ctx* c, const double* geom, double tstart, double
             h->src3(h->user, c->meqn, c->mbc, c->mx, c->my, c->mz, xl, yl, zl, dx, dy, dz,
                     hq.data(), c->maux, haux, t, method[4] == 2 ? dt / 2.0 : dt);
             if ((rc = push())) return rc;
+        } else if (dev_src && (method[4] == 1 || method[4] == 2)) {
+            if ((rc = dev_call(dh->src2, t, method[4] == 2 ? dt / 2.0 : dt))) return rc;
         }
         if (method[0] == 1) {
             if (cfl == 0.0) dt = dtv[1];
@@ -798,7 +900,18 @@ extern "C" int clawgpu3_claw3(clawgpu3_ctx* c, const double* geom, double tstart
                               const int* mthlim, const int* mthbc, int use_fwave, int* info_out)
 {
     return claw3_loop(c, geom, tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, use_fwave,
-                      nullptr, info_out);
+                      nullptr, nullptr, info_out);
+}
+
+extern "C" int clawgpu3_claw3_dev_hooks(clawgpu3_ctx* c, const double* geom, double tstart,
+                                        double* tend, double* dtv, double* cflv, int* nv,
+                                        const int* method, const int* mthlim, const int* mthbc,
+                                        int use_fwave, const clawgpu_dev_hooks* hooks,
+                                        int* info_out)
+{
+    if (!hooks) return set_err(CLAWGPU_ERR_ARG, "null hooks (use clawgpu3_claw3)");
+    return claw3_loop(c, geom, tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, use_fwave,
+                      nullptr, hooks, info_out);
 }
 
 extern "C" int clawgpu3_claw3_hooks(clawgpu3_ctx* c, const double* geom, double tstart,
@@ -810,7 +923,7 @@ extern "C" int clawgpu3_claw3_hooks(clawgpu3_ctx* c, const double* geom, double 
     if (c && c->maux > 0 && c->haux.empty() && (hooks->b4step3 || hooks->src3 || hooks->bc3))
         return set_err(CLAWGPU_ERR_ARG, "aux has not been set (clawgpu3_set_aux)");
     return claw3_loop(c, geom, tstart, tend, dtv, cflv, nv, method, mthlim, mthbc, use_fwave,
-                      hooks, info_out);
+                      hooks, nullptr, info_out);
 }
 
 extern "C" int clawgpu3_set_segment(clawgpu3_ctx* c, int cells)

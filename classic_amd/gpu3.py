@@ -148,9 +148,11 @@ class Grid3D:
         return cfl.value
 
     def claw3(self, tstart, tend, dtv, cflv, nmax, method=(1, 2, 22, 0, 0, 0, 2), mthlim=(3, 3),
-              mthbc=(1, 1, 1, 1, 1, 1), use_fwave=False, bc3=None, b4step3=None, src3=None):
+              mthbc=(1, 1, 1, 1, 1, 1), use_fwave=False, bc3=None, b4step3=None, src3=None,
+              dev_hooks=None):
         """The claw3 time loop on the device; returns Claw3Result.
-        bc3 / b4step3 / src3: optional user routines (see make_hooks3)."""
+        bc3 / b4step3 / src3: optional user routines (see make_hooks3);
+        dev_hooks: a gpu.DeviceHooks of 3D device routines instead."""
         te = C.c_double(tend)
         dtv_a = np.zeros(5)
         dtv_a[: len(dtv)] = dtv
@@ -160,7 +162,9 @@ class Grid3D:
         info = C.c_int(0)
         args = (_dp(self.geom), tstart, C.byref(te), _dp(dtv_a), _dp(cflv_a), _ip(nv),
                 _ip(_ivec(method, 7)), _ip(_ivec(mthlim, 2)), _ip(_ivec(mthbc, 6)), int(use_fwave))
-        if bc3 is None and b4step3 is None and src3 is None:
+        if dev_hooks is not None:
+            check(self.lib.clawgpu3_claw3_dev_hooks(self.h, *args, dev_hooks.h, C.byref(info)))
+        elif bc3 is None and b4step3 is None and src3 is None:
             check(self.lib.clawgpu3_claw3(self.h, *args, C.byref(info)))
         else:
             hooks, keep, errors = make_hooks3(bc3, b4step3, src3)

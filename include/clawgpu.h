@@ -455,6 +455,33 @@ int clawgpu3_claw3_hooks(clawgpu3_ctx* ctx, const double* geom, double tstart, d
                          double* dtv, double* cflv, int* nv, const int* method, const int* mthlim,
                          const int* mthbc, int use_fwave, const clawgpu3_hooks* hooks, int* info);
 
+/* Device-side 3D user routines (bc3, b4step3, src3 of claw3.f:766-885) as
+ * HIP kernels, loaded with clawgpu_dev_hooks_load (their names in the bc,
+ * b4step and src positions).  Each takes one clawgpu_dev_grid3 by value and
+ * is launched with one thread per cell, ghost cells included: thread t
+ * handles cell (i, j, k) with t = ((k + mbc - 1) * (my + 2 mbc) + (j + mbc -
+ * 1)) * (mx + 2 mbc) + (i + mbc - 1) (include/clawgpu_device.h). */
+typedef struct clawgpu_dev_grid3 {
+    double* q;       /* q(m, i, j, k) at q[m * fs + t(i, j, k)] */
+    double* aux;     /* aux(m, i, j, k) likewise */
+    double* user;    /* the hooks' user array, on the device */
+    long fs;         /* doubles per field plane: (mx + 2 mbc)(my + 2 mbc)(mz + 2 mbc) */
+    int meqn, maux, mbc, mx, my, mz;
+    int mthbc[6];
+    double xlower, ylower, zlower, dx, dy, dz, t, dt;
+} clawgpu_dev_grid3;
+
+/* clawgpu3_claw3_hooks with device routines, called at the same points on
+ * the device q (the state before a Strang source half step is kept on the
+ * device for a retake).  After a device b4step3 the sound-speed maximum
+ * (the Courant number) is recomputed on the device and the material table
+ * is checked against aux; if b4step3 changed a material, later steps read
+ * aux directly. */
+int clawgpu3_claw3_dev_hooks(clawgpu3_ctx* ctx, const double* geom, double tstart, double* tend,
+                             double* dtv, double* cflv, int* nv, const int* method,
+                             const int* mthlim, const int* mthbc, int use_fwave,
+                             const clawgpu_dev_hooks* hooks, int* info);
+
 /* Cells per segment of the y/z split sweeps (tuning; default 64). */
 int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
 

@@ -273,6 +273,118 @@ def test_device_hooks_keep_q_on_the_device():
     assert times[1] < times[0]
 
 
+@pytest.mark.parametrize("name", list(H.CASES3))
+def test_device_hooks_match_reference_claw3(name):
+    """The 3D test routines as device kernels (clawgpu3_claw3_dev_hooks):
+    the same calls with the same (t, dt) as the reference claw3 calling the
+    host routines, Strang half steps with the pre-source state kept on the
+    device, and the same final state (bit for bit for the split step, to
+    1e-12 for the unsplit one, as with host routines)."""
+    c = H.CASES3[name]
+    g0 = np.load(os.path.join(GOLD, f"claw3_hooks_{name}.npz"))
+    q0, aux = H.initial3(name)
+    with _dev_hooks(c["k"], len(g0["log"]) + 16, b4step2="test_b4step3", src2="test_src3") as dh, \
+            Grid3D(c["mx"], c["my"], c["mz"], maux=2) as g:
+        g.set_aux(aux)
+        g.set_q(q0)
+        out = g.claw3(0.0, c["tend"], list(c["dtv"]), [1.0, 0.9], 500, method=c["method"],
+                      mthlim=c["mthlim"], mthbc=c["mthbc"], dev_hooks=dh)
+        q = g.get_q()
+        log = _dev_log(dh)
+    print(name, "steps", out.nv[1], "calls", len(log), flush=True)
+    np.testing.assert_array_equal(log, g0["log"])
+    assert out.info == int(g0["info"]) and out.tend == float(g0["tend"])
+    assert out.nv[1] == g0["nv"][1]
+    np.testing.assert_array_equal(out.dtv[2:5], g0["dtv"][2:5])
+    np.testing.assert_array_equal(out.cflv[2:4], g0["cflv"][2:4])
+    if c["method"][2] < 0:
+        np.testing.assert_array_equal(q[2:-2, 2:-2, 2:-2], g0["q"][2:-2, 2:-2, 2:-2])
+    assert max_rel_err3(q, g0["q"]) <= 1e-12
+
+
+def test_device_bc3_equals_builtin():
+    """A device bc3 that extrapolates gives the built-in mthbc = 1 result bit
+    for bit (split step, retakes included)."""
+    q0, aux = H.initial3("acoustics3d_src_strang_split")
+    c = H.CASES3["acoustics3d_src_strang_split"]
+    res = []
+    for user in (False, True):
+        with _dev_hooks(0.0, 1, bc2="test_bc3_extrap") as dh, \
+                Grid3D(c["mx"], c["my"], c["mz"], maux=2) as g:
+            g.set_aux(aux)
+            g.set_q(q0)
+            kw = dict(dev_hooks=dh) if user else {}
+            out = g.claw3(0.0, 0.3, [0.3, 1e9], [1.0, 0.9], 500, method=(1, 2, -1, 0, 0, 0, 0),
+                          mthlim=(3, 3), mthbc=(0,) * 6 if user else (1,) * 6, **kw)
+            res.append((g.get_q()[2:-2, 2:-2, 2:-2], out.nv[1], out.tend))
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    assert res[0][1:] == res[1][1:]
+
+
+def test_device_b4step3_changing_aux_equals_host():
+    """A b4step3 that raises every sound speed by (1 + f dt) each attempt:
+    as a device kernel (the library recomputes the Courant bound on the
+    device and drops the material table once it no longer matches) it gives
+    what the same routine gives on the host (aux copied back, table rebuilt),
+    bit for bit: the split step, its dt sequence and its retakes."""
+    q0, aux = H.initial3("acoustics3d_src_strang_split")
+    c = H.CASES3["acoustics3d_src_strang_split"]
+    f = 0.25
+    calls = []
+
+    def b4(q, a, t, dt):
+        calls.append((t, dt))
+        a[..., 1] = a[..., 1] * (1.0 + f * dt)
+
+    res = []
+    for dev in (False, True):
+        with _dev_hooks(f, 400, b4step2="test_b4step3_speed") as dh, \
+                Grid3D(c["mx"], c["my"], c["mz"], maux=2) as g:
+            g.set_aux(aux)
+            g.set_q(q0)
+            kw = dict(dev_hooks=dh) if dev else dict(b4step3=b4)
+            out = g.claw3(0.0, 0.3, [0.3, 1e9], [1.0, 0.9], 500, method=(1, 2, -1, 0, 0, 0, 0),
+                          mthlim=(3, 3), mthbc=(1, 1, 3, 3, 1, 1), **kw)
+            log = _dev_log(dh) if dev else np.array([(1.0, t, dt) for t, dt in calls])
+            res.append((g.get_q()[2:-2, 2:-2, 2:-2], out, log))
+    (qh, oh, lh), (qd, od, ld) = res
+    assert len(lh) > oh.nv[1]  # some attempt was retaken
+    np.testing.assert_array_equal(ld, lh)
+    assert oh.nv[1] == od.nv[1] and oh.tend == od.tend
+    np.testing.assert_array_equal(od.dtv[2:5], oh.dtv[2:5])
+    np.testing.assert_array_equal(od.cflv[2:4], oh.cflv[2:4])
+    np.testing.assert_array_equal(qd, qh)
+
+
+def test_device_hooks_3d_keep_q_on_the_device():
+    """At 192^3 (unsplit step, src3 every step) the device routines run the
+    claw3 loop without moving q: the same state as with host routines, bit
+    for bit, and faster."""
+    import time
+
+    from classic_amd import problems as P
+    n, k, steps = 192, 0.6, 8
+    q0, aux, _ = P.acoustics3d_pulse(n, seed=3)
+    b4, src, log = H.python_hooks(k, dim=3)
+    times, qs = [], []
+    for dev in (False, True):
+        with _dev_hooks(k, 4 * steps, b4step2="test_b4step3", src2="test_src3") as dh, \
+                Grid3D(n, n, n, maux=2) as g:
+            g.set_aux(aux)
+            g.set_q(q0)
+            kw = dict(dev_hooks=dh) if dev else dict(b4step3=b4, src3=src)
+            t0 = time.perf_counter()
+            out = g.claw3(0.0, 1e9, [1e-3, 1e99], [1.0, 0.9], steps, method=(1, 2, 22, 0, 1, 0, 2),
+                          mthlim=(2, 2), mthbc=(3,) * 6, **kw)
+            times.append(time.perf_counter() - t0)
+            qs.append(g.get_q())
+            assert out.nv[1] == steps
+    print(f"192^3, {steps} steps with src3: host routines {times[0] * 1e3:.1f} ms, "
+          f"device routines {times[1] * 1e3:.1f} ms", flush=True)
+    np.testing.assert_array_equal(qs[0][2:-2, 2:-2, 2:-2], qs[1][2:-2, 2:-2, 2:-2])
+    assert times[1] < times[0]
+
+
 def test_device_hooks_load_errors():
     """A missing code object or kernel is an error of the load call alone:
     the next step on the device runs normally."""
