@@ -111,6 +111,10 @@ _SIGS = {
     "clawgpu3_claw3": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _I], _i),
     "clawgpu3_claw3_hooks": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
     "clawgpu3_claw3_dev_hooks": ([_V, _D, _d, _D, _D, _D, _I, _I, _I, _I, _i, _V, _I], _i),
+    "clawgpu_solver3_load": ([C.POINTER(_V), C.c_char_p, C.c_char_p], _i),
+    "clawgpu_solver3_destroy": ([_V], _i),
+    "clawgpu_solver3_shape": ([_V, _I, _I, _I, _I], _i),
+    "clawgpu3_create_user": ([C.POINTER(_V), _V, _i, _i, _i, _i, _D, _i], _i),
     "clawgpu3_set_segment": ([_V, _i], _i),
     "clawgpu3_set_tile": ([_V, _i, _i], _i),
     "clawgpu3_set_arith": ([_V, _i], _i),

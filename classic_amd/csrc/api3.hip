@@ -13,7 +13,9 @@
 #include "ctx.hpp"
 #include "devhooks.hpp"
 #include "step3d.hpp"
+#include "step3d_user.hpp"
 #include "step3g.hpp"
+#include "usolver.hpp"
 
 using namespace clawgpu::s3;
 
@@ -40,6 +42,8 @@ struct clawgpu3_ctx {
     int exact = 0;                  // unsplit step with the reference's rounding and order
     double* drec = nullptr;         // records of the exact / capacity unsplit step
     double* dsave = nullptr;        // q before a Strang source half step (device user routines)
+    const clawgpu_solver3* us = nullptr;  // user solver (split step only), not owned
+    SolverParams P{};                     // its parameters
 };
 
 namespace {
@@ -62,6 +66,7 @@ __global__ void planes_to_aos3(const double* __restrict__ src, double* __restric
 
 struct Bc3 {
     int code[6];
+    int meqn;                // fields (at most kMaxWaves)
     double t, et;            // time and exp(-500 t^2) (host libm)
     const double* exy;       // exp(-500 (x^2 + y^2)) per column (host libm)
 };
@@ -130,10 +135,10 @@ __global__ void bc3_kernel(double* __restrict__ q, Layout3 L, Bc3 b)
     }
     if (!ghost) return;  // not reached: every enumerated cell is a ghost cell
     const long src = L.off(s[0], s[1], s[2]);
-    double v[4];
-    for (int m = 0; m < 4; m++) v[m] = q[m * L.fs + src];
+    double v[kMaxWaves];
+    for (int m = 0; m < b.meqn; m++) v[m] = q[m * L.fs + src];
     for (int d = 0; d < 3; d++)
-        if (flip[d]) v[1 + d] = -v[1 + d];
+        if (flip[d] && 1 + d < b.meqn) v[1 + d] = -v[1 + d];
     if (piston) {
         double sp = 0.0;
         if (b.t < 0.1) {
@@ -145,7 +150,7 @@ __global__ void bc3_kernel(double* __restrict__ q, Layout3 L, Bc3 b)
         v[3] = 2.0 * sp - v[3];
     }
     const long dst = L.off(g[0], g[1], g[2]);
-    for (int m = 0; m < 4; m++) q[m * L.fs + dst] = v[m];
+    for (int m = 0; m < b.meqn; m++) q[m * L.fs + dst] = v[m];
 }
 
 // max |c| over cells with all indices in [0, n+1] (the cells adjacent to
@@ -208,11 +213,28 @@ int check_args3(clawgpu3_ctx* c, const int* method, const int* mthlim)
 {
     if (method[2] < 0 && method[2] != -1)
         return set_err(CLAWGPU_ERR_ARG, "method(3) < 0 must be -1 (dimsp3 is Godunov splitting only)");
+    if (c->us && method[2] != -1)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "a 3D user solver runs the split step (method(3) = -1) only");
+    if (c->us && method[5] != 0)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "a 3D user solver has no capacity function");
     if (check_method3(method) != 0) return set_err(CLAWGPU_ERR_ARG, "invalid method (chkmth info=6)");
     if (method[5] < 0 || method[5] > c->maux)
         return set_err(CLAWGPU_ERR_ARG, "method(6) must be 0 or an aux index (capacity)");
     for (int w = 0; w < c->mwaves; w++)
         if (mthlim[w] < 0 || mthlim[w] > 5) return set_err(CLAWGPU_ERR_ARG, "mthlim must be 0..5");
+    return CLAWGPU_OK;
+}
+
+// Walls negate components 2, 3, 4 (1-based) as bc3 does, and the piston
+// (code 0) is the acoustics example's: a user solver needs those components
+// and takes no piston.
+static int check_walls3(const clawgpu3_ctx* c, const int* mthbc)
+{
+    if (!c->us) return CLAWGPU_OK;
+    if (mthbc[5] == 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "the piston boundary (code 0) is the built-in acoustics solver's");
+    for (int d = 0; d < 3; d++)
+        if ((mthbc[2 * d] == 3 || mthbc[2 * d + 1] == 3) && 1 + d >= c->meqn)
+            return set_err(CLAWGPU_ERR_ARG, "a wall in this direction needs the normal velocity component (meqn too small)");
     return CLAWGPU_OK;
 }
 
@@ -256,8 +278,11 @@ int piston_table(clawgpu3_ctx* c, const double* geom)
 
 int enqueue_bc3(clawgpu3_ctx* c, double* q, const double* geom, double t, const int* mthbc)
 {
+    int rc0 = check_walls3(c, mthbc);
+    if (rc0) return rc0;
     Bc3 b;
     for (int f = 0; f < 6; f++) b.code[f] = mthbc[f];
+    b.meqn = c->meqn;
     b.t = t;
     b.et = 0.0;
     b.exy = nullptr;
@@ -443,6 +468,67 @@ int enqueue_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz, double dt, 
     return CLAWGPU_OK;
 }
 
+// dimsp3 with a user solver: x (A -> B), y (B -> C), z (C -> B) sweeps of
+// every field, each writing its Courant number to its own slot; the numbers
+// come back to the host after the sweeps (the wave speeds are known only
+// there), so every sweep runs and a rejected step's buffers are not used.
+// With stop (cflv(1)) the numbers come back after each sweep and the step
+// ends after the first sweep above it, as dimsp3's early returns.
+int enqueue_dimsp3_user(clawgpu3_ctx* c, double dx, double dy, double dz, double dt,
+                        const int* method, const int* mthlim, int use_fwave, const double* stop,
+                        int* res, double cf[3])
+{
+    const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
+    unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->staging);
+    HIPCHK(c, hipMemsetAsync(bits, 0, 3 * sizeof(unsigned long long), c->stream));
+    const int from[3] = {A, B, C}, to[3] = {B, C, B};
+    const double dd[3] = {dt / dx, dt / dy, dt / dz};
+    for (int dir = 0; dir < 3; dir++) {
+        s3u::Sweep3U a;
+        a.q = c->dq[from[dir]];
+        a.qn = c->dq[to[dir]];
+        a.aux = c->daux;
+        a.L = c->L;
+        a.d = dd[dir];
+        a.m2 = method[1];
+        a.fwave = use_fwave ? 1 : 0;
+        for (int w = 0; w < kMaxWaves; w++) a.lim[w] = w < c->mwaves ? mthlim[w] : 0;
+        a.seg = c->seg;
+        const int n = dir == 1 ? c->my : c->mz;
+        a.nsegs = (n + c->seg - 1) / c->seg;
+        a.nstrips = dir == 0 ? (c->mx + 2 + kOut - 1) / kOut : (c->L.nx + 63) / 64;
+        a.cfl_bits = bits + dir;
+        a.P = c->P;
+        const long nwaves =
+            dir == 0 ? (long)a.nstrips * (c->L.ny) * (c->L.nz)
+                     : (long)a.nstrips * (dir == 1 ? c->L.nz : c->L.ny) * a.nsegs;
+        if (usolver3_sweep(c->us, dir, a, (int)((nwaves + 3) / 4), c->stream))
+            return set_err(CLAWGPU_ERR_HIP, "user sweep kernel launch failed");
+        *res = to[dir];
+        if (stop && dir < 2) {
+            unsigned long long h;
+            HIPCHK(c, hipMemcpyAsync(&h, bits + dir, sizeof h, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            std::memcpy(&cf[dir], &h, sizeof(double));
+            if (cf[dir] > *stop) {
+                for (int e = dir + 1; e < 3; e++) cf[e] = 0.0;
+                return CLAWGPU_OK;
+            }
+        }
+    }
+    unsigned long long h[3];
+    HIPCHK(c, hipMemcpyAsync(h, bits, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int d = 0; d < 3; d++) std::memcpy(&cf[d], &h[d], sizeof(double));
+    return CLAWGPU_OK;
+}
+
+// dimsp3's Courant number from the three sweeps' (claw3 rejects above cflv(1))
+double split_cfl(const double cf[3], double cfl1)
+{
+    return cf[0] > cfl1 ? cf[0] : cf[1] > cfl1 ? cf[1] : fmax(cf[0], fmax(cf[1], cf[2]));
+}
+
 // the Courant number of an unsplit step (the max over all slices); NaN
 // when a sound speed is NaN
 int unsplit_cfl(clawgpu3_ctx* c, const int* method, double dx, double dy, double dz, double dt,
@@ -457,21 +543,16 @@ int unsplit_cfl(clawgpu3_ctx* c, const int* method, double dx, double dy, double
 
 }  // namespace
 
-extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int maux, int mbc, int mx,
-                               int my, int mz, int solver_id)
+static int create3_ctx(clawgpu3_ctx** out, int meqn, int mwaves, int maux, int mx, int my, int mz,
+                       int solver_id, const clawgpu_solver3* us)
 {
-    if (!out) return set_err(CLAWGPU_ERR_ARG, "null output pointer");
-    *out = nullptr;
-    if (solver_id != CLAWGPU_SOLVER_VC_ACOUSTICS)
-        return set_err(CLAWGPU_ERR_UNSUPPORTED, "the 3D path has the variable-coefficient acoustics solver only");
-    if (meqn != 4 || mwaves != 2 || maux < 2)
-        return set_err(CLAWGPU_ERR_ARG, "vc acoustics needs meqn=4, mwaves=2, maux>=2 (Z, c)");
-    if (mbc != 2) return set_err(CLAWGPU_ERR_UNSUPPORTED, "the 3D path needs mbc = 2");
+    const int mbc = 2;
     if (mx < 1 || my < 1 || mz < 1) return set_err(CLAWGPU_ERR_ARG, "grid sizes must be >= 1");
     clawgpu3_ctx* c = new (std::nothrow) clawgpu3_ctx();
     if (!c) return set_err(CLAWGPU_ERR_ARG, "out of host memory");
     c->meqn = meqn; c->mwaves = mwaves; c->maux = maux; c->mbc = mbc;
     c->mx = mx; c->my = my; c->mz = mz; c->solver = solver_id;
+    c->us = us;
     c->L.mx = mx; c->L.my = my; c->L.mz = mz; c->L.mbc = mbc;
     c->L.nx = mx + 2 * mbc; c->L.ny = my + 2 * mbc; c->L.nz = mz + 2 * mbc;
     c->L.fs = (long)c->L.nx * c->L.ny * c->L.nz;
@@ -484,6 +565,7 @@ extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int mau
     c->xcd = 3;
     c->cmax = 0.0;
     const size_t planes = (size_t)c->L.fs * sizeof(double);
+    const int naux = maux > 0 ? maux : 1;
     int rc = CLAWGPU_OK;
     auto fail = [&](hipError_t e) {
         snprintf(g_errbuf, sizeof g_errbuf, "device allocation failed: %s", hipGetErrorString(e));
@@ -491,13 +573,13 @@ extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int mau
     };
     hipError_t e;
     for (int k = 0; k < 3 && rc == CLAWGPU_OK; k++)
-        if ((e = hipMalloc((void**)&c->dq[k], 4 * planes)) != hipSuccess) fail(e);
-    if (rc == CLAWGPU_OK && (e = hipMalloc((void**)&c->daux, (size_t)maux * planes)) != hipSuccess) fail(e);
+        if ((e = hipMalloc((void**)&c->dq[k], (size_t)meqn * planes)) != hipSuccess) fail(e);
+    if (rc == CLAWGPU_OK && (e = hipMalloc((void**)&c->daux, (size_t)naux * planes)) != hipSuccess) fail(e);
     if (rc == CLAWGPU_OK &&
-        (e = hipMalloc((void**)&c->staging, (size_t)(meqn > maux ? meqn : maux) * planes)) != hipSuccess)
+        (e = hipMalloc((void**)&c->staging, (size_t)(meqn > naux ? meqn : naux) * planes)) != hipSuccess)
         fail(e);
     if (rc == CLAWGPU_OK && (e = hipStreamCreate(&c->stream)) != hipSuccess) fail(e);
-    if (rc == CLAWGPU_OK && (e = hipMemsetAsync(c->daux, 0, (size_t)maux * planes, c->stream)) != hipSuccess) fail(e);
+    if (rc == CLAWGPU_OK && (e = hipMemsetAsync(c->daux, 0, (size_t)naux * planes, c->stream)) != hipSuccess) fail(e);
     if (rc == CLAWGPU_OK && (e = hipStreamSynchronize(c->stream)) != hipSuccess) fail(e);
     if (rc != CLAWGPU_OK) {
         for (int k = 0; k < 3; k++) (void)hipFree(c->dq[k]);
@@ -507,6 +589,38 @@ extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int mau
         delete c;
         return rc;
     }
+    *out = c;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu3_create(clawgpu3_ctx** out, int meqn, int mwaves, int maux, int mbc, int mx,
+                               int my, int mz, int solver_id)
+{
+    if (!out) return set_err(CLAWGPU_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (solver_id != CLAWGPU_SOLVER_VC_ACOUSTICS)
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "the 3D path has the variable-coefficient acoustics solver only (or a user solver: clawgpu3_create_user)");
+    if (meqn != 4 || mwaves != 2 || maux < 2)
+        return set_err(CLAWGPU_ERR_ARG, "vc acoustics needs meqn=4, mwaves=2, maux>=2 (Z, c)");
+    if (mbc != 2) return set_err(CLAWGPU_ERR_UNSUPPORTED, "the 3D path needs mbc = 2");
+    return create3_ctx(out, meqn, mwaves, maux, mx, my, mz, solver_id, nullptr);
+}
+
+extern "C" int clawgpu3_create_user(clawgpu3_ctx** out, const clawgpu_solver3* s, int maux, int mx,
+                                    int my, int mz, const double* params, int nparams)
+{
+    if (!out) return set_err(CLAWGPU_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!s) return set_err(CLAWGPU_ERR_ARG, "null solver");
+    if (maux < s->maux) return set_err(CLAWGPU_ERR_ARG, "maux is below the solver's aux count");
+    if (nparams < 0 || nparams > clawgpu::kUserParams || (nparams > 0 && !params))
+        return set_err(CLAWGPU_ERR_ARG, "at most 16 solver parameters");
+    clawgpu3_ctx* c = nullptr;
+    int rc = create3_ctx(&c, s->meqn, s->mwaves, maux, mx, my, mz, CLAWGPU_SOLVER_USER, s);
+    if (rc) return rc;
+    c->P.nuser = nparams;
+    for (int k = 0; k < nparams; k++) c->P.user[k] = params[k];
+    c->exact = s->fast ? 0 : 1;
     *out = c;
     return CLAWGPU_OK;
 }
@@ -609,6 +723,10 @@ extern "C" int clawgpu3_set_aux(clawgpu3_ctx* c, const double* aux)
     hipLaunchKernelGGL(aos_to_planes3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                        c->staging, c->daux, n, c->maux, c->maux, n);
     HIPCHK(c, hipGetLastError());
+    if (c->us) {  // a user solver reads aux itself: no sound speeds or material table
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return CLAWGPU_OK;
+    }
     unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->staging);
     HIPCHK(c, hipMemsetAsync(bits, 0, 2 * sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(cmax_kernel, dim3(1024), dim3(256), 0, c->stream, c->daux + n, c->L, bits);
@@ -665,7 +783,13 @@ extern "C" int clawgpu3_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz,
     int rc = check_args3(c, method, mthlim);
     if (rc) return rc;
     int res = c->cur;
-    rc = enqueue_dimsp3(c, dx, dy, dz, dt, method, mthlim, cflv, use_fwave, &res, cfl);
+    if (c->us) {
+        double cf[3];
+        rc = enqueue_dimsp3_user(c, dx, dy, dz, dt, method, mthlim, use_fwave, cflv, &res, cf);
+        *cfl = split_cfl(cf, cflv[0]);
+    } else {
+        rc = enqueue_dimsp3(c, dx, dy, dz, dt, method, mthlim, cflv, use_fwave, &res, cfl);
+    }
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->cur = res;
@@ -677,6 +801,7 @@ extern "C" int clawgpu3_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz,
 // matches it (otherwise the steps read aux).
 static int refresh_aux_on_device(clawgpu3_ctx* c)
 {
+    if (c->us) return CLAWGPU_OK;
     const long n = c->L.fs;
     unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->staging);
     HIPCHK(c, hipMemsetAsync(bits, 0, 3 * sizeof(unsigned long long), c->stream));
@@ -825,24 +950,34 @@ static int claw3_loop(clawgpu3_ctx* c, const double* geom, double tstart, double
                 if ((rc = push())) return rc;
             }
             double cf[3];
-            if ((rc = courant3(c, method, dx, dy, dz, dt, cf))) return rc;
-            if (!std::isfinite(c->cmax) || !std::isfinite(cf[0]) || !std::isfinite(cf[1]) ||
-                !std::isfinite(cf[2])) {
-                *info_out = -5;
-                return set_err(CLAWGPU_ERR_NONFINITE, "non-finite sound speed (Courant number)");
-            }
-            if (method[2] >= 0) cfl = fmax(cf[0], fmax(cf[1], cf[2]));
-            else {
-                const double cx = cf[0], cy = cf[1], cz = cf[2];
-                cfl = cx > cflv[0] ? cx : cy > cflv[0] ? cy : fmax(cx, fmax(cy, cz));
+            int ures = c->cur;  // a user solver's step is taken before its Courant number is known
+            if (c->us) {
+                if ((rc = enqueue_dimsp3_user(c, dx, dy, dz, dt, method, mthlim, use_fwave, nullptr,
+                                              &ures, cf)))
+                    return rc;
+                cfl = split_cfl(cf, cflv[0]);
+                if (!std::isfinite(cfl)) {
+                    *info_out = -5;
+                    return set_err(CLAWGPU_ERR_NONFINITE, "non-finite wave speed (Courant number)");
+                }
+            } else {
+                if ((rc = courant3(c, method, dx, dy, dz, dt, cf))) return rc;
+                if (!std::isfinite(c->cmax) || !std::isfinite(cf[0]) || !std::isfinite(cf[1]) ||
+                    !std::isfinite(cf[2])) {
+                    *info_out = -5;
+                    return set_err(CLAWGPU_ERR_NONFINITE, "non-finite sound speed (Courant number)");
+                }
+                if (method[2] >= 0) cfl = fmax(cf[0], fmax(cf[1], cf[2]));
+                else cfl = split_cfl(cf, cflv[0]);
             }
             t = told + dt;
             if (cfl <= cflv[0]) {
                 int res = (c->cur + 1) % 3;
-                if (method[2] >= 0) rc = enqueue_step3(c, res, dx, dy, dz, dt, method, mthlim, use_fwave);
+                if (c->us) res = ures;
+                else if (method[2] >= 0) rc = enqueue_step3(c, res, dx, dy, dz, dt, method, mthlim, use_fwave);
                 else {
-                    double cf;
-                    rc = enqueue_dimsp3(c, dx, dy, dz, dt, method, mthlim, cflv, use_fwave, &res, &cf);
+                    double cfs;
+                    rc = enqueue_dimsp3(c, dx, dy, dz, dt, method, mthlim, cflv, use_fwave, &res, &cfs);
                 }
                 if (rc) return rc;
                 cflmax = fmax(cfl, cflmax);
@@ -963,6 +1098,7 @@ extern "C" int clawgpu3_time_steps(clawgpu3_ctx* c, const double* geom, double d
     if (!c || !geom || !method || !mthlim || !mthbc || !total_ms || nsteps < 1)
         return set_err(CLAWGPU_ERR_ARG, "bad argument");
     if (method[2] < -1) return set_err(CLAWGPU_ERR_ARG, "method(3) must be -1 or >= 0");
+    if (c->us) return set_err(CLAWGPU_ERR_UNSUPPORTED, "timed steps run the built-in solver; time a user solver's claw3");
     int rc = check_args3(c, method, mthlim);
     if (rc) return rc;
     if (check_bc3(mthbc) != 0) return set_err(CLAWGPU_ERR_UNSUPPORTED, "bad mthbc");
@@ -1006,6 +1142,8 @@ extern "C" int clawgpu3_time_steps(clawgpu3_ctx* c, const double* geom, double d
 extern "C" int clawgpu3_set_arith(clawgpu3_ctx* c, int mode)
 {
     if (!c || (mode != 0 && mode != 1)) return set_err(CLAWGPU_ERR_ARG, "mode must be 0 or 1");
+    if (c->us && mode != (c->us->fast ? 1 : 0))
+        return set_err(CLAWGPU_ERR_UNSUPPORTED, "a user solver's arithmetic is fixed when its code object is built");
     c->exact = mode == 0 ? 1 : 0;
     return CLAWGPU_OK;
 }

@@ -12,6 +12,7 @@
 
 #include "ctx.hpp"
 #include "split2d_kernel.hpp"
+#include "step3d_user.hpp"
 #include "usolver.hpp"
 
 extern "C" int clawgpu_solver2_load(clawgpu_solver2** out, const char* path, const char* name)
@@ -81,7 +82,95 @@ extern "C" int clawgpu_solver2_shape(const clawgpu_solver2* s, int* meqn, int* m
     return CLAWGPU_OK;
 }
 
+// 3D solvers (split step): NAME_info3, NAME_sweep3_x, NAME_sweep3_yz.
+extern "C" int clawgpu_solver3_load(clawgpu_solver3** out, const char* path, const char* name)
+{
+    if (!out || !path || !name || strlen(name) > 200) return set_err(CLAWGPU_ERR_ARG, "bad argument");
+    *out = nullptr;
+    clawgpu_solver3* s = new (std::nothrow) clawgpu_solver3();
+    if (!s) return set_err(CLAWGPU_ERR_ARG, "out of host memory");
+    hipError_t e = hipModuleLoad(&s->mod, path);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        snprintf(g_errbuf, sizeof g_errbuf, "cannot load code object %s: %s", path,
+                 hipGetErrorString(e));
+        delete s;
+        return CLAWGPU_ERR_ARG;
+    }
+    char sym[256];
+    int info[8] = {0};
+    hipDeviceptr_t dinfo = nullptr;
+    size_t bytes = 0;
+    snprintf(sym, sizeof sym, "%s_info3", name);
+    e = hipModuleGetGlobal(&dinfo, &bytes, s->mod, sym);
+    if (e == hipSuccess && bytes == sizeof info) e = hipMemcpy(info, dinfo, sizeof info, hipMemcpyDeviceToHost);
+    if (e != hipSuccess || bytes != sizeof info) {
+        (void)hipGetLastError();
+        snprintf(g_errbuf, sizeof g_errbuf, "3D solver %s not found in %s", name, path);
+        clawgpu_solver3_destroy(s);
+        return CLAWGPU_ERR_ARG;
+    }
+    if (info[0] != 1 || info[5] != (int)sizeof(s3u::Sweep3U) || info[6] != (int)sizeof(SolverParams)) {
+        clawgpu_solver3_destroy(s);
+        return set_err(CLAWGPU_ERR_ARG, "the code object was built against other kernel arguments; rebuild it");
+    }
+    if (info[1] < 1 || info[1] > kMaxWaves || info[2] < 1 || info[2] > kMaxWaves || info[3] < 0) {
+        clawgpu_solver3_destroy(s);
+        return set_err(CLAWGPU_ERR_ARG, "a 3D solver needs 1..8 equations and waves");
+    }
+    s->meqn = info[1];
+    s->mwaves = info[2];
+    s->maux = info[3];
+    s->fast = info[4];
+    const char* suffix[2] = {"sweep3_x", "sweep3_yz"};
+    hipFunction_t* fns[2] = {&s->sweep_x, &s->sweep_yz};
+    for (int k = 0; k < 2; k++) {
+        snprintf(sym, sizeof sym, "%s_%s", name, suffix[k]);
+        if (hipModuleGetFunction(fns[k], s->mod, sym) != hipSuccess) {
+            (void)hipGetLastError();
+            snprintf(g_errbuf, sizeof g_errbuf, "kernel %s not found in %s", sym, path);
+            clawgpu_solver3_destroy(s);
+            return CLAWGPU_ERR_ARG;
+        }
+    }
+    *out = s;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_solver3_destroy(clawgpu_solver3* s)
+{
+    if (!s) return CLAWGPU_OK;
+    if (s->mod) (void)hipModuleUnload(s->mod);
+    delete s;
+    return CLAWGPU_OK;
+}
+
+extern "C" int clawgpu_solver3_shape(const clawgpu_solver3* s, int* meqn, int* mwaves, int* maux,
+                                     int* arith)
+{
+    if (!s) return set_err(CLAWGPU_ERR_ARG, "null solver");
+    if (meqn) *meqn = s->meqn;
+    if (mwaves) *mwaves = s->mwaves;
+    if (maux) *maux = s->maux;
+    if (arith) *arith = s->fast ? CLAWGPU_ARITH_FAST : CLAWGPU_ARITH_EXACT;
+    return CLAWGPU_OK;
+}
+
 namespace clawgpu {
+
+int usolver3_sweep(const clawgpu_solver3* s, int dir, const s3u::Sweep3U& a, int nblocks,
+                   hipStream_t st)
+{
+    s3u::Sweep3U x = a;
+    int d = dir;
+    void* args_x[] = {&x};
+    void* args_yz[] = {&x, &d};
+    const hipError_t e =
+        hipModuleLaunchKernel(dir == 0 ? s->sweep_x : s->sweep_yz, (unsigned)nblocks, 1, 1, 256, 1,
+                              1, 0, st, dir == 0 ? args_x : args_yz, nullptr);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return (int)e;
+}
 
 int usolver_waves_per_cu(const clawgpu_solver2* s)
 {

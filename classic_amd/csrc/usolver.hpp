@@ -10,6 +10,9 @@
 
 namespace clawgpu {
 struct SweepArgs;
+namespace s3u {
+struct Sweep3U;
+}
 }
 
 struct clawgpu_solver2 {
@@ -19,7 +22,18 @@ struct clawgpu_solver2 {
     int meqn = 0, mwaves = 0, fast = 0;
 };
 
+// A 3D user solver (split step only): the x and y/z sweep kernels.
+struct clawgpu_solver3 {
+    hipModule_t mod = nullptr;
+    hipFunction_t sweep_x = nullptr, sweep_yz = nullptr;
+    int meqn = 0, mwaves = 0, maux = 0, fast = 0;
+};
+
 namespace clawgpu {
+
+// One sweep of a 3D user solver (dir 0 x, 1 y, 2 z); the HIP error code.
+int usolver3_sweep(const clawgpu_solver3* s, int dir, const s3u::Sweep3U& a, int nblocks,
+                   hipStream_t st);
 
 // Waves of the step kernel one compute unit holds (0 if unknown).
 int usolver_waves_per_cu(const clawgpu_solver2* s);

@@ -69,19 +69,54 @@ class Claw3Result:
         self.info, self.tend, self.dtv, self.cflv, self.nv = info, tend, dtv, cflv, nv
 
 
+class UserSolver3:
+    """A 3D user Riemann solver: solver `name` of the gfx950 code object at
+    path, built with CLAWGPU_EXPORT_SOLVER3 (include/clawgpu_solver.h).  Pass
+    it to Grid3D as solver; keep it open while its grids exist."""
+
+    def __init__(self, path, name):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        check(self.lib.clawgpu_solver3_load(C.byref(h), str(path).encode(), name.encode()))
+        self.h = h
+        m, w, a, r = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.clawgpu_solver3_shape(h, C.byref(m), C.byref(w), C.byref(a), C.byref(r)))
+        self.meqn, self.mwaves, self.maux, self.arith = m.value, w.value, a.value, r.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clawgpu_solver3_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 class Grid3D:
-    """Device-resident 3D grid (variable-coefficient acoustics, q = (p, u, v, w))."""
+    """Device-resident 3D grid: variable-coefficient acoustics (q = (p, u, v,
+    w)), or a UserSolver3 (solver=..., params reach it as SolverParams.user;
+    the split step only)."""
 
     def __init__(self, mx, my, mz, maux=2, mbc=2, lower=(-1.0, -1.0, -1.0),
-                 upper=(1.0, 1.0, 1.0)):
+                 upper=(1.0, 1.0, 1.0), solver=None, params=()):
         self.lib = _lib.load()
         self.mx, self.my, self.mz, self.mbc, self.maux = mx, my, mz, mbc, maux
-        self.meqn, self.mwaves = 4, 2
         d = [(upper[a] - lower[a]) / n for a, n in enumerate((mx, my, mz))]
         self.geom = np.array([lower[0], lower[1], lower[2], d[0], d[1], d[2]], dtype=np.float64)
         h = C.c_void_p()
-        check(self.lib.clawgpu3_create(C.byref(h), 4, 2, maux, mbc, mx, my, mz,
-                                       SOLVER_VC_ACOUSTICS))
+        self.solver = solver
+        if solver is not None:
+            self.meqn, self.mwaves = solver.meqn, solver.mwaves
+            p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+            check(self.lib.clawgpu3_create_user(C.byref(h), solver.h, maux, mx, my, mz, _dp(p),
+                                                p.size))
+        else:
+            self.meqn, self.mwaves = 4, 2
+            check(self.lib.clawgpu3_create(C.byref(h), 4, 2, maux, mbc, mx, my, mz,
+                                           SOLVER_VC_ACOUSTICS))
         self.h = h
 
     def close(self):
@@ -133,7 +168,7 @@ class Grid3D:
         cfl = C.c_double(0.0)
         dx, dy, dz = self.dx
         check(self.lib.clawgpu3_step3(self.h, dx, dy, dz, dt, _ip(_ivec(method, 7)),
-                                      _ip(_ivec(mthlim, 2)), int(use_fwave), C.byref(cfl)))
+                                      _ip(_ivec(mthlim, self.mwaves)), int(use_fwave), C.byref(cfl)))
         return cfl.value
 
     def dimsp3(self, dt, method=(1, 2, -1, 0, 0, 0, 2), mthlim=(3, 3), cflv=(1.0, 0.9),
@@ -143,7 +178,7 @@ class Grid3D:
         cv[: len(cflv)] = cflv
         dx, dy, dz = self.dx
         check(self.lib.clawgpu3_dimsp3(self.h, dx, dy, dz, dt, _ip(_ivec(method, 7)),
-                                       _ip(_ivec(mthlim, 2)), _dp(cv), int(use_fwave),
+                                       _ip(_ivec(mthlim, self.mwaves)), _dp(cv), int(use_fwave),
                                        C.byref(cfl)))
         return cfl.value
 
@@ -161,7 +196,8 @@ class Grid3D:
         nv = np.array([nmax, 0], dtype=np.int32)
         info = C.c_int(0)
         args = (_dp(self.geom), tstart, C.byref(te), _dp(dtv_a), _dp(cflv_a), _ip(nv),
-                _ip(_ivec(method, 7)), _ip(_ivec(mthlim, 2)), _ip(_ivec(mthbc, 6)), int(use_fwave))
+                _ip(_ivec(method, 7)), _ip(_ivec(mthlim, self.mwaves)), _ip(_ivec(mthbc, 6)),
+                int(use_fwave))
         if dev_hooks is not None:
             check(self.lib.clawgpu3_claw3_dev_hooks(self.h, *args, dev_hooks.h, C.byref(info)))
         elif bc3 is None and b4step3 is None and src3 is None:
@@ -228,7 +264,7 @@ class Grid3D:
         ker = C.c_float(0.0)
         cfl = C.c_double(0.0)
         check(self.lib.clawgpu3_time_steps(self.h, _dp(self.geom), dt, _ip(_ivec(method, 7)),
-                                           _ip(_ivec(mthlim, 2)), _ip(_ivec(mthbc, 6)),
+                                           _ip(_ivec(mthlim, self.mwaves)), _ip(_ivec(mthbc, 6)),
                                            int(use_fwave), nsteps, C.byref(tot), C.byref(ker),
                                            C.byref(cfl)))
         return tot.value, ker.value, cfl.value

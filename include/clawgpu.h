@@ -482,6 +482,31 @@ int clawgpu3_claw3_dev_hooks(clawgpu3_ctx* ctx, const double* geom, double tstar
                              const int* mthlim, const int* mthbc, int use_fwave,
                              const clawgpu_dev_hooks* hooks, int* info);
 
+/* User Riemann solvers in 3D: the rpn3 a caller links into claw3
+ * (claw3.f:292-330, called from flux3.f90:205 in the sweeps of step3ds.f90
+ * that dimsp3.f drives).  A solver is a policy struct compiled by
+ * CLAWGPU_EXPORT_SOLVER3 (include/clawgpu_solver.h) into a gfx950 code
+ * object that exports solver `name`; loading checks the kernel-argument
+ * layout the object was built against.  A context of clawgpu3_create_user
+ * runs the dimensionally split step (method(3) = -1: clawgpu3_dimsp3,
+ * clawgpu3_claw3 and its hook variants) with the user's sweeps over meqn
+ * fields, mwaves waves and the solver's aux planes, in the arithmetic the
+ * object was built with; it has no transverse terms (rpt3, rptt3), no
+ * capacity function and no piston boundary.  claw3 takes the step before
+ * its Courant number is known (the sweeps form it) and uses the result
+ * only if the step is accepted.  Keep the solver loaded while its
+ * contexts exist. */
+typedef struct clawgpu_solver3 clawgpu_solver3;
+int clawgpu_solver3_load(clawgpu_solver3** solver, const char* path, const char* name);
+int clawgpu_solver3_destroy(clawgpu_solver3* solver);
+/* meqn, mwaves, maux (aux planes it reads) and arith of a loaded solver */
+int clawgpu_solver3_shape(const clawgpu_solver3* solver, int* meqn, int* mwaves, int* maux,
+                          int* arith);
+/* clawgpu3_create for a user solver (mbc = 2; maux >= the solver's):
+ * params[0..nparams-1] (nparams <= 16) reach it as SolverParams.user. */
+int clawgpu3_create_user(clawgpu3_ctx** ctx, const clawgpu_solver3* solver, int maux, int mx,
+                         int my, int mz, const double* params, int nparams);
+
 /* Cells per segment of the y/z split sweeps (tuning; default 64). */
 int clawgpu3_set_segment(clawgpu3_ctx* ctx, int cells);
 

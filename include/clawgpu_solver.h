@@ -49,6 +49,7 @@
 #include "clawgpu.h"
 #include "step2d_kernel.hpp"
 #include "split2d_kernel.hpp"
+#include "step3d_user.hpp"
 
 /* version of the kernel-argument layout a code object was built against */
 #define CLAWGPU_SOLVER2_ABI 1
@@ -95,6 +96,45 @@
                                             double* amdq, double* apdq, double* bm, double* bp) \
     {                                                                                           \
         clawgpu::solver_probe_body<S>(n, ixy, P, ql, qr, asdq, wave, s, amdq, apdq, bm, bp);    \
+    }
+
+/* 3D solvers (rpn3 of src/3d/claw3.f:292-335) for the dimensionally split
+ * step (method(3) = -1: dimsp3/step3ds, which has no transverse terms, so
+ * no rpt3/rptt3):
+ *
+ *   struct MySolver3 {
+ *     using Math = clawgpu::ExactMath;  // or FastMath
+ *     static constexpr int M = ..., MW = ...;  // meqn, mwaves (1..8)
+ *     static constexpr int MAUX = ...;         // aux components read (0..8)
+ *     struct Cell { double ...; };             // what rpn needs of one cell
+ *     __device__ static Cell cell(const clawgpu::SolverParams& P,
+ *                                 const double* q, const double* aux);
+ *     template <int MU>                        // normal momentum: 1 x, 2 y, 3 z
+ *     __device__ static void rpn(const clawgpu::SolverParams& P, const Cell& L,
+ *                                const Cell& R, double w[MW][M], double s[MW],
+ *                                double amdq[M], double apdq[M]);
+ *   };
+ *
+ * cell() gets q(1..M) and aux(1..MAUX) of one cell; rpn solves the interface
+ * between L (cell i-1) and R (cell i) as the reference rpn3 does. */
+#define CLAWGPU_SOLVER3_ABI 1
+
+/* Kernels of 3D solver NAME: NAME_info3, NAME_sweep3_x, NAME_sweep3_yz. */
+#define CLAWGPU_EXPORT_SOLVER3(NAME, S)                                                          \
+    static_assert(S::M >= 1 && S::M <= clawgpu::kMaxWaves, "meqn out of range");                \
+    static_assert(S::MW >= 1 && S::MW <= clawgpu::kMaxWaves, "mwaves out of range");           \
+    static_assert(S::MAUX >= 0 && S::MAUX <= 8, "MAUX out of range");                          \
+    extern "C" __device__ __attribute__((used)) int NAME##_info3[8] = {                         \
+        CLAWGPU_SOLVER3_ABI, S::M, S::MW, S::MAUX, S::Math::fast ? 1 : 0,                        \
+        (int)sizeof(clawgpu::s3u::Sweep3U), (int)sizeof(clawgpu::SolverParams), 0};            \
+    extern "C" __global__ __launch_bounds__(256) void NAME##_sweep3_x(clawgpu::s3u::Sweep3U a)  \
+    {                                                                                           \
+        clawgpu::s3u::sweep3u_x_body<S>(a);                                                     \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void NAME##_sweep3_yz(clawgpu::s3u::Sweep3U a, \
+                                                                       int dir)                 \
+    {                                                                                           \
+        clawgpu::s3u::sweep3u_yz_body<S>(a, dir);                                               \
     }
 
 #endif

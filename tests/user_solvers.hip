@@ -7,6 +7,11 @@
 //   user_advection  q_t + u q_x + v q_y = 0, params (u, v): the
 //                   variable-coefficient advection solver of the reference's
 //                   annulus example with constant edge velocities.
+//   user_vc_acoustics3  3D variable-coefficient acoustics, aux (Z, c): the
+//                   algorithm of the built-in 3D solver (rpn3 of the
+//                   reference's heterogeneous acoustics example), so dimsp3
+//                   and claw3 must give the built-in split results.
+//   user_advection3 q_t + u q_x + v q_y + w q_z = 0, params (u, v, w).
 #ifdef USER_SOLVERS_FAST
 #pragma clang fp contract(fast)
 #endif
@@ -127,3 +132,72 @@ struct UserAdvection {
 
 CLAWGPU_EXPORT_SOLVER2(user_acoustics, UserAcoustics)
 CLAWGPU_EXPORT_SOLVER2(user_advection, UserAdvection)
+
+// 3D split-step solvers (CLAWGPU_EXPORT_SOLVER3): rpn<MU> solves the
+// Riemann problem normal to direction MU (1 x, 2 y, 3 z) between cells L and R.
+struct UserVcAcoustics3 {
+    using Math = Mth;
+    static constexpr int M = 4, MW = 2, MAUX = 2;
+    struct Cell {
+        double q[4];
+        double Z, c;
+    };
+
+    __device__ static inline Cell cell(const SolverParams&, const double* q, const double* aux)
+    {
+        return Cell{{q[0], q[1], q[2], q[3]}, aux[0], aux[1]};
+    }
+
+    // the jump in (p, normal velocity) splits into (-Z_L, 1) a1 at -c_L and
+    // (Z_R, 1) a2 at +c_R
+    template <int MU>
+    __device__ static inline void rpn(const SolverParams&, const Cell& L, const Cell& R,
+                                      double w[MW][M], double s[MW], double am[M], double ap[M])
+    {
+        const double d1 = R.q[0] - L.q[0], d2 = R.q[MU] - L.q[MU];
+        const double den = L.Z + R.Z;
+        const double a1 = Mth::div(-d1 + R.Z * d2, den);
+        const double a2 = Mth::div(d1 + L.Z * d2, den);
+        for (int m = 0; m < M; m++) {
+            w[0][m] = 0.0;
+            w[1][m] = 0.0;
+        }
+        w[0][0] = -a1 * L.Z;
+        w[0][MU] = a1;
+        w[1][0] = a2 * R.Z;
+        w[1][MU] = a2;
+        s[0] = -L.c;
+        s[1] = R.c;
+        for (int m = 0; m < M; m++) {
+            am[m] = s[0] * w[0][m];
+            ap[m] = s[1] * w[1][m];
+        }
+    }
+};
+
+struct UserAdvection3 {
+    using Math = Mth;
+    static constexpr int M = 1, MW = 1, MAUX = 0;
+    struct Cell {
+        double q[1];
+    };
+
+    __device__ static inline Cell cell(const SolverParams&, const double* q, const double*)
+    {
+        return Cell{{q[0]}};
+    }
+
+    template <int MU>
+    __device__ static inline void rpn(const SolverParams& P, const Cell& L, const Cell& R,
+                                      double w[MW][M], double s[MW], double am[M], double ap[M])
+    {
+        const double u = P.user[MU - 1];
+        w[0][0] = R.q[0] - L.q[0];
+        s[0] = u;
+        am[0] = fmin(u, 0.0) * w[0][0];
+        ap[0] = fmax(u, 0.0) * w[0][0];
+    }
+};
+
+CLAWGPU_EXPORT_SOLVER3(user_vc_acoustics3, UserVcAcoustics3)
+CLAWGPU_EXPORT_SOLVER3(user_advection3, UserAdvection3)
