@@ -196,6 +196,18 @@ __global__ void table_check_kernel(const double* __restrict__ Z, const double* _
     if (__any(diff) && (threadIdx.x & 63) == 0) atomicOr(&out[0], 1ull);
 }
 
+// max of n non-negative per-wave Courant numbers, as raw bits in *out
+// (one atomic per wave of this kernel)
+__global__ void wave_max_kernel(const double* __restrict__ part, long n, unsigned long long* out)
+{
+    double m = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x)
+        m = fmax(m, part[e]);
+    for (int d = 32; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
 int check_method3(const int* method)
 {
     int info = 0;
@@ -480,6 +492,8 @@ int enqueue_dimsp3_user(clawgpu3_ctx* c, double dx, double dy, double dz, double
 {
     const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
     unsigned long long* bits = reinterpret_cast<unsigned long long*>(c->staging);
+    double* part = c->staging + 8;  // per-wave Courant numbers of the running sweep
+    const long room = (long)(c->meqn > c->maux ? c->meqn : (c->maux > 0 ? c->maux : 1)) * c->L.fs - 8;
     HIPCHK(c, hipMemsetAsync(bits, 0, 3 * sizeof(unsigned long long), c->stream));
     const int from[3] = {A, B, C}, to[3] = {B, C, B};
     const double dd[3] = {dt / dx, dt / dy, dt / dz};
@@ -497,13 +511,18 @@ int enqueue_dimsp3_user(clawgpu3_ctx* c, double dx, double dy, double dz, double
         const int n = dir == 1 ? c->my : c->mz;
         a.nsegs = (n + c->seg - 1) / c->seg;
         a.nstrips = dir == 0 ? (c->mx + 2 + kOut - 1) / kOut : (c->L.nx + 63) / 64;
-        a.cfl_bits = bits + dir;
+        a.cfl_part = part;
         a.P = c->P;
         const long nwaves =
             dir == 0 ? (long)a.nstrips * (c->L.ny) * (c->L.nz)
                      : (long)a.nstrips * (dir == 1 ? c->L.nz : c->L.ny) * a.nsegs;
+        if (nwaves > room) return set_err(CLAWGPU_ERR_ARG, "grid too large for the Courant partials");
         if (usolver3_sweep(c->us, dir, a, (int)((nwaves + 3) / 4), c->stream))
             return set_err(CLAWGPU_ERR_HIP, "user sweep kernel launch failed");
+        const long nb = (nwaves + 255) / 256;
+        hipLaunchKernelGGL(wave_max_kernel, dim3((unsigned)(nb < 512 ? nb : 512)), dim3(256), 0,
+                           c->stream, part, nwaves, bits + dir);
+        HIPCHK(c, hipGetLastError());
         *res = to[dir];
         if (stop && dir < 2) {
             unsigned long long h;

@@ -13,7 +13,10 @@
 // q + ((0 - d apdq(i)) - d amdq(i+1)) - d (cqxx(i+1) - cqxx(i))
 // (step3ds.f90), in IEEE arithmetic without contraction when the solver
 // uses ExactMath.  Each sweep also forms its Courant number d max|s| over
-// the interfaces 1..n+1 of the slices step3ds visits.
+// the interfaces 1..n+1 of the slices step3ds visits: every wave stores its
+// maximum in cfl_part[wave] and the library reduces those afterwards (one
+// atomic per wave on one address would serialise at about 10 ns each, and
+// the x sweep runs a wave per strip of every x-line).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,6 +29,9 @@
 namespace clawgpu {
 namespace s3u {
 
+// version of the sweep-argument contract, checked when a code object loads
+constexpr int kAbi = 2;
+
 struct Sweep3U {
     const double* __restrict__ q;
     double* __restrict__ qn;
@@ -35,7 +41,7 @@ struct Sweep3U {
     int m2, fwave;
     int lim[kMaxWaves];
     int nstrips, seg, nsegs;
-    unsigned long long* cfl_bits;  // the sweep's Courant number, as raw bits
+    double* cfl_part;  // per-wave Courant numbers, indexed by the wave number
     SolverParams P;
 };
 
@@ -107,6 +113,15 @@ __device__ inline void cqxx_u(const IfaceU<S>& f, double d, int fwave, double (&
     }
 }
 
+// the wave's maximum of a non-negative value, stored by lane 0 (every lane
+// of the wave must call it)
+__device__ inline void store_wave_max(double v, double* part, long w)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d));
+    if ((threadIdx.x & 63) == 0) part[w] = v;
+}
+
 template <class S>
 __device__ inline double courant_u(const IfaceU<S>& f, double d)
 {
@@ -147,7 +162,7 @@ __device__ __forceinline__ void sweep3u_x_body(const Sweep3U& a)
     S::template rpn<1>(a.P, Lc, R, f.w, f.s, f.am, f.ap);
     const bool slice = j >= 0 && j <= L.my + 1 && k >= 0 && k <= L.mz + 1;
     const double cfl = (l >= 1 && i >= 1 && i <= L.mx + 1 && slice) ? courant_u<S>(f, a.d) : 0.0;
-    publish_cfl(cfl, a.cfl_bits);
+    store_wave_max(cfl, a.cfl_part, w);
     double cq[M];
 #pragma unroll
     for (int m = 0; m < M; m++) cq[m] = 0.0;
@@ -265,7 +280,7 @@ __device__ __forceinline__ void sweep3u_yz_body(const Sweep3U& a, int dir)
         }
         c1 = c2;
     }
-    publish_cfl(cfl, a.cfl_bits);
+    store_wave_max(cfl, a.cfl_part, w);
 }
 
 }  // namespace s3u

@@ -110,7 +110,8 @@ extern "C" int clawgpu_solver3_load(clawgpu_solver3** out, const char* path, con
         clawgpu_solver3_destroy(s);
         return CLAWGPU_ERR_ARG;
     }
-    if (info[0] != 1 || info[5] != (int)sizeof(s3u::Sweep3U) || info[6] != (int)sizeof(SolverParams)) {
+    if (info[0] != s3u::kAbi || info[5] != (int)sizeof(s3u::Sweep3U) ||
+        info[6] != (int)sizeof(SolverParams)) {
         clawgpu_solver3_destroy(s);
         return set_err(CLAWGPU_ERR_ARG, "the code object was built against other kernel arguments; rebuild it");
     }

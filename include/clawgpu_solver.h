@@ -117,7 +117,7 @@
  *
  * cell() gets q(1..M) and aux(1..MAUX) of one cell; rpn solves the interface
  * between L (cell i-1) and R (cell i) as the reference rpn3 does. */
-#define CLAWGPU_SOLVER3_ABI 1
+#define CLAWGPU_SOLVER3_ABI clawgpu::s3u::kAbi
 
 /* Kernels of 3D solver NAME: NAME_info3, NAME_sweep3_x, NAME_sweep3_yz. */
 #define CLAWGPU_EXPORT_SOLVER3(NAME, S)                                                          \
