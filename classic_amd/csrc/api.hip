@@ -445,6 +445,7 @@ static int enqueue_step(clawgpu_ctx* c, double dx, double dy, double dt, const i
         g.fwave = use_fwave ? 1 : 0;
         g.lim0 = mthlim[0];
         g.cfl_bits = c->dcfl;
+        g.cfl_part = c->staging;  // free during a step
         HIPCHK(c, (hipError_t)launch_step2_general(g, c->stream));
         return CLAWGPU_OK;
     }
@@ -513,6 +514,7 @@ static int enqueue_sweep(clawgpu_ctx* c, int dir, int from, int to, double dtd, 
         g.fwave = use_fwave ? 1 : 0;
         g.lim0 = mthlim[0];
         g.cfl_bits = c->dcfl + slot;
+        g.cfl_part = c->staging;  // free during a step
         HIPCHK(c, (hipError_t)launch_split_sweep_general(dir, g, c->stream));
         return CLAWGPU_OK;
     }

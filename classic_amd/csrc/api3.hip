@@ -200,12 +200,7 @@ __global__ void table_check_kernel(const double* __restrict__ Z, const double* _
 // (one atomic per wave of this kernel)
 __global__ void wave_max_kernel(const double* __restrict__ part, long n, unsigned long long* out)
 {
-    double m = 0.0;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (long)gridDim.x * blockDim.x)
-        m = fmax(m, part[e]);
-    for (int d = 32; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    clawgpu::wave_max_reduce(part, n, out);
 }
 
 int check_method3(const int* method)

@@ -68,6 +68,31 @@ __device__ inline T lane_up_t(const T& v)
     return r;
 }
 
+// The wave's maximum of a non-negative value, stored by lane 0 in part[w]
+// (every lane of the wave must call it).  Kernels with a wave per few cells
+// collect their Courant numbers this way and reduce them afterwards with
+// wave_max_reduce: one atomic per wave on one address serialises at about
+// 11 ns each.
+__device__ inline void store_wave_max(double v, double* part, long w)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d));
+    if ((threadIdx.x & 63) == 0) part[w] = v;
+}
+
+// Body of a reduction kernel: the max of part[0..n-1] (non-negative) into
+// *out as raw bits, one atomic per wave of that kernel.
+__device__ inline void wave_max_reduce(const double* part, long n, unsigned long long* out)
+{
+    double m = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x)
+        m = fmax(m, part[e]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
 // Max of a non-negative double over the wave, then one atomic per wave.
 __device__ inline void publish_cfl(double cfl, unsigned long long* bits)
 {

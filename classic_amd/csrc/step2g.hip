@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void records_kernel(GenArgs a)
     Pos ps;
     eval_position(a.q, a.aux, a.afs, vel, [&](int pp) { return off(pp, sc); }, pc, n, live, dtb,
                   a.capa, a.order, a.lim0, a.fwave, ps);
-    publish_cfl(ps.cfl, a.cfl_bits);
+    store_wave_max(ps.cfl, a.cfl_part, ((long)blockIdx.y * gridDim.x + blockIdx.x) * 4 + threadIdx.y);
     const double* dtd = ps.dtd;
     const double* am = ps.am;
     const double* ap = ps.ap;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void split_sweep_kernel(SplitArgs a)
     Pos o;
     eval_position(a.qin, a.aux, a.afs, vel, off, pc, n, live, a.dtd, a.capa, a.order, a.lim0,
                   a.fwave, o);
-    publish_cfl(o.cfl, a.cfl_bits);
+    store_wave_max(o.cfl, a.cfl_part, ((long)blockIdx.y * gridDim.x + blockIdx.x) * 4 + threadIdx.y);
     if (!in_grid) return;
     if (!live) {
         const long c = off(p);
@@ -228,15 +228,33 @@ __global__ __launch_bounds__(256) void split_sweep_kernel(SplitArgs a)
     a.qout[c] = v;
 }
 
+__global__ void cfl_reduce_kernel(const double* part, long n, unsigned long long* out)
+{
+    wave_max_reduce(part, n, out);
+}
+
+// the per-wave Courant numbers of a launch of grid g (blocks of 64 x 4)
+// into *out
+void reduce_cfl(const double* part, dim3 g, unsigned long long* out, hipStream_t s)
+{
+    const long n = (long)g.x * g.y * 4;
+    const long nb = (n + 255) / 256;
+    hipLaunchKernelGGL(cfl_reduce_kernel, dim3((unsigned)(nb < 512 ? nb : 512)), dim3(256), 0, s,
+                       part, n, out);
+}
+
 }  // namespace g2
 
+// The kernels' blocks are 64 x 4 threads, one wave per row of the block.
 int launch_step2_general(const g2::GenArgs& a, hipStream_t s)
 {
     const dim3 blk(64, 4);
     const dim3 gx((a.L.mx + 63) / 64, (a.L.my + 2 + 3) / 4);
     const dim3 gy((a.L.my + 63) / 64, (a.L.mx + 2 + 3) / 4);
     hipLaunchKernelGGL(g2::records_kernel<1>, gx, blk, 0, s, a);
+    g2::reduce_cfl(a.cfl_part, gx, a.cfl_bits, s);
     hipLaunchKernelGGL(g2::records_kernel<2>, gy, blk, 0, s, a);
+    g2::reduce_cfl(a.cfl_part, gy, a.cfl_bits, s);
     const long n = (long)(a.L.mx + 2 * a.L.mbc) * (a.L.my + 2 * a.L.mbc);
     hipLaunchKernelGGL(g2::combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
     return (int)hipGetLastError();
@@ -251,6 +269,7 @@ int launch_split_sweep_general(int dir, const g2::SplitArgs& a, hipStream_t s)
         hipLaunchKernelGGL(g2::split_sweep_kernel<1>, g, blk, 0, s, a);
     else
         hipLaunchKernelGGL(g2::split_sweep_kernel<2>, g, blk, 0, s, a);
+    g2::reduce_cfl(a.cfl_part, g, a.cfl_bits, s);
     return (int)hipGetLastError();
 }
 

@@ -19,6 +19,7 @@ struct GenArgs {
     double dtdx, dtdy;
     int capa, order, trans, fwave, lim0;
     unsigned long long* cfl_bits;
+    double* cfl_part;  // per-wave Courant numbers, reduced into cfl_bits after each kernel
 };
 
 // one sweep of the dimensionally split step (step2ds)
@@ -31,6 +32,7 @@ struct SplitArgs {
     double dtd;  // dt/dx (x sweep) or dt/dy (y sweep) of this sweep
     int capa, order, fwave, lim0;
     unsigned long long* cfl_bits;
+    double* cfl_part;  // per-wave Courant numbers, reduced into cfl_bits after each kernel
 };
 
 }  // namespace g2

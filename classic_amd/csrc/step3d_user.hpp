@@ -113,15 +113,6 @@ __device__ inline void cqxx_u(const IfaceU<S>& f, double d, int fwave, double (&
     }
 }
 
-// the wave's maximum of a non-negative value, stored by lane 0 (every lane
-// of the wave must call it)
-__device__ inline void store_wave_max(double v, double* part, long w)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d));
-    if ((threadIdx.x & 63) == 0) part[w] = v;
-}
-
 template <class S>
 __device__ inline double courant_u(const IfaceU<S>& f, double d)
 {
