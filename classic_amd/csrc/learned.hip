@@ -652,8 +652,9 @@ __global__ void dphi_kernel(DphiArgs a)
 {
     const Layout2 L = a.L;
     const int mx = L.mx, my = L.my, M = a.meqn, MW = a.mwaves;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * a.ntab) return;
+    const long t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = t0 < 2 * a.ntab;  // every lane stays for the loss reduction below
+    const long t = on ? t0 : 0;
     const int dir = (int)(t / a.ntab);
     const long slot = t % a.ntab;
     const int i = (int)(slot % (mx + 4)) - 1, J = (int)(slot / (mx + 4)) - 1;
@@ -664,7 +665,7 @@ __global__ void dphi_kernel(DphiArgs a)
         long o = m * L.fs + L.off(ii, jj);
         return a.qnew[o] - a.target[o];
     };
-    for (int mw = 0; mw < MW; mw++) {
+    for (int mw = 0; on && mw < MW; mw++) {
         const long w = ((long)dir * a.ntab + slot) * MW + mw;
         double d = 0.0;
         if (a.l1) {
@@ -678,15 +679,20 @@ __global__ void dphi_kernel(DphiArgs a)
             a.dphi[w] = (float)(2.0 * a.scale * d);
         }
     }
-    // loss: one thread per interior cell (dir 0 slots with i, J interior)
-    if (dir == 0 && i >= 1 && i <= mx && J >= 1 && J <= my) {
-        double l = 0.0;
+    // loss: one term per interior cell (dir 0 slots with i, J interior),
+    // summed over the wave and added with one atomic per wave (one per cell
+    // on one address would serialise at about 11 ns each)
+    double l = 0.0;
+    if (on && dir == 0 && i >= 1 && i <= mx && J >= 1 && J <= my) {
         for (int m = 0; m < M; m++) {
             double e = res(i, J, m);
             l += a.l1 ? fabs(e) : e * e;
         }
-        atomicAdd(a.lossacc, l * a.scale);
+        l = l * a.scale;
     }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) l += __shfl_xor(l, d);
+    if ((threadIdx.x & 63) == 0 && l != 0.0) atomicAdd(a.lossacc, l);
 }
 
 }  // namespace clawgpu
