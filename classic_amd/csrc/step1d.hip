@@ -42,6 +42,7 @@ struct Args1 {
     int order, fwave, lim0, lim1;
     P1 P;
     unsigned long long* cfl_bits;
+    double* cfl_part;  // per-strip Courant numbers, reduced into cfl_bits afterwards
 };
 
 // rp1 of one interface: q and aux of the left (L) and right (R) cell
@@ -109,6 +110,11 @@ struct Rp1<CLAWGPU_SOLVER_ADVECTION> {
     }
 };
 
+__global__ void cfl1_reduce_kernel(const double* part, long n, unsigned long long* out)
+{
+    clawgpu::wave_max_reduce(part, n, out);
+}
+
 template <int SOLVER>
 __global__ __launch_bounds__(256) void step1_kernel(Args1 a)
 {
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(256) void step1_kernel(Args1 a)
             cfl = fmax(cfl, (t == t) ? t : __builtin_huge_val());
         }
     }
-    clawgpu::publish_cfl(cfl, a.cfl_bits);
+    clawgpu::store_wave_max(cfl, a.cfl_part, strip);
 
     // second-order flux f(i) at this interface from the limited waves
     double f[M];
@@ -228,6 +234,7 @@ struct clawgpu1_ctx {
     int cur;
     double* daux;
     unsigned long long* dcfl;
+    double* dpart;  // per-strip Courant numbers of a step
     hipStream_t stream;
     int last_hip_error;
     std::vector<double> haux;  // host copy of aux for the user routines
@@ -273,8 +280,9 @@ int enqueue_step1(clawgpu1_ctx* c, double dx, double dt, const int* method, cons
     a.lim1 = c->mwaves > 1 ? mthlim[1] : 0;
     a.P = c->P;
     a.cfl_bits = c->dcfl;
+    a.cfl_part = c->dpart;
     HIPCHK(c, hipMemsetAsync(c->dcfl, 0, sizeof(unsigned long long), c->stream));
-    const int nstrips = (c->mx + 2) / 60 + 1;
+    const int nstrips = (c->mx + 2) / 60 + 1;  // strips 0..nstrips-1 store their Courant numbers
     const unsigned nb = (unsigned)((nstrips + 3) / 4);
     switch (c->solver) {
     case CLAWGPU_SOLVER_ACOUSTICS:
@@ -287,6 +295,9 @@ int enqueue_step1(clawgpu1_ctx* c, double dx, double dt, const int* method, cons
         hipLaunchKernelGGL(step1_kernel<CLAWGPU_SOLVER_ADVECTION>, dim3(nb), dim3(256), 0, c->stream, a);
         break;
     }
+    const long rb = (nstrips + 255) / 256;
+    hipLaunchKernelGGL(cfl1_reduce_kernel, dim3((unsigned)(rb < 512 ? rb : 512)), dim3(256), 0,
+                       c->stream, c->dpart, (long)nstrips, c->dcfl);
     HIPCHK(c, hipGetLastError());
     return CLAWGPU_OK;
 }
@@ -348,12 +359,14 @@ extern "C" int clawgpu1_create(clawgpu1_ctx** out, int meqn, int mwaves, int mau
     if (e == hipSuccess) e = hipMalloc((void**)&c->dq[1], n * meqn * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&c->daux, n * (maux > 0 ? maux : 1) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&c->dcfl, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->dpart, (size_t)((mx + 2) / 60 + 1) * sizeof(double));
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->dq[1], 0, n * meqn * sizeof(double), c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         snprintf(g_errbuf, sizeof g_errbuf, "device allocation failed: %s", hipGetErrorString(e));
         (void)hipFree(c->dq[0]); (void)hipFree(c->dq[1]); (void)hipFree(c->daux); (void)hipFree(c->dcfl);
+        (void)hipFree(c->dpart);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return CLAWGPU_ERR_HIP;
@@ -367,6 +380,7 @@ extern "C" int clawgpu1_destroy(clawgpu1_ctx* c)
     if (!c) return CLAWGPU_OK;
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->dq[0]); (void)hipFree(c->dq[1]); (void)hipFree(c->daux); (void)hipFree(c->dcfl);
+    (void)hipFree(c->dpart);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return CLAWGPU_OK;
