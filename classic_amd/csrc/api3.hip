@@ -798,9 +798,9 @@ extern "C" int clawgpu3_dimsp3(clawgpu3_ctx* c, double dx, double dy, double dz,
     if (rc) return rc;
     int res = c->cur;
     if (c->us) {
-        double cf[3];
+        double cf[3] = {0.0, 0.0, 0.0};
         rc = enqueue_dimsp3_user(c, dx, dy, dz, dt, method, mthlim, use_fwave, cflv, &res, cf);
-        *cfl = split_cfl(cf, cflv[0]);
+        if (rc == CLAWGPU_OK) *cfl = split_cfl(cf, cflv[0]);
     } else {
         rc = enqueue_dimsp3(c, dx, dy, dz, dt, method, mthlim, cflv, use_fwave, &res, cfl);
     }
@@ -963,7 +963,7 @@ static int claw3_loop(clawgpu3_ctx* c, const double* geom, double tstart, double
                         hq.data(), c->maux, haux, told, dt / 2.0);
                 if ((rc = push())) return rc;
             }
-            double cf[3];
+            double cf[3] = {0.0, 0.0, 0.0};
             int ures = c->cur;  // a user solver's step is taken before its Courant number is known
             if (c->us) {
                 if ((rc = enqueue_dimsp3_user(c, dx, dy, dz, dt, method, mthlim, use_fwave, nullptr,
