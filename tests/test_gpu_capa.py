@@ -126,3 +126,35 @@ def test_annulus_example_split_matches_reference_run(m3):
     np.testing.assert_array_equal(r["nv"], gold["nv_" + key])
     np.testing.assert_array_equal(r["dtv"], gold["dtv_" + key])
     np.testing.assert_array_equal(qk[2:-2, 2:-2], gold["q_" + key])
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_general_path_large_grid_courant_reduction(split):
+    """1100 x 700 cells: the records and split-sweep kernels run about 12 k
+    waves, whose Courant numbers are stored per wave and reduced by several
+    blocks of a second kernel; the step and its Courant number still equal
+    the oracle's bit for bit.  The fastest edge velocity sits in one cell
+    near the far corner, so a reduction that dropped any block would miss
+    it."""
+    mx, my = 1100, 700
+    q, aux = _random(mx, my, 21, 0)
+    aux[my - 3, mx - 7, 0] = 3.0  # the largest |u| of the grid, in the last blocks
+    method = (1, 2, -1 if split else 2, 0, 0, 0, 3)
+    p = O.Problem(1, 1, mx, my, O.SOLVER_VC_ADVECTION, O.solver_params(), maux=3, method=method,
+                  mthlim=[4], mthbc=(1, 1, 2, 2))
+    q = O.bc2(p, q)
+    dx, dy, dt = 1.0 / mx, 1.0 / my, 0.2 / my
+    if split:
+        ref, cfl_ref = O.dimsp2(p, q, dx, dy, dt, [1.0, 0.9], aux=aux)
+    else:
+        ref, cfl_ref = O.step2(p, q, dx, dy, dt, aux=aux)
+    with gpu.Grid2D(SOLVER_VC_ADVECTION, mx, my, [], maux=3) as g:
+        g.set_aux(aux)
+        g.set_q(q)
+        if split:
+            cfl = g.dimsp2(dx, dy, dt, method, [4], (1.0, 0.9))
+        else:
+            cfl = g.step2(dx, dy, dt, method, [4])
+        out = g.get_q()
+    assert cfl == cfl_ref == dt / dx * 3.0
+    np.testing.assert_array_equal(out[2:-2, 2:-2], ref[2:-2, 2:-2])
