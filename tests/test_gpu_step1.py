@@ -108,3 +108,26 @@ def test_examples_frames_and_regression_sums():
         np.testing.assert_array_equal(qq[2:-2], g2["q"][k][2:-2])
         np.testing.assert_allclose(qq[2:-2].sum(axis=0), g2["regression_sums"][k], rtol=1e-14,
                                    atol=1e-14)
+
+
+def test_step1_long_grid_courant_reduction():
+    """40,000 cells: 667 strips, whose Courant numbers are stored per strip
+    and reduced by several blocks; the fastest sound speed sits in one cell
+    near the end, and the step and its Courant number equal the oracle's
+    bit for bit."""
+    mx = 40000
+    rng = np.random.default_rng(9)
+    p = O1.Problem1(mx, O1.SOLVER_VC_ACOUSTICS, O1.params1(1.3, 2.9), meqn=2, mwaves=2, maux=2,
+                    method=(1, 2, 0, 0, 0, 0, 0), mthlim=(4, 4))
+    q = rng.standard_normal((mx + 4, 2))
+    aux = np.stack([rng.uniform(0.5, 3, mx + 4), rng.uniform(0.5, 2, mx + 4)], -1)
+    aux[mx - 5, 1] = 4.0  # the largest sound speed, in the last strips
+    dx, dt = 2.0 / mx, 0.4 / mx
+    ref, cfl_ref = O1.step1(p, q, dx, dt, aux=aux)
+    with Grid1D(SOLVER_VC_ACOUSTICS, mx, _params(SOLVER_VC_ACOUSTICS), maux=2) as g:
+        g.set_aux(aux)
+        g.set_q(q)
+        cfl = g.step1(dx, dt, p.method, (4, 4))
+        out = g.get_q()
+    assert cfl == cfl_ref == dt / dx * 4.0
+    np.testing.assert_array_equal(out[2:-2], ref[2:-2])
