@@ -512,3 +512,21 @@ def test_user_solver3_refusals(uvc3, uadv3):
     with pytest.raises(ClawGpuError) as e:
         Grid3D(8, 8, 8, maux=2, solver=uvc3, params=np.zeros(17))
     assert e.value.code == ERR_ARG
+
+
+def test_user_vc_acoustics3_courant_reduction_planted_maximum(uvc3):
+    """130 x 40 x 30 cells (about 4,500 x-sweep waves, their Courant numbers
+    reduced by several blocks) with the fastest sound speed in one cell
+    near the far corner: dimsp3's Courant number and state equal the
+    oracle's bit for bit."""
+    rng = np.random.default_rng(12)
+    p = O3.Problem3(130, 40, 30, mthlim=(3, 3), method=(1, 2, -1, 0, 0, 0, 0))
+    q, aux = random_layered(p, rng)
+    aux[30, 39, 125, 1] = 2.5  # (k, j, i) = (29, 38, 124): the largest c of the grid
+    dt = 0.3 * p.geom.dx / 2.5
+    ref, cfl_ref = O3.dimsp3(p, q, aux, dt)
+    with _ugrid3(uvc3, p, aux, q) as g:
+        cfl = g.dimsp3(dt, method=p.method, mthlim=(3, 3))
+        out = g.get_q()
+    assert cfl == cfl_ref == dt / p.geom.dx * 2.5
+    np.testing.assert_array_equal(out, ref)
